@@ -1,0 +1,331 @@
+"""catabrafft -- CatabraFFT.jl's user API on the MI355X HIP library.
+
+Same names, argument meaning and error behaviour as the reference's module (CatabraFFT.jl):
+
+    fft(x)            CatabraFFT.jl:65-72    forward DFT, returns a new array, x untouched
+    ifft(x)           CatabraFFT.jl:88-101   conj(fft(conj x)) / n
+    bfft(x)           CatabraFFT.jl:103-115  conj(fft(conj x))
+    plan_fft(x, flags) CatabraFFT.jl:149-171 -> Spell (spells.jl:18-37)
+    p * x, p(x), mul(y, p, x)  CatabraFFT.jl:186-196, 224-226
+    empty_cache()     CatabraFFT.jl:39-43
+    NO_FLAG, MEASURE, ENCHANT  spells.jl:4-8
+
+x is a complex64 / complex128 numpy array (host, staged through the device) or a complex torch
+tensor on a GPU (executed in place on the device, on torch's current stream).  A 1-D x is one
+transform; a 2-D x is a batch of rows (an extension: the reference has no batch API).
+
+Differences from the reference, all fixes: a plan's flag is honoured by p * x (the reference
+always runs the NO_FLAG function, CatabraFFT.jl:186-189 -> kernel.jl:97); mul(y, p, x) never
+overwrites x (radix_exec.jl:63-68 does for plans of >= 3 passes); plan_bfft / plan_ifft work
+(their reference stubs reference fields Spell does not have, CatabraFFT.jl:173-183).
+"""
+from __future__ import annotations
+
+import ctypes
+import threading
+
+import numpy as np
+
+from . import _lib
+from ._lib import (BACKWARD, C32, C64, FORWARD, CatabraFFTArgumentError, CatabraFFTError,
+                   check, lib)
+
+NO_FLAG, MEASURE, ENCHANT = 0, 1, 2
+
+__all__ = ["fft", "ifft", "bfft", "plan_fft", "plan_bfft", "plan_ifft", "mul", "empty_cache",
+           "Spell", "Plan", "UnevenPlan", "Context", "context", "NO_FLAG", "MEASURE", "ENCHANT",
+           "CatabraFFTError", "CatabraFFTArgumentError"]
+
+
+# ------------------------------------------------------------------------------ context
+class Context:
+    """A cfft_ctx on one HIP device."""
+
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        check(lib().cfft_init(device, ctypes.byref(h)))
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            lib().cfft_finalize(self.h)
+            self.h = None
+
+    def empty_cache(self):
+        check(lib().cfft_empty_cache(self.h))
+
+    def synchronize(self):
+        check(lib().cfft_synchronize(self.h))
+
+
+_ctx_lock = threading.Lock()
+_contexts: dict[int, Context] = {}
+
+
+def context(device: int = 0) -> Context:
+    with _ctx_lock:
+        c = _contexts.get(device)
+        if c is None:
+            c = _contexts[device] = Context(device)
+        return c
+
+
+# ------------------------------------------------------------------------------ dtypes
+def _dtype_code(dt) -> int:
+    dt = np.dtype(dt)
+    if dt == np.complex64:
+        return C32
+    if dt == np.complex128:
+        return C64
+    raise TypeError(f"CatabraFFT transforms complex vectors (ComplexF32 / ComplexF64), got {dt}")
+
+
+def _is_torch(x) -> bool:
+    t = _lib.torch
+    return t is not None and isinstance(x, t.Tensor)
+
+
+def _torch_dtype_code(x) -> int:
+    t = _lib.torch
+    if x.dtype == t.complex64:
+        return C32
+    if x.dtype == t.complex128:
+        return C64
+    raise TypeError(f"CatabraFFT transforms complex vectors (ComplexF32 / ComplexF64), got {x.dtype}")
+
+
+# ------------------------------------------------------------------------------ plans
+class Plan:
+    """A cfft_plan: `batch` transforms of length n, rows at idist / odist elements."""
+
+    def __init__(self, n: int, batch: int = 1, dtype=np.complex64, flags: int = NO_FLAG,
+                 idist: int | None = None, odist: int | None = None, device: int = 0):
+        self.ctx = context(device)
+        self.n, self.batch = int(n), int(batch)
+        self.idist = int(idist if idist is not None else n)
+        self.odist = int(odist if odist is not None else n)
+        self.dtype_code = dtype if isinstance(dtype, int) else _dtype_code(dtype)
+        self.dtype = np.complex64 if self.dtype_code == C32 else np.complex128
+        self.flags = int(flags)
+        h = ctypes.c_void_p()
+        check(lib().cfft_plan_1d(self.ctx.h, self.n, self.batch, self.idist, self.odist,
+                                 self.dtype_code, self.flags, ctypes.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value and _lib._lib is not None:
+            _lib._lib.cfft_plan_destroy(h)
+            self.h = None
+
+    # device pointers
+    def exec_ptr(self, in_ptr: int, out_ptr: int, direction: int = FORWARD,
+                 normalize: bool = False, stream: int | None = None):
+        check(lib().cfft_exec(self.h, ctypes.c_void_p(in_ptr), ctypes.c_void_p(out_ptr),
+                              direction, int(normalize), ctypes.c_void_p(stream or 0)))
+
+    def exec_torch(self, x, out=None, direction: int = FORWARD, normalize: bool = False):
+        t = _lib.torch
+        if not x.is_contiguous():
+            x = x.contiguous()
+        if out is None:
+            out = t.empty_like(x)
+        stream = t.cuda.current_stream(x.device).cuda_stream
+        self.exec_ptr(x.data_ptr(), out.data_ptr(), direction, normalize, stream)
+        return out
+
+    def exec_host(self, x: np.ndarray, out: np.ndarray | None = None, direction: int = FORWARD,
+                  normalize: bool = False) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=self.dtype)
+        if out is None:
+            out = np.empty_like(x)
+        check(lib().cfft_exec_host(self.h, x.ctypes.data_as(ctypes.c_void_p),
+                                   out.ctypes.data_as(ctypes.c_void_p), direction,
+                                   int(normalize)))
+        return out
+
+    def describe(self) -> str:
+        buf = ctypes.create_string_buffer(4096)
+        check(lib().cfft_plan_describe(self.h, buf, len(buf)))
+        return buf.value.decode()
+
+    def device_bytes(self) -> int:
+        v = ctypes.c_int64()
+        check(lib().cfft_plan_device_bytes(self.h, ctypes.byref(v)))
+        return v.value
+
+
+class UnevenPlan:
+    """cfft_plan_many_uneven: transform i has lengths[i] elements at offsets[i]."""
+
+    def __init__(self, lengths, offsets, dtype=np.complex64, flags: int = NO_FLAG,
+                 device: int = 0):
+        self.ctx = context(device)
+        self.lengths = np.ascontiguousarray(lengths, dtype=np.int64)
+        self.offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+        self.dtype_code = dtype if isinstance(dtype, int) else _dtype_code(dtype)
+        P = ctypes.POINTER(ctypes.c_int64)
+        h = ctypes.c_void_p()
+        check(lib().cfft_plan_many_uneven(self.ctx.h, len(self.lengths),
+                                          self.lengths.ctypes.data_as(P),
+                                          self.offsets.ctypes.data_as(P), self.dtype_code,
+                                          int(flags), ctypes.byref(h)))
+        self.h = h
+
+    __del__ = Plan.__del__
+    exec_ptr = Plan.exec_ptr
+    describe = Plan.describe
+    device_bytes = Plan.device_bytes
+
+    def exec_torch(self, x, out=None, direction: int = FORWARD, normalize: bool = False):
+        t = _lib.torch
+        if out is None:
+            out = x.clone()
+        stream = t.cuda.current_stream(x.device).cuda_stream
+        self.exec_ptr(x.data_ptr(), out.data_ptr(), direction, normalize, stream)
+        return out
+
+
+# ------------------------------------------------------------------------------ user API
+_plan_lock = threading.Lock()
+_plans: dict = {}
+
+
+def _cached_plan(n: int, batch: int, code: int, flags: int, device: int) -> Plan:
+    key = (n, batch, code, flags, device)
+    with _plan_lock:
+        p = _plans.get(key)
+        if p is None:
+            p = _plans[key] = Plan(n, batch, code, flags, device=device)
+        return p
+
+
+def _shape(x) -> tuple[int, int]:
+    if x.ndim == 1:
+        return x.shape[0], 1
+    if x.ndim == 2:
+        return x.shape[1], x.shape[0]
+    raise ValueError("x must be a vector (one transform) or a 2-D array of rows")
+
+
+def _run(x, direction: int, normalize: bool, flags: int = NO_FLAG):
+    if _is_torch(x):
+        code = _torch_dtype_code(x)
+        if x.device.type != "cuda":
+            raise TypeError("torch tensors must live on a GPU; pass numpy arrays for host data")
+        n, batch = _shape(x)
+        if n == 0 or batch == 0:
+            return x.clone()
+        p = _cached_plan(n, batch, code, flags, x.device.index or 0)
+        return p.exec_torch(x, None, direction, normalize)
+    x = np.asarray(x)
+    code = _dtype_code(x.dtype)
+    n, batch = _shape(x)
+    if n == 0 or batch == 0:
+        return x.copy()
+    p = _cached_plan(n, batch, code, flags, 0)
+    return p.exec_host(x, None, direction, normalize)
+
+
+def fft(x):
+    """Forward DFT y[k] = sum_j x[j] e^{-2 pi i jk/n} (CatabraFFT.jl:65-72)."""
+    return _run(x, FORWARD, False)
+
+
+def ifft(x):
+    """Inverse DFT conj(fft(conj x)) / n (CatabraFFT.jl:88-101)."""
+    return _run(x, BACKWARD, True)
+
+
+def bfft(x):
+    """Unnormalised backward DFT conj(fft(conj x)) (CatabraFFT.jl:103-115)."""
+    return _run(x, BACKWARD, False)
+
+
+class Spell:
+    """A plan handle with the fields of spells.jl:18-28 (n, datatype, flag, inverse)."""
+
+    def __init__(self, n: int, datatype, flag: int = NO_FLAG, direction: int = FORWARD,
+                 normalize: bool = False, batch: int = 1):
+        if flag not in (NO_FLAG, MEASURE, ENCHANT):
+            raise ValueError(f"unknown flag {flag}")
+        self.n, self.datatype, self.flag = int(n), np.dtype(datatype), int(flag)
+        self.batch = int(batch)
+        self.direction, self.normalize = direction, normalize
+        self.inverse = None
+        self._plan = _cached_plan(self.n, self.batch, _dtype_code(self.datatype), self.flag, 0)
+
+    def __repr__(self):
+        return f"Spell{{{self.datatype}}}({self.n}, flag={self.flag})"
+
+    def __eq__(self, other):
+        return (isinstance(other, Spell) and (self.n, self.datatype, self.flag, self.direction,
+                                               self.normalize, self.batch) ==
+                (other.n, other.datatype, other.flag, other.direction, other.normalize,
+                 other.batch))
+
+    def __hash__(self):
+        return hash((self.n, self.datatype, self.flag, self.direction, self.normalize))
+
+    def _check(self, x):
+        n, batch = _shape(x)
+        if n != self.n or batch != self.batch:
+            raise ValueError(f"plan is for n={self.n} (batch {self.batch}), got n={n} "
+                             f"(batch {batch})")
+
+    def __mul__(self, x):
+        """p * x (CatabraFFT.jl:192-196): a new output; this plan (and its flag) runs."""
+        self._check(x)
+        if _is_torch(x):
+            return self._plan.exec_torch(x, None, self.direction, self.normalize)
+        return self._plan.exec_host(np.asarray(x), None, self.direction, self.normalize)
+
+    __rmul__ = None
+
+    def __call__(self, x):
+        """p(x) (CatabraFFT.jl:224-226) computes fft(x)."""
+        return fft(x)
+
+    def describe(self) -> str:
+        return self._plan.describe()
+
+
+def plan_fft(x, flags: int = NO_FLAG) -> Spell:
+    """A forward plan for x's length and element type (CatabraFFT.jl:149-171)."""
+    n, batch = _shape(x)
+    dt = np.complex64 if (_is_torch(x) and _torch_dtype_code(x) == C32) else (
+        np.complex128 if _is_torch(x) else np.asarray(x).dtype)
+    return Spell(n, dt, flags, FORWARD, False, batch)
+
+
+def plan_bfft(x, flags: int = NO_FLAG) -> Spell:
+    n, batch = _shape(x)
+    return Spell(n, np.asarray(x).dtype if not _is_torch(x) else plan_fft(x).datatype, flags,
+                 BACKWARD, False, batch)
+
+
+def plan_ifft(x, flags: int = NO_FLAG) -> Spell:
+    n, batch = _shape(x)
+    return Spell(n, np.asarray(x).dtype if not _is_torch(x) else plan_fft(x).datatype, flags,
+                 BACKWARD, True, batch)
+
+
+def mul(y, p: Spell, x):
+    """mul!(y, p, x) (CatabraFFT.jl:186-189): writes p * x into y; x is left untouched."""
+    p._check(x)
+    if _is_torch(x):
+        p._plan.exec_torch(x, y, p.direction, p.normalize)
+        return y
+    out = p._plan.exec_host(np.asarray(x), None, p.direction, p.normalize)
+    y[...] = out
+    return y
+
+
+def empty_cache():
+    """Drops cached plans and staging buffers (CatabraFFT.jl:39-43)."""
+    with _plan_lock:
+        _plans.clear()
+    with _ctx_lock:
+        for c in _contexts.values():
+            c.empty_cache()

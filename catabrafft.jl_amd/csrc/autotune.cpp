@@ -1,0 +1,152 @@
+// autotune.cpp -- MEASURE / ENCHANT planning.
+//
+// CatabraFFT.jl's MEASURE enumerates radix decompositions and times each candidate function once
+// with @elapsed (radix_plan.jl:23-68, radix_exec.jl:270-297); ENCHANT additionally A/B-tests
+// @simd ivdep variants (radix_exec.jl:140-259).  Here the candidates are kernel configurations
+// of the same transform: per-thread capacity E, threads per workgroup, the four-step split and
+// its column tiles.  Each candidate is built, checked to run, and timed with HIP events as the
+// median of several batches of launches on the real batch size; the fastest wins.  The choice is
+// a pure function of (n, batch, dtype, layout) and of the measured times, never of the data.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <sstream>
+#include <vector>
+
+#include "catabrafft.h"
+#include "library.h"
+
+namespace cfft {
+
+static std::vector<PlanOptions> candidates(Ctx* ctx, const Plan& p, int flags) {
+    std::vector<PlanOptions> out;
+    const bool enchant = flags >= CFFT_ENCHANT;
+    const std::vector<int> Es = enchant ? std::vector<int>{8, 16} : std::vector<int>{16};
+    const std::vector<int> tts = {128, 256, 512};
+    PlanOptions base;
+    base.explicit_opts = true;
+    if (p.n == 1) {
+        out.push_back(base);
+        return out;
+    }
+    if (is_smooth(p.n)) {
+        // direct shapes
+        for (int E : Es)
+            for (int tt : tts) {
+                PlanOptions o = base;
+                o.E = E;
+                o.target_threads = tt;
+                out.push_back(o);
+            }
+        // four-step splits near sqrt(n) with a few tile widths
+        std::vector<int64_t> splits;
+        for (int64_t a = 2; a < p.n; ++a) {
+            if (p.n % a) continue;
+            const int64_t b = p.n / a;
+            if (!is_smooth(a) || !is_smooth(b)) continue;
+            if (a < 16 || b < 16) continue;
+            const double r = std::fabs(std::log((double)a / (double)b));
+            if (r <= (enchant ? 2.1 : 0.8)) splits.push_back(a);
+        }
+        const std::vector<int> tiles =
+            enchant ? std::vector<int>{4, 8, 16, 32} : std::vector<int>{8, 16};
+        for (int64_t a : splits)
+            for (int E : Es)
+                for (int ca : tiles)
+                    for (int cb : tiles) {
+                        PlanOptions o = base;
+                        o.E = E;
+                        o.n1 = a;
+                        o.ca = ca;
+                        o.cb = cb;
+                        out.push_back(o);
+                    }
+    } else {
+        for (int E : Es)
+            for (int tt : tts) {
+                PlanOptions o = base;
+                o.E = E;
+                o.target_threads = tt;
+                out.push_back(o);
+            }
+    }
+    (void)ctx;
+    return out;
+}
+
+static void copy_shape(const Plan& from, Plan& to) {
+    to.ctx = from.ctx;
+    to.n = from.n;
+    to.batch = from.batch;
+    to.idist = from.idist;
+    to.odist = from.odist;
+    to.dtype = from.dtype;
+    to.flags = from.flags;
+    to.prec = from.prec;
+}
+
+int autotune(Ctx* ctx, Plan& p, int flags, const PlanOptions& base) {
+    std::vector<PlanOptions> cands = candidates(ctx, p, flags);
+    if (cands.size() <= 1 || p.batch == 0) return plan_build(ctx, p, base);
+    const size_t es = p.prec == Prec::F32 ? 8 : 16;
+    // time on the real shape, but never on more than 1 GiB per buffer
+    int64_t tb = p.batch;
+    const size_t per_in = (size_t)p.idist * es, per_out = (size_t)p.odist * es;
+    while (tb > 1 && (size_t)tb * std::max(per_in, per_out) > ((size_t)1 << 30)) tb /= 2;
+    DevBuf in, out;
+    int rc;
+    if ((rc = in.alloc((size_t)((tb - 1) * p.idist + p.n) * es))) return rc;
+    if ((rc = out.alloc((size_t)((tb - 1) * p.odist + p.n) * es))) return rc;
+    if (hipMemset(in.p, 0, in.bytes) != hipSuccess) return fail(CFFT_ERR_HIP, "hipMemset failed");
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        return fail(CFFT_ERR_HIP, "hipEventCreate failed");
+    double best = 1e300;
+    PlanOptions bestopt = base;
+    bool found = false;
+    const int reps = flags >= CFFT_ENCHANT ? 5 : 3;
+    for (const PlanOptions& o : cands) {
+        Plan t;
+        copy_shape(p, t);
+        t.batch = tb;
+        if (plan_build(ctx, t, o) != CFFT_OK) continue;
+        bool ok = true;
+        for (int w = 0; w < 2 && ok; ++w)
+            ok = plan_exec(t, in.p, out.p, CFFT_FORWARD, 0, ctx->stream) == CFFT_OK;
+        if (!ok || hipStreamSynchronize(ctx->stream) != hipSuccess) {
+            (void)hipGetLastError();
+            continue;
+        }
+        std::vector<double> ms;
+        for (int r = 0; r < reps; ++r) {
+            (void)hipEventRecord(e0, ctx->stream);
+            for (int k = 0; k < 3; ++k)
+                (void)plan_exec(t, in.p, out.p, CFFT_FORWARD, 0, ctx->stream);
+            (void)hipEventRecord(e1, ctx->stream);
+            (void)hipEventSynchronize(e1);
+            float m = 0;
+            (void)hipEventElapsedTime(&m, e0, e1);
+            ms.push_back(m);
+        }
+        std::sort(ms.begin(), ms.end());
+        const double med = ms[ms.size() / 2];
+        if (med < best) {
+            best = med;
+            bestopt = o;
+            found = true;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (!found) return plan_build(ctx, p, base);
+    rc = plan_build(ctx, p, bestopt);
+    if (rc) return rc;
+    std::ostringstream os;
+    os << " tuned(" << (flags >= CFFT_ENCHANT ? "enchant" : "measure") << ", " << cands.size()
+       << " candidates, " << best / 3.0 << " ms)";
+    p.desc = plan_description(p) + os.str();
+    return CFFT_OK;
+}
+
+}  // namespace cfft

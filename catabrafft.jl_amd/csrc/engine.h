@@ -1,0 +1,168 @@
+// engine.h -- device-side Stockham engine shared by every kernel of the library.
+//
+// A "line" is one 1-D transform of length N held by Tt threads of a workgroup.  The passes are
+// the self-sorting Stockham DIF passes of CatabraFFT.jl's layered codelets
+// (radix_2_codelets.jl:402-447 and the radix-3/5/7 twins):
+//
+//     out[q + s(R p + k)] = w_g^{p k} * sum_j in[q + s(p + j n1)] w_R^{j k}
+//     g = N / s, n1 = g / R, q < s, p < n1, butterfly index b = q + s p < N / R
+//
+// Butterfly b reads in[b + j N/R] (unit stride across lanes) and, in the last pass (n1 = 1),
+// writes out[b + k N/R] (unit stride across lanes).  So the first pass can read HBM directly and
+// the last pass can write HBM directly, both fully coalesced; passes in between exchange through
+// LDS.  LDS holds re and im in separate arrays with one pad word every 2^PS words.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "codelets.h"
+
+namespace cfft {
+
+template <typename T> struct Vec2;
+template <> struct Vec2<float> { using type = float2; };
+template <> struct Vec2<double> { using type = double2; };
+
+constexpr int kMaxPasses = 20;
+
+// Radix sequence of one line FFT plus the offsets of each pass's twiddle block
+// tw[twoff[i] + p (R-1) + (k-1)] = w_g^{p k}.
+struct PassTable {
+    int npass;
+    int radix[kMaxPasses];
+    int twoff[kMaxPasses];
+};
+
+template <typename V>
+__device__ __forceinline__ V cmul(V a, V b) {
+    return V{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename V>
+__device__ __forceinline__ V cconj(V a) { return V{a.x, -a.y}; }
+
+template <typename T> struct PadShift;
+template <> struct PadShift<float> { static constexpr int v = 5; };
+template <> struct PadShift<double> { static constexpr int v = 4; };
+
+template <typename T>
+__host__ __device__ constexpr int lds_index(int e) { return e + (e >> PadShift<T>::v); }
+
+// One line's slice of LDS (structure of arrays).
+template <typename T>
+struct LdsLine {
+    using V = typename Vec2<T>::type;
+    T* re;
+    T* im;
+    __device__ __forceinline__ V operator()(int e) const {
+        const int a = lds_index<T>(e);
+        return V{re[a], im[a]};
+    }
+    __device__ __forceinline__ void put(int e, V v) const {
+        const int a = lds_index<T>(e);
+        re[a] = v.x;
+        im[a] = v.y;
+    }
+};
+
+// Per-thread butterfly capacity: MB butterflies of radix R with E complex registers.
+__host__ __device__ constexpr int bfly_per_thread(int E, int R) { return E / R > 0 ? E / R : 1; }
+
+// One Stockham pass.  SRC(e) returns element e of the pass input, DST.put(e, v) stores element e
+// of its output.  mid_barrier separates the reads from the writes when both go to the same LDS.
+template <typename T, int R, int MB, class SRC, class DST>
+__device__ __forceinline__ void stockham_pass(const int N, const int s, const int t, const int Tt,
+                                              const bool active,
+                                              const typename Vec2<T>::type* __restrict__ tw,
+                                              const SRC& src, const DST& dst,
+                                              const bool mid_barrier) {
+    using V = typename Vec2<T>::type;
+    const int nb = N / R;
+    V v[MB][R];
+#pragma unroll
+    for (int ib = 0; ib < MB; ++ib) {
+        const int b = t + ib * Tt;
+        if (active && b < nb) {
+#pragma unroll
+            for (int j = 0; j < R; ++j) v[ib][j] = src(b + j * nb);
+        }
+    }
+    if (mid_barrier) __syncthreads();
+#pragma unroll
+    for (int ib = 0; ib < MB; ++ib) {
+        const int b = t + ib * Tt;
+        if (active && b < nb) {
+            Dft<R>::run(v[ib]);
+            const int p = b / s;
+            const int q = b - p * s;
+            if (p > 0) {
+                const V* w = tw + p * (R - 1);
+#pragma unroll
+                for (int k = 1; k < R; ++k) v[ib][k] = cmul(v[ib][k], w[k - 1]);
+            }
+            const int base = q + s * R * p;
+#pragma unroll
+            for (int k = 0; k < R; ++k) dst.put(base + s * k, v[ib][k]);
+        }
+    }
+}
+
+// Dispatch of one pass on its radix.  The four source/destination combinations are
+// instantiated by the callers below.
+#define CFFT_RADIX_SWITCH(R_, CALL)                                  \
+    switch (R_) {                                                    \
+        case 16: CALL(16); break;                                    \
+        case 13: CALL(13); break;                                    \
+        case 11: CALL(11); break;                                    \
+        case 9: CALL(9); break;                                      \
+        case 8: CALL(8); break;                                      \
+        case 7: CALL(7); break;                                      \
+        case 5: CALL(5); break;                                      \
+        case 4: CALL(4); break;                                      \
+        case 3: CALL(3); break;                                      \
+        case 2: CALL(2); break;                                      \
+        default: break;                                              \
+    }
+
+// Runs every pass of one line.  GSRC / GDST are the input and output functors of the line (HBM,
+// or a gather / scatter over another LDS buffer); when from_global (to_global) is false the
+// line's input is already in LDS (its output is left in LDS).  dst_aliases_lds asks for a
+// barrier between the reads and writes of the last pass because GDST writes the line's own LDS.
+// The caller provides the barrier before the first pass when it staged the input into LDS.
+template <typename T, int E, class GSRC, class GDST>
+__device__ __forceinline__ void run_line(const PassTable& pt, const int N, const int t,
+                                         const int Tt, const bool active,
+                                         const typename Vec2<T>::type* __restrict__ tw,
+                                         const GSRC& gsrc, const GDST& gdst,
+                                         const LdsLine<T>& lds, const bool from_global,
+                                         const bool to_global,
+                                         const bool dst_aliases_lds = false) {
+    int s = 1;
+    for (int ip = 0; ip < pt.npass; ++ip) {
+        const int R = pt.radix[ip];
+        const bool first = ip == 0, last = ip == pt.npass - 1;
+        const typename Vec2<T>::type* twp = tw + pt.twoff[ip];
+        const bool sg = first && from_global;
+        const bool dg = last && to_global;
+        if (sg && dg) {
+#define CFFT_CALL_GG(RR) stockham_pass<T, RR, bfly_per_thread(E, RR)>(N, s, t, Tt, active, twp, gsrc, gdst, dst_aliases_lds)
+            CFFT_RADIX_SWITCH(R, CFFT_CALL_GG)
+#undef CFFT_CALL_GG
+        } else if (sg) {
+#define CFFT_CALL_GL(RR) stockham_pass<T, RR, bfly_per_thread(E, RR)>(N, s, t, Tt, active, twp, gsrc, lds, false)
+            CFFT_RADIX_SWITCH(R, CFFT_CALL_GL)
+#undef CFFT_CALL_GL
+        } else if (dg) {
+#define CFFT_CALL_LG(RR) stockham_pass<T, RR, bfly_per_thread(E, RR)>(N, s, t, Tt, active, twp, lds, gdst, dst_aliases_lds)
+            CFFT_RADIX_SWITCH(R, CFFT_CALL_LG)
+#undef CFFT_CALL_LG
+        } else {
+#define CFFT_CALL_LL(RR) stockham_pass<T, RR, bfly_per_thread(E, RR)>(N, s, t, Tt, active, twp, lds, lds, true)
+            CFFT_RADIX_SWITCH(R, CFFT_CALL_LL)
+#undef CFFT_CALL_LL
+        }
+        // The pass output sits in LDS for the next pass (or for the caller's staged store).
+        if (!dg) __syncthreads();
+        s *= R;
+    }
+}
+
+}  // namespace cfft

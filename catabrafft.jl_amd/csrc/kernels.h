@@ -1,0 +1,103 @@
+// kernels.h -- host-visible launch interface of the HIP kernels (kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "engine.h"
+
+namespace cfft {
+
+enum class Prec : int { F32 = 0, F64 = 1 };
+
+// Epilogue applied by the last store of a line FFT that is the forward half of a convolution:
+// v -> conj(v * K[e]) and dc[line] = v when e == 0 (Rader needs sum(x) = x0 + A^[0]).
+struct ConvEpilogue {
+    const void* K;   // kernel spectrum, length = line length, same precision as the line
+    void* dc;        // one complex per line, or nullptr
+};
+
+// Batched contiguous lines: line b reads in[b*idist + e], writes out[b*odist + e].
+struct RowsArgs {
+    int N, C, Tt, Ls, E;
+    int64_t nlines, idist, odist;
+    int sgn_in, sgn_out;   // -1 conjugates on load / store (backward = conj(fft(conj x)))
+    double scale;          // applied on store
+    ConvEpilogue epi;      // epi.K == nullptr: none
+};
+
+// Four-step pass A for n = N1*N2 (lines = the N2 columns of the N1 x N2 row-major input):
+//   out[b*odist + j2*N1 + k1] = w_n^{j2 k1} * DFT_N1(in[b*idist + j1*N2 + j2])[k1]
+// w_n^e = T1[e & mask] * T2[e >> shift] (two small tables, e < n).
+struct FourStepAArgs {
+    int N1, N2, C, Tt, Ls, E;
+    int64_t batch, idist, odist;
+    int sgn_in;
+    int shift;
+    int mask;
+    const void* T1;
+    const void* T2;
+};
+
+// Four-step pass B (lines = the N1 values of k1, elements j2 at stride N1):
+//   out[b*odist + k1 + N1*k2] = DFT_N2(in[b*idist + j2*N1 + k1])[k2]
+// Reads and writes the same index set per tile, so in == out is allowed.
+struct FourStepBArgs {
+    int N1, N2, C, Tt, Ls, E;
+    int64_t batch, idist, odist;
+    int sgn_out;
+    double scale;
+    ConvEpilogue epi;
+};
+
+// Fused small convolution FFT (Rader or Bluestein) entirely in LDS: one launch, one HBM read
+// and one HBM write per transform.
+enum class ConvMode : int { Rader = 0, Bluestein = 1 };
+struct ConvSmallArgs {
+    int n, L, C, Tt, LsX, LsA, E;
+    int mode;              // ConvMode
+    int64_t nlines, idist, odist;
+    int sgn_in, sgn_out;
+    double scale;
+    const int* perm;       // Rader: g^i mod n, i < n-1
+    const int* iperm;      // Rader: (g^-1)^i mod n, i < n-1
+    const void* chirp;     // Bluestein: e^{-pi i m^2/n}, m < n (line precision)
+    const void* K;         // FFT_L of the convolution kernel (line precision)
+};
+
+// Large convolution: gather/pre-multiply into the (zero-padded) length-L work buffer, and the
+// final scatter/post-multiply.  Tio = precision of x / y, Tc = precision of the work buffer.
+struct ConvGatherArgs {
+    int64_t n, L, nlines, idist, wdist;
+    int mode, sgn_in;
+    const int* perm;
+    const void* chirp;     // Tc precision
+};
+struct ConvScatterArgs {
+    int64_t n, L, nlines, odist, idist, wdist;
+    int mode, sgn_in, sgn_out;
+    double invL;           // 1/L of the inverse transform
+    double scale;          // output scale (1/n for ifft)
+    const int* iperm;
+    const void* chirp;     // Tc precision
+    const void* dc;        // per line A^[0] (Tc precision), Rader only
+    const void* x;         // original input (Tio), Rader reads x0 from it
+};
+
+hipError_t launch_rows(Prec p, const void* in, void* out, const RowsArgs& a, const PassTable& pt,
+                       const void* tw, hipStream_t st);
+hipError_t launch_fourstep_a(Prec p, const void* in, void* out, const FourStepAArgs& a,
+                             const PassTable& pt, const void* tw, hipStream_t st);
+hipError_t launch_fourstep_b(Prec p, const void* in, void* out, const FourStepBArgs& a,
+                             const PassTable& pt, const void* tw, hipStream_t st);
+hipError_t launch_conv_small(Prec p, const void* in, void* out, const ConvSmallArgs& a,
+                             const PassTable& pt, const void* tw, hipStream_t st);
+hipError_t launch_conv_gather(Prec pio, Prec pc, const void* x, void* w, const ConvGatherArgs& a,
+                              hipStream_t st);
+hipError_t launch_conv_scatter(Prec pio, Prec pc, const void* w, void* y,
+                               const ConvScatterArgs& a, hipStream_t st);
+
+// Largest per-thread complex capacity compiled (template E values).
+constexpr int kEValues[] = {8, 16};
+
+}  // namespace cfft

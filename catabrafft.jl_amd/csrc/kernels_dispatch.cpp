@@ -1,0 +1,34 @@
+// kernels_dispatch.cpp -- launch_* entry points of kernels.h, dispatched on precision.
+#include "kernels.h"
+#include "kernels_prec.h"
+
+namespace cfft {
+
+hipError_t launch_rows(Prec p, const void* in, void* out, const RowsArgs& a, const PassTable& pt,
+                       const void* tw, hipStream_t st) {
+    return p == Prec::F32 ? rows_f32(in, out, a, pt, tw, st) : rows_f64(in, out, a, pt, tw, st);
+}
+hipError_t launch_fourstep_a(Prec p, const void* in, void* out, const FourStepAArgs& a,
+                             const PassTable& pt, const void* tw, hipStream_t st) {
+    return p == Prec::F32 ? fsa_f32(in, out, a, pt, tw, st) : fsa_f64(in, out, a, pt, tw, st);
+}
+hipError_t launch_fourstep_b(Prec p, const void* in, void* out, const FourStepBArgs& a,
+                             const PassTable& pt, const void* tw, hipStream_t st) {
+    return p == Prec::F32 ? fsb_f32(in, out, a, pt, tw, st) : fsb_f64(in, out, a, pt, tw, st);
+}
+hipError_t launch_conv_small(Prec p, const void* in, void* out, const ConvSmallArgs& a,
+                             const PassTable& pt, const void* tw, hipStream_t st) {
+    return p == Prec::F32 ? conv_f32(in, out, a, pt, tw, st) : conv_f64(in, out, a, pt, tw, st);
+}
+hipError_t launch_conv_gather(Prec pio, Prec pc, const void* x, void* w, const ConvGatherArgs& a,
+                              hipStream_t st) {
+    if (pio == Prec::F32) return pc == Prec::F32 ? gather_f32_f32(x, w, a, st) : gather_f32_f64(x, w, a, st);
+    return gather_f64_f64(x, w, a, st);
+}
+hipError_t launch_conv_scatter(Prec pio, Prec pc, const void* w, void* y,
+                               const ConvScatterArgs& a, hipStream_t st) {
+    if (pio == Prec::F32) return pc == Prec::F32 ? scatter_f32_f32(w, y, a, st) : scatter_f32_f64(w, y, a, st);
+    return scatter_f64_f64(w, y, a, st);
+}
+
+}  // namespace cfft

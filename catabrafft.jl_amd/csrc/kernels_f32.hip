@@ -1,0 +1,30 @@
+// kernels_f32.hip -- single-precision instantiation of the kernels (kernels_impl.h).
+#include "kernels_impl.h"
+#include "kernels_prec.h"
+
+namespace cfft {
+using namespace detail;
+#define CFFT_E(FN, T, E_, ...) ((E_) <= 8 ? FN<T, 8>(__VA_ARGS__) : FN<T, 16>(__VA_ARGS__))
+hipError_t rows_f32(const void* in, void* out, const RowsArgs& a, const PassTable& pt,
+                    const void* tw, hipStream_t st) {
+    return CFFT_E(rows_t, float, a.E, in, out, a, pt, tw, st);
+}
+hipError_t fsa_f32(const void* in, void* out, const FourStepAArgs& a, const PassTable& pt,
+                   const void* tw, hipStream_t st) {
+    return CFFT_E(fsa_t, float, a.E, in, out, a, pt, tw, st);
+}
+hipError_t fsb_f32(const void* in, void* out, const FourStepBArgs& a, const PassTable& pt,
+                   const void* tw, hipStream_t st) {
+    return CFFT_E(fsb_t, float, a.E, in, out, a, pt, tw, st);
+}
+hipError_t conv_f32(const void* in, void* out, const ConvSmallArgs& a, const PassTable& pt,
+                    const void* tw, hipStream_t st) {
+    return CFFT_E(conv_t, float, a.E, in, out, a, pt, tw, st);
+}
+hipError_t gather_f32_f32(const void* x, void* w, const ConvGatherArgs& a, hipStream_t st) {
+    return gather_t<float, float>(x, w, a, st);
+}
+hipError_t scatter_f32_f32(const void* w, void* y, const ConvScatterArgs& a, hipStream_t st) {
+    return scatter_t<float, float>(w, y, a, st);
+}
+}  // namespace cfft

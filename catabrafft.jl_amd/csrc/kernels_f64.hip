@@ -1,0 +1,37 @@
+// kernels_f64.hip -- double-precision instantiation of the kernels (kernels_impl.h), plus the
+// FP32-I/O / FP64-work gather and scatter of large convolutions.
+#include "kernels_impl.h"
+#include "kernels_prec.h"
+
+namespace cfft {
+using namespace detail;
+#define CFFT_E(FN, T, E_, ...) ((E_) <= 8 ? FN<T, 8>(__VA_ARGS__) : FN<T, 16>(__VA_ARGS__))
+hipError_t rows_f64(const void* in, void* out, const RowsArgs& a, const PassTable& pt,
+                    const void* tw, hipStream_t st) {
+    return CFFT_E(rows_t, double, a.E, in, out, a, pt, tw, st);
+}
+hipError_t fsa_f64(const void* in, void* out, const FourStepAArgs& a, const PassTable& pt,
+                   const void* tw, hipStream_t st) {
+    return CFFT_E(fsa_t, double, a.E, in, out, a, pt, tw, st);
+}
+hipError_t fsb_f64(const void* in, void* out, const FourStepBArgs& a, const PassTable& pt,
+                   const void* tw, hipStream_t st) {
+    return CFFT_E(fsb_t, double, a.E, in, out, a, pt, tw, st);
+}
+hipError_t conv_f64(const void* in, void* out, const ConvSmallArgs& a, const PassTable& pt,
+                    const void* tw, hipStream_t st) {
+    return CFFT_E(conv_t, double, a.E, in, out, a, pt, tw, st);
+}
+hipError_t gather_f32_f64(const void* x, void* w, const ConvGatherArgs& a, hipStream_t st) {
+    return gather_t<float, double>(x, w, a, st);
+}
+hipError_t gather_f64_f64(const void* x, void* w, const ConvGatherArgs& a, hipStream_t st) {
+    return gather_t<double, double>(x, w, a, st);
+}
+hipError_t scatter_f32_f64(const void* w, void* y, const ConvScatterArgs& a, hipStream_t st) {
+    return scatter_t<float, double>(w, y, a, st);
+}
+hipError_t scatter_f64_f64(const void* w, void* y, const ConvScatterArgs& a, hipStream_t st) {
+    return scatter_t<double, double>(w, y, a, st);
+}
+}  // namespace cfft

@@ -1,0 +1,29 @@
+// kernels_prec.h -- per-precision launch entry points (defined in kernels_f32.hip and
+// kernels_f64.hip, dispatched on Prec by kernels_dispatch.cpp).
+#pragma once
+#include "kernels.h"
+
+namespace cfft {
+
+#define CFFT_DECLARE_LAUNCHERS(SFX)                                                          \
+    hipError_t rows_##SFX(const void* in, void* out, const RowsArgs& a, const PassTable& pt, \
+                          const void* tw, hipStream_t st);                                  \
+    hipError_t fsa_##SFX(const void* in, void* out, const FourStepAArgs& a,                 \
+                         const PassTable& pt, const void* tw, hipStream_t st);              \
+    hipError_t fsb_##SFX(const void* in, void* out, const FourStepBArgs& a,                 \
+                         const PassTable& pt, const void* tw, hipStream_t st);              \
+    hipError_t conv_##SFX(const void* in, void* out, const ConvSmallArgs& a,                \
+                          const PassTable& pt, const void* tw, hipStream_t st);
+
+CFFT_DECLARE_LAUNCHERS(f32)
+CFFT_DECLARE_LAUNCHERS(f64)
+#undef CFFT_DECLARE_LAUNCHERS
+
+hipError_t gather_f32_f32(const void* x, void* w, const ConvGatherArgs& a, hipStream_t st);
+hipError_t gather_f32_f64(const void* x, void* w, const ConvGatherArgs& a, hipStream_t st);
+hipError_t gather_f64_f64(const void* x, void* w, const ConvGatherArgs& a, hipStream_t st);
+hipError_t scatter_f32_f32(const void* w, void* y, const ConvScatterArgs& a, hipStream_t st);
+hipError_t scatter_f32_f64(const void* w, void* y, const ConvScatterArgs& a, hipStream_t st);
+hipError_t scatter_f64_f64(const void* w, void* y, const ConvScatterArgs& a, hipStream_t st);
+
+}  // namespace cfft

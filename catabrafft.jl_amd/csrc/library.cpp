@@ -1,0 +1,870 @@
+// library.cpp -- context, planner, plan cache, execution and the C ABI (include/catabrafft.h).
+//
+// Strategy per length n (the role of kernel.jl:72-92's size classification):
+//   n == 1 ............................ copy
+//   n smooth (factors 2,3,5,7,11,13), one line fits a workgroup's LDS
+//                                       DIRECT: one k_rows launch (all passes in LDS)
+//   n smooth, larger .................. FOURSTEP n = N1*N2 (mixed_radix.jl:44-78): k_fs_a, k_fs_b
+//   n prime ........................... Rader (prime.jl:5-82): cyclic convolution of length
+//                                       n-1, zero-padded to a smooth L >= 2n-3 when n-1 is not
+//                                       smooth; fused in one k_conv_small launch when it fits LDS,
+//                                       else gather -> FFT_L -> FFT_L -> scatter
+//   other n ........................... Bluestein with a smooth L >= 2n-1, same two shapes
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "catabrafft.h"
+#include "hostmath.h"
+#include "kernels.h"
+#include "library.h"
+
+namespace cfft {
+
+// ------------------------------------------------------------------------------- errors
+static thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define CFFT_HIP(expr)                                                                   \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return fail(CFFT_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// ------------------------------------------------------------------------------- device buffers
+DevBuf::~DevBuf() { reset(); }
+void DevBuf::reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+int DevBuf::alloc(size_t nbytes) {
+    reset();
+    if (nbytes == 0) return CFFT_OK;
+    if (hipMalloc(&p, nbytes) != hipSuccess) {
+        p = nullptr;
+        (void)hipGetLastError();
+        return fail(CFFT_ERR_NOMEM, "device allocation of " + std::to_string(nbytes) + " bytes failed");
+    }
+    bytes = nbytes;
+    return CFFT_OK;
+}
+
+// Uploads long double complex values rounded once to the plan precision.
+static int upload(DevBuf& buf, const std::vector<cld>& v, Prec pr) {
+    if (v.empty()) return CFFT_OK;
+    const size_t esz = pr == Prec::F32 ? 8 : 16;
+    int rc = buf.alloc(v.size() * esz);
+    if (rc) return rc;
+    if (pr == Prec::F32) {
+        std::vector<float> h(2 * v.size());
+        for (size_t i = 0; i < v.size(); ++i) {
+            h[2 * i] = (float)v[i].real();
+            h[2 * i + 1] = (float)v[i].imag();
+        }
+        CFFT_HIP(hipMemcpy(buf.p, h.data(), buf.bytes, hipMemcpyHostToDevice));
+    } else {
+        std::vector<double> h(2 * v.size());
+        for (size_t i = 0; i < v.size(); ++i) {
+            h[2 * i] = (double)v[i].real();
+            h[2 * i + 1] = (double)v[i].imag();
+        }
+        CFFT_HIP(hipMemcpy(buf.p, h.data(), buf.bytes, hipMemcpyHostToDevice));
+    }
+    return CFFT_OK;
+}
+
+static int upload_int(DevBuf& buf, const std::vector<int>& v) {
+    if (v.empty()) return CFFT_OK;
+    int rc = buf.alloc(v.size() * sizeof(int));
+    if (rc) return rc;
+    CFFT_HIP(hipMemcpy(buf.p, v.data(), buf.bytes, hipMemcpyHostToDevice));
+    return CFFT_OK;
+}
+
+// ------------------------------------------------------------------------------- factorisation
+static const int kPrimes[] = {2, 3, 5, 7, 11, 13};
+
+bool is_smooth(int64_t n) {
+    if (n < 1) return false;
+    for (int p : kPrimes)
+        while (n % p == 0) n /= p;
+    return n == 1;
+}
+
+static bool is_smooth7(int64_t n) {
+    for (int p : {2, 3, 5, 7})
+        while (n % p == 0) n /= p;
+    return n == 1;
+}
+
+// Smallest 7-smooth length >= m (convolution lengths; 11 and 13 are kept out so that the
+// padded FFTs stay on the cheapest codelets).
+int64_t next_smooth(int64_t m) {
+    int64_t best = 1;
+    while (best < m) best *= 2;
+    for (int64_t a = 1; a <= best; a *= 2)
+        for (int64_t b = a; b <= best; b *= 3)
+            for (int64_t c = b; c <= best; c *= 5)
+                for (int64_t d = c; d <= best; d *= 7)
+                    if (d >= m && d < best) best = d;
+    return best;
+}
+
+// Radix sequence of one line: 16s (with one 8, 4 or 2 for the remainder), 9s and a 3, then 13, 11,
+// 7, 5; largest first.  Empty when n is not smooth over {2,3,5,7,11,13}.
+std::vector<int> line_radices(int64_t n) {
+    std::vector<int> r;
+    if (!is_smooth(n) || n < 2) return r;
+    int cnt[14] = {0};
+    int64_t m = n;
+    for (int p : kPrimes)
+        while (m % p == 0) {
+            cnt[p]++;
+            m /= p;
+        }
+    const int a = cnt[2], q = a / 4, rem = a % 4;
+    if (rem == 1 && q >= 1) {
+        for (int i = 0; i < q - 1; ++i) r.push_back(16);
+        r.push_back(8);
+        r.push_back(4);
+    } else {
+        for (int i = 0; i < q; ++i) r.push_back(16);
+        if (rem == 1) r.push_back(2);
+        if (rem == 2) r.push_back(4);
+        if (rem == 3) r.push_back(8);
+    }
+    for (int i = 0; i < cnt[3] / 2; ++i) r.push_back(9);
+    if (cnt[3] % 2) r.push_back(3);
+    for (int p : {13, 11, 7, 5})
+        for (int i = 0; i < cnt[p]; ++i) r.push_back(p);
+    std::sort(r.begin(), r.end(), std::greater<int>());
+    return r;
+}
+
+int threads_for(const std::vector<int>& rad, int64_t N, int E) {
+    int tt = 1;
+    for (int R : rad) {
+        const int64_t nb = N / R;
+        const int mb = bfly_per_thread(E, R);
+        tt = std::max<int>(tt, (int)((nb + mb - 1) / mb));
+    }
+    return tt;
+}
+
+template <typename T>
+static int line_stride(int64_t N) {
+    int ls = lds_index<T>((int)N - 1) + 1;
+    if ((ls & 1) == 0) ++ls;  // odd line stride: consecutive lines fall in different banks
+    return ls;
+}
+
+int line_stride_for(Prec p, int64_t N) {
+    return p == Prec::F32 ? line_stride<float>(N) : line_stride<double>(N);
+}
+
+// Builds the pass table, thread count and twiddles of one line FFT.  E = 0 picks the default.
+int make_line(LinePlan& lp, int64_t N, Prec pr, int E, int max_threads) {
+    lp.N = (int)N;
+    std::vector<int> rad = line_radices(N);
+    if (rad.empty() || (int)rad.size() > kMaxPasses)
+        return fail(CFFT_ERR_UNSUPPORTED, "line length " + std::to_string(N) + " is not smooth");
+    if (E == 0) E = 16;
+    lp.E = E;
+    lp.Tt = threads_for(rad, N, E);
+    if (lp.Tt > max_threads)
+        return fail(CFFT_ERR_UNSUPPORTED, "line length " + std::to_string(N) + " needs " +
+                                              std::to_string(lp.Tt) + " threads");
+    lp.pt.npass = (int)rad.size();
+    std::vector<cld> tw;
+    int64_t s = 1;
+    for (size_t i = 0; i < rad.size(); ++i) {
+        const int R = rad[i];
+        lp.pt.radix[i] = R;
+        lp.pt.twoff[i] = (int)tw.size();
+        const int64_t g = N / s, n1 = g / R;
+        for (int64_t p = 0; p < n1; ++p)
+            for (int k = 1; k < R; ++k) tw.push_back(omega(p * k, g));
+        s *= R;
+    }
+    if (tw.empty()) tw.push_back(cld(1, 0));
+    lp.Ls = line_stride_for(pr, N);
+    return upload(lp.dtw, tw, pr);
+}
+
+static int64_t esize(Prec p) { return p == Prec::F32 ? 8 : 16; }
+
+// ------------------------------------------------------------------------------- planning
+static int pow2_floor(int64_t v) {
+    int r = 1;
+    while ((int64_t)r * 2 <= v) r *= 2;
+    return r;
+}
+
+// Lines per workgroup for the rows kernel.
+static int rows_lines_per_block(const LinePlan& lp, Prec pr, int64_t nlines, size_t lds_budget,
+                                int target_threads) {
+    const size_t per = 2 * (size_t)(pr == Prec::F32 ? 4 : 8) * lp.Ls;
+    int c = std::max(1, target_threads / lp.Tt);
+    c = std::min<int64_t>(c, std::max<size_t>(1, lds_budget / per));
+    c = std::min<int64_t>(c, std::max<int64_t>(1, nlines));
+    c = std::min(c, std::max(1, 1024 / lp.Tt));
+    return c;
+}
+
+static int plan_direct(Ctx* ctx, Plan& p, int E, int target_threads) {
+    int rc = make_line(p.line, p.n, p.prec, E, 1024);
+    if (rc) return rc;
+    const size_t per = 2 * (size_t)esize(p.prec) / 2 * p.line.Ls;
+    if (per > ctx->lds_max) return fail(CFFT_ERR_UNSUPPORTED, "line does not fit LDS");
+    p.C = rows_lines_per_block(p.line, p.prec, p.batch, std::max<size_t>(per, ctx->lds_rows_budget),
+                               target_threads);
+    p.kind = Kind::Direct;
+    return CFFT_OK;
+}
+
+// Can a line of length N run as one workgroup (threads and LDS)?
+static bool line_fits(Ctx* ctx, int64_t N, Prec pr, int E) {
+    if (N < 2 || !is_smooth(N) || N > (1 << 24)) return false;
+    std::vector<int> rad = line_radices(N);
+    if ((int)rad.size() > kMaxPasses) return false;
+    if (threads_for(rad, N, E) > 1024) return false;
+    const size_t per = 2 * (size_t)(pr == Prec::F32 ? 4 : 8) * line_stride_for(pr, N);
+    return per <= ctx->lds_max;
+}
+
+static int pick_tile(Ctx* ctx, const LinePlan& lp, Prec pr, int want) {
+    const size_t per = 2 * (size_t)(pr == Prec::F32 ? 4 : 8) * lp.Ls;
+    int c = pow2_floor(std::max<int64_t>(1, (int64_t)(ctx->lds_max / per)));
+    c = std::min(c, pow2_floor(std::max(1, 1024 / lp.Tt)));
+    c = std::min(c, want);
+    return std::max(c, 1);
+}
+
+static int plan_fourstep_factors(Ctx* ctx, FourStepPlan& fs, int64_t n, Prec pr, int E,
+                                 int64_t n1_force, int ca, int cb) {
+    int64_t best1 = 0;
+    if (n1_force > 0) {
+        best1 = n1_force;
+    } else {
+        // balanced split with both factors fitting one workgroup
+        double bestscore = 1e300;
+        for (int64_t a = 2; a * a <= n * 64 && a < n; ++a) {
+            if (n % a) continue;
+            const int64_t b = n / a;
+            if (!line_fits(ctx, a, pr, E) || !line_fits(ctx, b, pr, E)) continue;
+            const double score = std::fabs(std::log((double)a / (double)b)) + (a < b ? 0.01 : 0.0);
+            if (score < bestscore) {
+                bestscore = score;
+                best1 = a;
+            }
+        }
+    }
+    if (best1 == 0) return fail(CFFT_ERR_UNSUPPORTED, "no four-step split for n=" + std::to_string(n));
+    fs.N1 = (int)best1;
+    fs.N2 = (int)(n / best1);
+    int rc = make_line(fs.a, fs.N1, pr, E, 1024);
+    if (rc) return rc;
+    rc = make_line(fs.b, fs.N2, pr, E, 1024);
+    if (rc) return rc;
+    const int want = pr == Prec::F32 ? 16 : 8;
+    fs.CA = ca > 0 ? ca : pick_tile(ctx, fs.a, pr, want);
+    fs.CB = cb > 0 ? cb : pick_tile(ctx, fs.b, pr, want);
+    // a requested tile must be a power of two that fits 1024 threads and the LDS
+    for (auto [C, lp] : {std::make_pair(fs.CA, &fs.a), std::make_pair(fs.CB, &fs.b)}) {
+        const size_t lds = 2 * (size_t)(pr == Prec::F32 ? 4 : 8) * lp->Ls * (size_t)C;
+        if (C < 1 || (C & (C - 1)) || C * lp->Tt > 1024 || lds > ctx->lds_max)
+            return fail(CFFT_ERR_UNSUPPORTED, "four-step tile does not fit a workgroup");
+    }
+    // w_n^e for e < n as T1[e & mask] * T2[e >> shift]
+    int shift = 0;
+    while (((int64_t)1 << (2 * shift)) < n) ++shift;
+    fs.shift = shift;
+    fs.mask = (1 << shift) - 1;
+    std::vector<cld> t1((size_t)1 << shift), t2((size_t)((n >> shift) + 1));
+    for (int64_t i = 0; i < (int64_t)t1.size(); ++i) t1[i] = omega(i, n);
+    for (int64_t i = 0; i < (int64_t)t2.size(); ++i) t2[i] = omega(i << shift, n);
+    rc = upload(fs.T1, t1, pr);
+    if (rc) return rc;
+    return upload(fs.T2, t2, pr);
+}
+
+static int plan_fft_core(Ctx* ctx, Plan& p, const PlanOptions& o);
+
+// Rader / Bluestein planning.
+static int plan_conv(Ctx* ctx, Plan& p, const PlanOptions& o) {
+    const int64_t n = p.n;
+    const bool rader = is_prime(n);
+    p.mode = rader ? ConvMode::Rader : ConvMode::Bluestein;
+    std::vector<cld> b;  // convolution kernel, length L
+    std::vector<cld> chirp;
+    std::vector<int> perm, iperm;
+    if (rader) {
+        const int64_t N = n - 1;
+        p.L = is_smooth(N) ? N : next_smooth(2 * N - 1);
+        const int64_t g = primitive_root(n);
+        const int64_t ginv = powmod(g, n - 2, n);
+        perm.resize(N);
+        iperm.resize(N);
+        std::vector<cld> w(N);
+        int64_t gp = 1, gi = 1;
+        for (int64_t i = 0; i < N; ++i) {
+            perm[i] = (int)gp;
+            iperm[i] = (int)gi;
+            w[i] = omega(gi, n);  // w[i] = w_n^{(g^-1)^i}
+            gp = gp * g % n;
+            gi = gi * ginv % n;
+        }
+        b.assign(p.L, cld(0, 0));
+        for (int64_t m = 0; m < N; ++m) b[m] = w[m];
+        for (int64_t m = p.L - N + 1; m < p.L; ++m) b[m] = w[m - (p.L - N)];
+        p.g = g;
+    } else {
+        p.L = next_smooth(2 * n - 1);
+        chirp.resize(n);
+        for (int64_t m = 0; m < n; ++m) {
+            const int64_t e = (int64_t)((unsigned __int128)m * m % (unsigned __int128)(2 * n));
+            chirp[m] = omega(e, 2 * n);  // e^{-pi i m^2 / n}
+        }
+        b.assign(p.L, cld(0, 0));
+        for (int64_t m = 0; m < n; ++m) b[m] = std::conj(chirp[m]);
+        for (int64_t m = 1; m < n; ++m) b[p.L - m] = std::conj(chirp[m]);
+    }
+    if (!host_fft_ld(b)) return fail(CFFT_ERR_UNSUPPORTED, "kernel spectrum length not smooth");
+    // precision of the convolution: FP64 when the I/O is FP32 and the prime exceeds 2^16
+    p.cprec = p.prec;
+    if (p.prec == Prec::F32 && n > 65536) p.cprec = Prec::F64;
+    int rc;
+    // fused small shape?
+    const int E = o.E ? o.E : 16;
+    bool small = false;
+    if (p.cprec == p.prec && line_fits(ctx, p.L, p.prec, E)) {
+        rc = make_line(p.line, p.L, p.prec, E, 1024);
+        if (rc) return rc;
+        p.LsX = line_stride_for(p.prec, n);
+        p.LsA = p.line.Ls;
+        const size_t per = 2 * (size_t)(esize(p.prec) / 2) * (p.LsX + p.LsA) + esize(p.prec);
+        if (per <= ctx->lds_max) {
+            small = true;
+            int c = std::max(1, o.target_threads / p.line.Tt);
+            c = std::min<int64_t>(c, std::max<size_t>(1, std::max(ctx->lds_rows_budget, per) / per));
+            c = std::min<int64_t>(c, std::max<int64_t>(1, p.batch));
+            c = std::min(c, std::max(1, 1024 / p.line.Tt));
+            p.C = c;
+        }
+    }
+    if (rader) {
+        if ((rc = upload_int(p.perm, perm))) return rc;
+        if ((rc = upload_int(p.iperm, iperm))) return rc;
+    } else {
+        if ((rc = upload(p.chirp, chirp, p.cprec))) return rc;
+    }
+    if ((rc = upload(p.K, b, p.cprec))) return rc;
+    if (small) {
+        p.kind = Kind::ConvSmall;
+        return CFFT_OK;
+    }
+    // large shape: sub-plan for the length-L FFT in cprec, batch lines, contiguous
+    p.kind = Kind::ConvLarge;
+    p.sub.reset(new Plan());
+    Plan& s = *p.sub;
+    s.ctx = ctx;
+    s.n = p.L;
+    s.batch = p.batch;
+    s.idist = s.odist = p.L;
+    s.prec = p.cprec;
+    s.dtype = p.cprec == Prec::F32 ? CFFT_C32 : CFFT_C64;
+    PlanOptions so = o;
+    so.n1 = 0;
+    rc = plan_fft_core(ctx, s, so);
+    if (rc) return rc;
+    if (s.kind != Kind::Direct && s.kind != Kind::FourStep)
+        return fail(CFFT_ERR_UNSUPPORTED, "convolution length not plannable");
+    const size_t wbytes = (size_t)p.batch * p.L * esize(p.cprec);
+    if ((rc = p.w1.alloc(wbytes))) return rc;
+    if ((rc = p.w2.alloc(wbytes))) return rc;
+    if ((rc = p.dc.alloc((size_t)p.batch * esize(p.cprec)))) return rc;
+    return CFFT_OK;
+}
+
+static int plan_fft_core(Ctx* ctx, Plan& p, const PlanOptions& o) {
+    if (p.n == 1) {
+        p.kind = Kind::Trivial;
+        return CFFT_OK;
+    }
+    const int E = o.E ? o.E : 16;
+    if (o.force_conv == 0 && is_smooth(p.n)) {
+        if (o.n1 == 0 && line_fits(ctx, p.n, p.prec, E)) return plan_direct(ctx, p, E, o.target_threads);
+        int rc = plan_fourstep_factors(ctx, p.fs, p.n, p.prec, E, o.n1, o.ca, o.cb);
+        if (rc) return rc;
+        p.kind = Kind::FourStep;
+        return CFFT_OK;
+    }
+    return plan_conv(ctx, p, o);
+}
+
+int plan_build(Ctx* ctx, Plan& p, const PlanOptions& o) { return plan_fft_core(ctx, p, o); }
+
+static std::string describe(const Plan& p) {
+    std::ostringstream os;
+    auto rad = [](const LinePlan& l) {
+        std::ostringstream r;
+        r << "[";
+        for (int i = 0; i < l.pt.npass; ++i) r << (i ? "," : "") << l.pt.radix[i];
+        r << "] Tt=" << l.Tt << " E=" << l.E;
+        return r.str();
+    };
+    os << "n=" << p.n << " batch=" << p.batch << " prec=" << (p.prec == Prec::F32 ? "f32" : "f64")
+       << " ";
+    switch (p.kind) {
+        case Kind::Trivial: os << "trivial"; break;
+        case Kind::Direct: os << "direct " << rad(p.line) << " C=" << p.C; break;
+        case Kind::FourStep:
+            os << "fourstep N1=" << p.fs.N1 << " " << rad(p.fs.a) << " CA=" << p.fs.CA
+               << " N2=" << p.fs.N2 << " " << rad(p.fs.b) << " CB=" << p.fs.CB;
+            break;
+        case Kind::ConvSmall:
+            os << (p.mode == ConvMode::Rader ? "rader" : "bluestein") << "-fused L=" << p.L << " "
+               << rad(p.line) << " C=" << p.C;
+            if (p.mode == ConvMode::Rader) os << " g=" << p.g;
+            break;
+        case Kind::ConvLarge:
+            os << (p.mode == ConvMode::Rader ? "rader" : "bluestein") << "-large L=" << p.L
+               << " conv=" << (p.cprec == Prec::F32 ? "f32" : "f64") << " {" << describe(*p.sub)
+               << "}";
+            if (p.mode == ConvMode::Rader) os << " g=" << p.g;
+            break;
+        default: os << "uneven"; break;
+    }
+    return os.str();
+}
+
+// ------------------------------------------------------------------------------- execution
+static int run_fft(const Plan& p, const void* in, void* out, int sgn, double scale,
+                   ConvEpilogue epi, hipStream_t st);
+
+static int run_fourstep(const Plan& p, const void* in, void* mid, void* out, int64_t idist,
+                        int64_t mdist, int64_t odist, int sgn, double scale, ConvEpilogue epi,
+                        hipStream_t st) {
+    const FourStepPlan& fs = p.fs;
+    FourStepAArgs a{};
+    a.N1 = fs.N1;
+    a.N2 = fs.N2;
+    a.C = fs.CA;
+    a.Tt = fs.a.Tt;
+    a.Ls = fs.a.Ls;
+    a.E = fs.a.E;
+    a.batch = p.batch;
+    a.idist = idist;
+    a.odist = mdist;
+    a.sgn_in = sgn;
+    a.shift = fs.shift;
+    a.mask = fs.mask;
+    a.T1 = fs.T1.p;
+    a.T2 = fs.T2.p;
+    CFFT_HIP(launch_fourstep_a(p.prec, in, mid, a, fs.a.pt, fs.a.dtw.p, st));
+    FourStepBArgs b{};
+    b.N1 = fs.N1;
+    b.N2 = fs.N2;
+    b.C = fs.CB;
+    b.Tt = fs.b.Tt;
+    b.Ls = fs.b.Ls;
+    b.E = fs.b.E;
+    b.batch = p.batch;
+    b.idist = mdist;
+    b.odist = odist;
+    b.sgn_out = sgn;
+    b.scale = scale;
+    b.epi = epi;
+    CFFT_HIP(launch_fourstep_b(p.prec, mid, out, b, fs.b.pt, fs.b.dtw.p, st));
+    return CFFT_OK;
+}
+
+static int run_fft(const Plan& p, const void* in, void* out, int sgn, double scale,
+                   ConvEpilogue epi, hipStream_t st) {
+    switch (p.kind) {
+        case Kind::Trivial: {
+            const int64_t es = esize(p.prec);
+            if (p.batch == 0) return CFFT_OK;
+            if (in == out && p.idist == p.odist) return CFFT_OK;
+            CFFT_HIP(hipMemcpy2DAsync(out, p.odist * es, in, p.idist * es, es, p.batch,
+                                      hipMemcpyDeviceToDevice, st));
+            return CFFT_OK;
+        }
+        case Kind::Direct: {
+            RowsArgs a{};
+            a.N = (int)p.n;
+            a.C = p.C;
+            a.Tt = p.line.Tt;
+            a.Ls = p.line.Ls;
+            a.E = p.line.E;
+            a.nlines = p.batch;
+            a.idist = p.idist;
+            a.odist = p.odist;
+            a.sgn_in = sgn;
+            a.sgn_out = sgn;
+            a.scale = scale;
+            a.epi = epi;
+            CFFT_HIP(launch_rows(p.prec, in, out, a, p.line.pt, p.line.dtw.p, st));
+            return CFFT_OK;
+        }
+        case Kind::FourStep: {
+            if (in == out) {
+                std::lock_guard<std::mutex> g(p.scratch_mu);
+                const size_t need = (size_t)p.batch * p.n * esize(p.prec);
+                if (p.scratch.bytes < need) {
+                    int rc = p.scratch.alloc(need);
+                    if (rc) return rc;
+                }
+                return run_fourstep(p, in, p.scratch.p, out, p.idist, p.n, p.odist, sgn, scale,
+                                    epi, st);
+            }
+            return run_fourstep(p, in, out, out, p.idist, p.odist, p.odist, sgn, scale, epi, st);
+        }
+        case Kind::ConvSmall: {
+            ConvSmallArgs a{};
+            a.n = (int)p.n;
+            a.L = (int)p.L;
+            a.C = p.C;
+            a.Tt = p.line.Tt;
+            a.LsX = p.LsX;
+            a.LsA = p.LsA;
+            a.E = p.line.E;
+            a.mode = (int)p.mode;
+            a.nlines = p.batch;
+            a.idist = p.idist;
+            a.odist = p.odist;
+            a.sgn_in = sgn;
+            a.sgn_out = sgn;
+            a.scale = scale;
+            a.perm = (const int*)p.perm.p;
+            a.iperm = (const int*)p.iperm.p;
+            a.chirp = p.chirp.p;
+            a.K = p.K.p;
+            CFFT_HIP(launch_conv_small(p.prec, in, out, a, p.line.pt, p.line.dtw.p, st));
+            return CFFT_OK;
+        }
+        case Kind::ConvLarge: {
+            ConvGatherArgs g{};
+            g.n = p.n;
+            g.L = p.L;
+            g.nlines = p.batch;
+            g.idist = p.idist;
+            g.wdist = p.L;
+            g.mode = (int)p.mode;
+            g.sgn_in = sgn;
+            g.perm = (const int*)p.perm.p;
+            g.chirp = p.chirp.p;
+            CFFT_HIP(launch_conv_gather(p.prec, p.cprec, in, p.w1.p, g, st));
+            ConvEpilogue e{p.K.p, p.dc.p};
+            int rc = run_fft(*p.sub, p.w1.p, p.w2.p, 1, 1.0, e, st);
+            if (rc) return rc;
+            rc = run_fft(*p.sub, p.w2.p, p.w1.p, 1, 1.0, ConvEpilogue{nullptr, nullptr}, st);
+            if (rc) return rc;
+            ConvScatterArgs s{};
+            s.n = p.n;
+            s.L = p.L;
+            s.nlines = p.batch;
+            s.odist = p.odist;
+            s.idist = p.idist;
+            s.wdist = p.L;
+            s.mode = (int)p.mode;
+            s.sgn_in = sgn;
+            s.sgn_out = sgn;
+            s.invL = 1.0 / (double)p.L;
+            s.scale = scale;
+            s.iperm = (const int*)p.iperm.p;
+            s.chirp = p.chirp.p;
+            s.dc = p.dc.p;
+            s.x = in;
+            CFFT_HIP(launch_conv_scatter(p.prec, p.cprec, p.w1.p, out, s, st));
+            return CFFT_OK;
+        }
+        default: break;
+    }
+    return fail(CFFT_ERR_UNSUPPORTED, "plan kind not executable");
+}
+
+int plan_exec(const Plan& p, const void* in, void* out, int direction, int normalize,
+              hipStream_t st) {
+    if (p.kind == Kind::Uneven) return uneven_exec(p, in, out, direction, normalize, st);
+    if (p.batch == 0) return CFFT_OK;
+    if (!in || !out) return fail(CFFT_ERR_INVALID, "null buffer");
+    if (p.kind == Kind::ConvLarge && in == out)
+        return fail(CFFT_ERR_INVALID, "in-place execution is not supported for this length");
+    const int sgn = direction > 0 ? -1 : 1;
+    const double scale = normalize ? 1.0 / (double)p.n : 1.0;
+    return run_fft(p, in, out, sgn, scale, ConvEpilogue{nullptr, nullptr}, st);
+}
+
+size_t plan_device_bytes(const Plan& p) {
+    size_t b = p.line.dtw.bytes + p.fs.a.dtw.bytes + p.fs.b.dtw.bytes + p.fs.T1.bytes +
+               p.fs.T2.bytes + p.scratch.bytes + p.perm.bytes + p.iperm.bytes + p.chirp.bytes +
+               p.K.bytes + p.dc.bytes + p.w1.bytes + p.w2.bytes;
+    if (p.sub) b += plan_device_bytes(*p.sub);
+    for (const auto& s : p.parts) b += plan_device_bytes(*s);
+    b += p.uneven_bytes;
+    return b;
+}
+
+int plan_create(Ctx* ctx, int64_t n, int64_t batch, int64_t idist, int64_t odist, int dtype,
+                int flags, const PlanOptions& opts, std::shared_ptr<Plan>* out) {
+    if (n < 1) return fail(CFFT_ERR_INVALID, "n must be >= 1");
+    if (batch < 0) return fail(CFFT_ERR_INVALID, "batch must be >= 0");
+    if (dtype != CFFT_C32 && dtype != CFFT_C64) return fail(CFFT_ERR_INVALID, "bad dtype");
+    if (flags < CFFT_NO_FLAG || flags > CFFT_ENCHANT) return fail(CFFT_ERR_INVALID, "bad flags");
+    if (idist < n || odist < n) return fail(CFFT_ERR_INVALID, "idist/odist must be >= n");
+    if (n > ((int64_t)1 << 30)) return fail(CFFT_ERR_UNSUPPORTED, "n above 2^30");
+    auto p = std::make_shared<Plan>();
+    p->ctx = ctx;
+    p->n = n;
+    p->batch = batch;
+    p->idist = idist;
+    p->odist = odist;
+    p->dtype = dtype;
+    p->flags = flags;
+    p->prec = dtype == CFFT_C32 ? Prec::F32 : Prec::F64;
+    CFFT_HIP(hipSetDevice(ctx->device));
+    int rc;
+    if (flags == CFFT_NO_FLAG || opts.explicit_opts) {
+        rc = plan_fft_core(ctx, *p, opts);
+    } else {
+        rc = autotune(ctx, *p, flags, opts);
+    }
+    if (rc) return rc;
+    if (p->desc.empty()) p->desc = describe(*p);
+    *out = p;
+    return CFFT_OK;
+}
+
+std::string plan_description(const Plan& p) { return p.desc.empty() ? describe(p) : p.desc; }
+
+// ------------------------------------------------------------------------------- context
+int ctx_create(int device, Ctx** out) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+        (void)hipGetLastError();
+        return fail(CFFT_ERR_HIP, "no HIP device available");
+    }
+    if (device < 0 || device >= count) return fail(CFFT_ERR_INVALID, "bad device index");
+    CFFT_HIP(hipSetDevice(device));
+    std::unique_ptr<Ctx> c(new Ctx());
+    c->device = device;
+    int v = 0;
+    CFFT_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+    c->lds_max = std::min<size_t>((size_t)v, 160 * 1024);
+    CFFT_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, device));
+    c->num_cu = v;
+    CFFT_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    *out = c.release();
+    return CFFT_OK;
+}
+
+void ctx_destroy(Ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    c->cache.clear();
+    c->stage_in.reset();
+    c->stage_out.reset();
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+}  // namespace cfft
+
+// =============================================================================== C ABI
+using namespace cfft;
+
+struct cfft_ctx_s {
+    Ctx* c;
+};
+struct cfft_plan_s {
+    std::shared_ptr<Plan> p;
+    cfft_ctx_s* ctx;
+};
+
+extern "C" {
+
+const char* cfft_last_error(void) { return g_err.c_str(); }
+
+const char* cfft_version(void) { return "catabrafft-hip 0.1 (gfx950)"; }
+
+int cfft_init(int device, cfft_ctx* ctx) {
+    if (!ctx) return fail(CFFT_ERR_INVALID, "null ctx pointer");
+    Ctx* c = nullptr;
+    int rc = ctx_create(device, &c);
+    if (rc) return rc;
+    *ctx = new cfft_ctx_s{c};
+    g_err.clear();
+    return CFFT_OK;
+}
+
+int cfft_finalize(cfft_ctx ctx) {
+    if (!ctx) return fail(CFFT_ERR_INVALID, "null ctx");
+    ctx_destroy(ctx->c);
+    delete ctx;
+    return CFFT_OK;
+}
+
+int cfft_plan_1d(cfft_ctx ctx, int64_t n, int64_t batch, int64_t idist, int64_t odist, int dtype,
+                 int flags, cfft_plan* plan) {
+    if (!ctx || !plan) return fail(CFFT_ERR_INVALID, "null argument");
+    Ctx* c = ctx->c;
+    PlanKey key{n, batch, idist, odist, dtype, flags};
+    std::shared_ptr<Plan> p;
+    {
+        std::lock_guard<std::mutex> g(c->mu);
+        auto it = c->cache.find(key);
+        if (it != c->cache.end()) p = it->second;
+    }
+    if (!p) {
+        PlanOptions o;
+        int rc = plan_create(c, n, batch, idist, odist, dtype, flags, o, &p);
+        if (rc) return rc;
+        std::lock_guard<std::mutex> g(c->mu);
+        c->cache[key] = p;
+    }
+    *plan = new cfft_plan_s{p, ctx};
+    return CFFT_OK;
+}
+
+int cfft_plan_many_uneven(cfft_ctx ctx, int64_t count, const int64_t* lengths,
+                          const int64_t* offsets, int dtype, int flags, cfft_plan* plan) {
+    if (!ctx || !plan || (count > 0 && (!lengths || !offsets)))
+        return fail(CFFT_ERR_INVALID, "null argument");
+    std::shared_ptr<Plan> p;
+    int rc = uneven_create(ctx->c, count, lengths, offsets, dtype, flags, &p);
+    if (rc) return rc;
+    *plan = new cfft_plan_s{p, ctx};
+    return CFFT_OK;
+}
+
+int cfft_exec(cfft_plan plan, const void* in, void* out, int direction, int normalize,
+              void* stream) {
+    if (!plan) return fail(CFFT_ERR_INVALID, "null plan");
+    if (direction != CFFT_FORWARD && direction != CFFT_BACKWARD)
+        return fail(CFFT_ERR_INVALID, "direction must be -1 or +1");
+    CFFT_HIP(hipSetDevice(plan->p->ctx->device));
+    return plan_exec(*plan->p, in, out, direction, normalize, (hipStream_t)stream);
+}
+
+int cfft_exec_host(cfft_plan plan, const void* in, void* out, int direction, int normalize) {
+    if (!plan || !in || !out) return fail(CFFT_ERR_INVALID, "null argument");
+    if (direction != CFFT_FORWARD && direction != CFFT_BACKWARD)
+        return fail(CFFT_ERR_INVALID, "direction must be -1 or +1");
+    const Plan& p = *plan->p;
+    Ctx* c = p.ctx;
+    CFFT_HIP(hipSetDevice(c->device));
+    const int64_t es = esize(p.prec);
+    size_t ib, ob;
+    if (p.kind == Kind::Uneven) {
+        ib = ob = (size_t)p.uneven_span * es;
+    } else {
+        if (p.batch == 0) return CFFT_OK;
+        ib = (size_t)((p.batch - 1) * p.idist + p.n) * es;
+        ob = (size_t)((p.batch - 1) * p.odist + p.n) * es;
+    }
+    std::lock_guard<std::mutex> g(c->stage_mu);
+    int rc;
+    if (c->stage_in.bytes < ib && (rc = c->stage_in.alloc(ib))) return rc;
+    if (c->stage_out.bytes < ob && (rc = c->stage_out.alloc(ob))) return rc;
+    CFFT_HIP(hipMemcpyAsync(c->stage_in.p, in, ib, hipMemcpyHostToDevice, c->stream));
+    if (p.kind == Kind::Uneven)  // untouched gaps come back as they went in
+        CFFT_HIP(hipMemcpyAsync(c->stage_out.p, in, ob, hipMemcpyHostToDevice, c->stream));
+    rc = plan_exec(p, c->stage_in.p, c->stage_out.p, direction, normalize, c->stream);
+    if (rc) return rc;
+    CFFT_HIP(hipMemcpyAsync(out, c->stage_out.p, ob, hipMemcpyDeviceToHost, c->stream));
+    CFFT_HIP(hipStreamSynchronize(c->stream));
+    return CFFT_OK;
+}
+
+int cfft_plan_device_bytes(cfft_plan plan, int64_t* bytes) {
+    if (!plan || !bytes) return fail(CFFT_ERR_INVALID, "null argument");
+    *bytes = (int64_t)plan_device_bytes(*plan->p);
+    return CFFT_OK;
+}
+
+int cfft_plan_describe(cfft_plan plan, char* buf, int64_t buflen) {
+    if (!plan || !buf || buflen <= 0) return fail(CFFT_ERR_INVALID, "null argument");
+    const std::string d = plan_description(*plan->p);
+    std::snprintf(buf, (size_t)buflen, "%s", d.c_str());
+    return CFFT_OK;
+}
+
+int cfft_plan_destroy(cfft_plan plan) {
+    if (!plan) return fail(CFFT_ERR_INVALID, "null plan");
+    (void)hipSetDevice(plan->p->ctx->device);
+    delete plan;
+    return CFFT_OK;
+}
+
+int cfft_empty_cache(cfft_ctx ctx) {
+    if (!ctx) return fail(CFFT_ERR_INVALID, "null ctx");
+    Ctx* c = ctx->c;
+    (void)hipSetDevice(c->device);
+    {
+        std::lock_guard<std::mutex> g(c->mu);
+        c->cache.clear();
+    }
+    std::lock_guard<std::mutex> g(c->stage_mu);
+    c->stage_in.reset();
+    c->stage_out.reset();
+    return CFFT_OK;
+}
+
+int cfft_malloc(cfft_ctx ctx, int64_t bytes, void** ptr) {
+    if (!ctx || !ptr || bytes < 0) return fail(CFFT_ERR_INVALID, "bad argument");
+    CFFT_HIP(hipSetDevice(ctx->c->device));
+    if (hipMalloc(ptr, (size_t)bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        *ptr = nullptr;
+        return fail(CFFT_ERR_NOMEM, "hipMalloc failed");
+    }
+    return CFFT_OK;
+}
+
+int cfft_free(cfft_ctx ctx, void* ptr) {
+    if (!ctx) return fail(CFFT_ERR_INVALID, "null ctx");
+    CFFT_HIP(hipSetDevice(ctx->c->device));
+    CFFT_HIP(hipFree(ptr));
+    return CFFT_OK;
+}
+
+int cfft_memcpy_h2d(cfft_ctx ctx, void* dst, const void* src, int64_t bytes) {
+    if (!ctx || bytes < 0) return fail(CFFT_ERR_INVALID, "bad argument");
+    CFFT_HIP(hipSetDevice(ctx->c->device));
+    CFFT_HIP(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyHostToDevice));
+    return CFFT_OK;
+}
+
+int cfft_memcpy_d2h(cfft_ctx ctx, void* dst, const void* src, int64_t bytes) {
+    if (!ctx || bytes < 0) return fail(CFFT_ERR_INVALID, "bad argument");
+    CFFT_HIP(hipSetDevice(ctx->c->device));
+    CFFT_HIP(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost));
+    return CFFT_OK;
+}
+
+int cfft_synchronize(cfft_ctx ctx) {
+    if (!ctx) return fail(CFFT_ERR_INVALID, "null ctx");
+    CFFT_HIP(hipSetDevice(ctx->c->device));
+    CFFT_HIP(hipDeviceSynchronize());
+    return CFFT_OK;
+}
+
+}  // extern "C"

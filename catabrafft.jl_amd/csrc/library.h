@@ -1,0 +1,128 @@
+// library.h -- host-side types shared by the library's translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "kernels.h"
+
+namespace cfft {
+
+int fail(int code, const std::string& msg);
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf();
+    int alloc(size_t nbytes);
+    void reset();
+};
+
+// One line FFT of length N: radix sequence, threads per line, LDS line stride, twiddles.
+struct LinePlan {
+    int N = 0, E = 16, Tt = 1, Ls = 1;
+    PassTable pt{};
+    DevBuf dtw;
+};
+
+struct FourStepPlan {
+    int N1 = 0, N2 = 0, CA = 1, CB = 1;
+    LinePlan a, b;
+    DevBuf T1, T2;
+    int shift = 0, mask = 0;
+};
+
+enum class Kind : int { Trivial, Direct, FourStep, ConvSmall, ConvLarge, Uneven };
+
+struct Ctx;
+
+struct Plan {
+    Ctx* ctx = nullptr;
+    Kind kind = Kind::Trivial;
+    int64_t n = 0, batch = 0, idist = 0, odist = 0;
+    int dtype = 0, flags = 0;
+    Prec prec = Prec::F32;
+    // direct and fused convolution: the line FFT and lines per workgroup
+    LinePlan line;
+    int C = 1;
+    // four-step
+    FourStepPlan fs;
+    mutable std::mutex scratch_mu;
+    mutable DevBuf scratch;
+    // convolution (Rader / Bluestein)
+    ConvMode mode = ConvMode::Rader;
+    int64_t L = 0, g = 0;
+    int LsX = 0, LsA = 0;
+    Prec cprec = Prec::F32;
+    DevBuf perm, iperm, chirp, K, dc, w1, w2;
+    std::unique_ptr<Plan> sub;
+    // uneven batch
+    std::vector<std::unique_ptr<Plan>> parts;
+    std::shared_ptr<void> uneven;  // opaque state owned by uneven.cpp
+    size_t uneven_bytes = 0;
+    int64_t uneven_span = 0;       // elements covered by the uneven layout
+    std::string desc;
+};
+
+// Planner knobs; the autotuner enumerates them, NO_FLAG uses the defaults.
+struct PlanOptions {
+    int E = 0;                 // per-thread complex capacity (8 or 16), 0 = default
+    int target_threads = 256;  // threads per workgroup the rows kernels aim for
+    int64_t n1 = 0;            // four-step split (0 = balanced / direct when it fits)
+    int ca = 0, cb = 0;        // four-step column tiles (0 = default)
+    int force_conv = 0;
+    bool explicit_opts = false;
+};
+
+using PlanKey = std::tuple<int64_t, int64_t, int64_t, int64_t, int, int>;
+
+struct Ctx {
+    int device = 0;
+    size_t lds_max = 64 * 1024;
+    size_t lds_rows_budget = 64 * 1024;
+    int num_cu = 256;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    std::map<PlanKey, std::shared_ptr<Plan>> cache;
+    std::mutex stage_mu;
+    DevBuf stage_in, stage_out;
+};
+
+int ctx_create(int device, Ctx** out);
+void ctx_destroy(Ctx* c);
+
+bool is_smooth(int64_t n);
+int64_t next_smooth(int64_t m);
+std::vector<int> line_radices(int64_t n);
+int threads_for(const std::vector<int>& rad, int64_t N, int E);
+int line_stride_for(Prec p, int64_t N);
+int make_line(LinePlan& lp, int64_t N, Prec pr, int E, int max_threads);
+
+// Plans p (fields n, batch, idist, odist, prec already set) with the given knobs.
+int plan_build(Ctx* ctx, Plan& p, const PlanOptions& o);
+int plan_create(Ctx* ctx, int64_t n, int64_t batch, int64_t idist, int64_t odist, int dtype,
+                int flags, const PlanOptions& opts, std::shared_ptr<Plan>* out);
+int plan_exec(const Plan& p, const void* in, void* out, int direction, int normalize,
+              hipStream_t st);
+size_t plan_device_bytes(const Plan& p);
+std::string plan_description(const Plan& p);
+
+// autotune.cpp: MEASURE / ENCHANT planning (times candidate PlanOptions on the device).
+int autotune(Ctx* ctx, Plan& p, int flags, const PlanOptions& base);
+
+// uneven.cpp: many transforms of different lengths in one plan.
+int uneven_create(Ctx* ctx, int64_t count, const int64_t* lengths, const int64_t* offsets,
+                  int dtype, int flags, std::shared_ptr<Plan>* out);
+int uneven_exec(const Plan& p, const void* in, void* out, int direction, int normalize,
+                hipStream_t st);
+
+}  // namespace cfft

@@ -33,8 +33,8 @@ from ._lib import (BACKWARD, C32, C64, FORWARD, CatabraFFTArgumentError, Catabra
 NO_FLAG, MEASURE, ENCHANT = 0, 1, 2
 
 __all__ = ["fft", "ifft", "bfft", "plan_fft", "plan_bfft", "plan_ifft", "mul", "empty_cache",
-           "Spell", "Plan", "UnevenPlan", "Context", "context", "NO_FLAG", "MEASURE", "ENCHANT",
-           "CatabraFFTError", "CatabraFFTArgumentError"]
+           "Spell", "Plan", "UnevenPlan", "Server", "Context", "context", "NO_FLAG", "MEASURE",
+           "ENCHANT", "CatabraFFTError", "CatabraFFTArgumentError"]
 
 
 # ------------------------------------------------------------------------------ context
@@ -185,6 +185,73 @@ class UnevenPlan:
         stream = t.cuda.current_stream(x.device).cuda_stream
         self.exec_ptr(x.data_ptr(), out.data_ptr(), direction, normalize, stream)
         return out
+
+
+class Server:
+    """The continuous-batching serving loop (cfft_server_*).
+
+    submit() takes a 1-D or 2-D (batch, n) complex array: a numpy array (host; staged by the
+    server) or a GPU torch tensor (device-resident).  The output buffer is allocated here unless
+    given; it is valid once wait(ticket) returns.
+    """
+
+    def __init__(self, hbm_cap_bytes: int = 200 * 10 ** 9, nstreams: int = 4, device: int = 0):
+        self.ctx = context(device)
+        h = ctypes.c_void_p()
+        check(lib().cfft_server_create(self.ctx.h, int(hbm_cap_bytes), int(nstreams),
+                                       ctypes.byref(h)))
+        self.h = h
+        self._live: dict[int, tuple] = {}
+
+    def submit(self, x, out=None, direction: int = FORWARD, normalize: bool = False) -> int:
+        n, batch = _shape(x)
+        if _is_torch(x):
+            code = _torch_dtype_code(x)
+            if not x.is_contiguous():
+                x = x.contiguous()
+            if out is None:
+                out = _lib.torch.empty_like(x)
+            pin, pout, dev = x.data_ptr(), out.data_ptr(), 1
+        else:
+            x = np.ascontiguousarray(x)
+            code = _dtype_code(x.dtype)
+            if out is None:
+                out = np.empty_like(x)
+            pin, pout, dev = x.ctypes.data, out.ctypes.data, 0
+        t = ctypes.c_int64()
+        check(lib().cfft_server_submit(self.h, n, batch, code, direction, int(normalize),
+                                       ctypes.c_void_p(pin), ctypes.c_void_p(pout), dev,
+                                       ctypes.byref(t)))
+        self._live[t.value] = (x, out)
+        return t.value
+
+    def poll(self, ticket: int) -> bool:
+        rc = lib().cfft_server_poll(self.h, ticket)
+        if rc == _lib.CFFT_ERR_NOTREADY:
+            return False
+        check(rc)
+        return True
+
+    def wait(self, ticket: int):
+        check(lib().cfft_server_wait(self.h, ticket))
+        return self._live.pop(ticket)[1]
+
+    def stats(self) -> dict:
+        v = [ctypes.c_double() for _ in range(4)]
+        check(lib().cfft_server_stats(self.h, *[ctypes.byref(a) for a in v]))
+        return {"p50_s": v[0].value, "p99_s": v[1].value, "transforms_per_s": v[2].value,
+                "gflops": v[3].value}
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            check(lib().cfft_server_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ------------------------------------------------------------------------------ user API

@@ -48,6 +48,26 @@ inline cld omega(int64_t m, int64_t n) {
     return cld(c, -s);
 }
 
+// w_n^m for any integer m from two tables of ~sqrt(n) long-double entries:
+// w_n^m = t1[m mod 2^shift] * t2[m >> shift] (one extra long-double rounding, ~1e-19).
+struct OmegaTable {
+    int64_t n = 1;
+    int shift = 0;
+    std::vector<cld> t1, t2;
+    explicit OmegaTable(int64_t n_) : n(n_) {
+        while (((int64_t)1 << (2 * shift)) < n) ++shift;
+        t1.resize((size_t)1 << shift);
+        t2.resize((size_t)((n >> shift) + 1));
+        for (size_t i = 0; i < t1.size(); ++i) t1[i] = omega((int64_t)i, n);
+        for (size_t i = 0; i < t2.size(); ++i) t2[i] = omega((int64_t)i << shift, n);
+    }
+    cld operator()(int64_t m) const {
+        m %= n;
+        if (m < 0) m += n;
+        return t1[m & (((int64_t)1 << shift) - 1)] * t2[m >> shift];
+    }
+};
+
 inline bool is_prime(int64_t n) {
     if (n < 2) return false;
     if (n < 4) return true;
@@ -108,6 +128,7 @@ inline bool host_fft_ld(std::vector<cld>& x) {
     cld* dst = y.data();
     int64_t s = 1;
     std::vector<cld> wR;
+    const OmegaTable om(n);  // w_g^{pk} = w_n^{s p k}
     for (int R : rad) {
         const int64_t g = n / s, n1 = g / R;
         wR.resize((size_t)R * R);
@@ -116,7 +137,7 @@ inline bool host_fft_ld(std::vector<cld>& x) {
         cld v[16], o[16];
         for (int64_t p = 0; p < n1; ++p) {
             cld tw[16];
-            for (int k = 0; k < R; ++k) tw[k] = omega(p * k, g);
+            for (int k = 0; k < R; ++k) tw[k] = om(s * p * k);
             for (int64_t q = 0; q < s; ++q) {
                 for (int j = 0; j < R; ++j) v[j] = src[q + s * (p + j * n1)];
                 for (int k = 0; k < R; ++k) {

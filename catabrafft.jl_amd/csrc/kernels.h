@@ -11,10 +11,10 @@ namespace cfft {
 enum class Prec : int { F32 = 0, F64 = 1 };
 
 // Epilogue applied by the last store of a line FFT that is the forward half of a convolution:
-// v -> conj(v * K[e]) and dc[line] = v when e == 0 (Rader needs sum(x) = x0 + A^[0]).
+// v -> conj(v * K[e]) and slot[0] = v when e == 0 (Rader needs sum(x) = x0 + A^[0]).
 struct ConvEpilogue {
     const void* K;   // kernel spectrum, length = line length, same precision as the line
-    void* dc;        // one complex per line, or nullptr
+    void* dc;        // two complex slots per line {A^[0], x0}, or nullptr
 };
 
 // Batched contiguous lines: line b reads in[b*idist + e], writes out[b*odist + e].
@@ -72,6 +72,7 @@ struct ConvGatherArgs {
     int mode, sgn_in;
     const int* perm;
     const void* chirp;     // Tc precision
+    void* dc;              // per line slots {A^[0], x0} (Tc precision); the gather writes x0
 };
 struct ConvScatterArgs {
     int64_t n, L, nlines, odist, idist, wdist;
@@ -80,8 +81,7 @@ struct ConvScatterArgs {
     double scale;          // output scale (1/n for ifft)
     const int* iperm;
     const void* chirp;     // Tc precision
-    const void* dc;        // per line A^[0] (Tc precision), Rader only
-    const void* x;         // original input (Tio), Rader reads x0 from it
+    const void* dc;        // per line slots {A^[0], x0} (Tc precision)
 };
 
 hipError_t launch_rows(Prec p, const void* in, void* out, const RowsArgs& a, const PassTable& pt,

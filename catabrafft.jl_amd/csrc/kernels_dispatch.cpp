@@ -1,6 +1,7 @@
 // kernels_dispatch.cpp -- launch_* entry points of kernels.h, dispatched on precision.
 #include "kernels.h"
 #include "kernels_prec.h"
+#include "multi.h"
 
 namespace cfft {
 
@@ -29,6 +30,19 @@ hipError_t launch_conv_scatter(Prec pio, Prec pc, const void* w, void* y,
                                const ConvScatterArgs& a, hipStream_t st) {
     if (pio == Prec::F32) return pc == Prec::F32 ? scatter_f32_f32(w, y, a, st) : scatter_f32_f64(w, y, a, st);
     return scatter_f64_f64(w, y, a, st);
+}
+
+hipError_t launch_multi(const MultiLaunch& L, const MultiArgs& ma, hipStream_t st) {
+    if (L.nitems <= 0) return hipSuccess;
+    const MultiKernel k = (MultiKernel)L.kernel;
+    if (k == MultiKernel::Gather || k == MultiKernel::Scatter) {
+        if (L.prec == (int)Prec::F32)
+            return L.cprec == (int)Prec::F32 ? gs_multi_f32_f32(L, ma, st)
+                                             : gs_multi_f32_f64(L, ma, st);
+        return gs_multi_f64_f64(L, ma, st);
+    }
+    return L.prec == (int)Prec::F32 ? multi_f32(L, ma, st)
+                                    : multi_f64(L, ma, st);
 }
 
 }  // namespace cfft

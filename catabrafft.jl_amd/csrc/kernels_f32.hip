@@ -27,4 +27,10 @@ hipError_t gather_f32_f32(const void* x, void* w, const ConvGatherArgs& a, hipSt
 hipError_t scatter_f32_f32(const void* w, void* y, const ConvScatterArgs& a, hipStream_t st) {
     return scatter_t<float, float>(w, y, a, st);
 }
+hipError_t multi_f32(const MultiLaunch& L, const MultiArgs& ma, hipStream_t st) {
+    return multi_t<float>(L, ma, st);
+}
+hipError_t gs_multi_f32_f32(const MultiLaunch& L, const MultiArgs& ma, hipStream_t st) {
+    return gs_multi_t<float, float>(L, ma, st);
+}
 }  // namespace cfft

@@ -34,4 +34,13 @@ hipError_t scatter_f32_f64(const void* w, void* y, const ConvScatterArgs& a, hip
 hipError_t scatter_f64_f64(const void* w, void* y, const ConvScatterArgs& a, hipStream_t st) {
     return scatter_t<double, double>(w, y, a, st);
 }
+hipError_t multi_f64(const MultiLaunch& L, const MultiArgs& ma, hipStream_t st) {
+    return multi_t<double>(L, ma, st);
+}
+hipError_t gs_multi_f32_f64(const MultiLaunch& L, const MultiArgs& ma, hipStream_t st) {
+    return gs_multi_t<float, double>(L, ma, st);
+}
+hipError_t gs_multi_f64_f64(const MultiLaunch& L, const MultiArgs& ma, hipStream_t st) {
+    return gs_multi_t<double, double>(L, ma, st);
+}
 }  // namespace cfft

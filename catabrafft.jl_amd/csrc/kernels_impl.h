@@ -1,18 +1,22 @@
 // kernels_impl.h -- the HIP kernels of the library and their templated launchers (gfx950).
 // Instantiated per precision by kernels_f32.hip and kernels_f64.hip.
 //
-//   k_rows         batched contiguous line FFTs, one HBM read + one HBM write per element
-//                  (radix_exec.jl's pass sequence for a whole transform, in one launch)
-//   k_fs_a/k_fs_b  the two passes of the four-step n = N1*N2 (mixed_radix.jl:44-78), twiddle
-//                  w_n^{j2 k1} fused into pass A's store, transpose fused into pass B's store
-//   k_conv_small   Rader (prime.jl:5-82) or Bluestein fused in LDS
-//   k_conv_gather / k_conv_scatter
-//                  the permutation / chirp ends of a large convolution whose FFTs run on the
-//                  kernels above
+//   rows        batched contiguous line FFTs, one HBM read + one HBM write per element
+//               (radix_exec.jl's whole pass sequence in one launch)
+//   fs_a/fs_b   the two passes of the four-step n = N1*N2 (mixed_radix.jl:44-78), twiddle
+//               w_n^{j2 k1} fused into pass A's store, transpose fused into pass B's store
+//   conv        Rader (prime.jl:5-82) or Bluestein fused in LDS
+//   gather / scatter
+//               the permutation / chirp ends of a large convolution whose FFTs run on the
+//               kernels above
+// Each kernel is a device-function body called by a single-plan __global__ wrapper (arguments
+// in a struct) and by a work-list wrapper (arguments from an MItem / MPlan table), so the two
+// paths execute the same arithmetic for the same transform.
 #pragma once
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
+#include "multi.h"
 
 namespace cfft {
 
@@ -53,28 +57,84 @@ __device__ __forceinline__ LdsLine<T> lds_line(unsigned char* smem, int C, int L
 }
 
 // ------------------------------------------------------------------------------------------
-// k_rows
+// Line addressing policies: where line c of a workgroup's lines lives.
 // ------------------------------------------------------------------------------------------
+template <typename T>
+struct UniformLines {  // line c at in0 + c*idist / out0 + c*odist, slots dc0 + 2c
+    const V_t<T>* in0;
+    V_t<T>* out0;
+    int64_t idist, odist;
+    V_t<T>* dc0;
+    __device__ __forceinline__ const V_t<T>* in(int c) const { return in0 + c * idist; }
+    __device__ __forceinline__ V_t<T>* out(int c) const { return out0 + c * odist; }
+    __device__ __forceinline__ V_t<T>* dc(int c) const { return dc0 ? dc0 + 2 * c : nullptr; }
+};
+
+template <typename T>
+struct ItemLines {  // an MItem's lines: uniform distances or a table of {in, out, dc}
+    uint64_t ib, ob;
+    const MItem* it;
+    __device__ __forceinline__ const uint64_t* entry(int c) const {
+        return reinterpret_cast<const uint64_t*>(it->aux) + 3 * c;
+    }
+    __device__ __forceinline__ const V_t<T>* in(int c) const {
+        if (it->flags & kLineTable)
+            return reinterpret_cast<const V_t<T>*>(mresolve(entry(c)[0], it->flags & kSelInMask, ib, ob));
+        return reinterpret_cast<const V_t<T>*>(mresolve(it->in, it->flags & kSelInMask, ib, ob)) +
+               c * it->idist;
+    }
+    __device__ __forceinline__ V_t<T>* out(int c) const {
+        const int so = (it->flags >> kSelOutShift) & 15;
+        if (it->flags & kLineTable) return reinterpret_cast<V_t<T>*>(mresolve(entry(c)[1], so, ib, ob));
+        return reinterpret_cast<V_t<T>*>(mresolve(it->out, so, ib, ob)) + c * it->odist;
+    }
+    __device__ __forceinline__ V_t<T>* dc(int c) const {
+        if (it->flags & kLineTable) return reinterpret_cast<V_t<T>*>(entry(c)[2]);
+        return it->dc ? reinterpret_cast<V_t<T>*>(it->dc) + 2 * c : nullptr;
+    }
+};
+
+// ------------------------------------------------------------------------------------------
+// rows body: up to C lines of length N (`count` valid).  A line with no passes (N = 1) is a
+// scaled copy.
+// ------------------------------------------------------------------------------------------
+template <typename T, int E, class LINES>
+__device__ __forceinline__ void rows_body(unsigned char* smem, const LINES& lines, int count,
+                                          int N, int C, int Tt, int Ls, const PassTable& pt,
+                                          const V_t<T>* tw, int sgn, double scale,
+                                          const V_t<T>* K) {
+    const int tid = threadIdx.x;
+    const int c = tid / Tt;
+    const int t = tid - c * Tt;
+    const bool active = (c < C) && (c < count);
+    const int li = active ? c : 0;
+    GSrc<T> gs{lines.in(li), (T)sgn};
+    GRowDst<T> gd{lines.out(li), (T)scale, (T)(scale * sgn), K, K ? lines.dc(li) : nullptr};
+    if (pt.npass == 0) {
+        if (active && t == 0) gd.put(0, gs(0));
+        return;
+    }
+    const LdsLine<T> L = lds_line<T>(smem, C, Ls, c < C ? c : 0);
+    run_line<T, E>(pt, N, t, Tt, active, tw, gs, gd, L, true, true);
+}
+
 template <typename T, int E>
 __global__ void __launch_bounds__(1024)
 k_rows(const V_t<T>* __restrict__ in, V_t<T>* __restrict__ out, RowsArgs a, PassTable pt,
        const V_t<T>* __restrict__ tw) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const int tid = threadIdx.x;
-    const int c = tid / a.Tt;
-    const int t = tid - c * a.Tt;
-    const int64_t line = (int64_t)blockIdx.x * a.C + c;
-    const bool active = (c < a.C) && (line < a.nlines);
-    const int64_t li = active ? line : 0;
-    GSrc<T> gs{in + li * a.idist, (T)a.sgn_in};
-    GRowDst<T> gd{out + li * a.odist, (T)a.scale, (T)(a.scale * a.sgn_out),
-                  (const V_t<T>*)a.epi.K, a.epi.dc ? (V_t<T>*)a.epi.dc + li : nullptr};
-    const LdsLine<T> L = lds_line<T>(smem, a.C, a.Ls, c < a.C ? c : 0);
-    run_line<T, E>(pt, a.N, t, a.Tt, active, tw, gs, gd, L, true, true);
+    const int64_t line0 = (int64_t)blockIdx.x * a.C;
+    const int64_t left = a.nlines - line0;
+    const int count = (int)(left < a.C ? left : a.C);
+    UniformLines<T> lines{in + line0 * a.idist, out + line0 * a.odist, a.idist, a.odist,
+                          a.epi.dc ? (V_t<T>*)a.epi.dc + 2 * line0 : nullptr};
+    rows_body<T, E>(smem, lines, count, a.N, a.C, a.Tt, a.Ls, pt, tw, a.sgn_in, a.scale,
+                    (const V_t<T>*)a.epi.K);
 }
 
 // ------------------------------------------------------------------------------------------
-// Four-step pass A.  C is a power of two (cshift = log2 C).
+// Four-step pass A body: columns j20 .. j20+C-1 of one transform (N1 x N2 row-major).
+//   out[j2*N1 + k1] = w_n^{j2 k1} * DFT_N1(in[j1*N2 + j2])[k1]
 // ------------------------------------------------------------------------------------------
 template <typename T>
 struct TwiddleDst {
@@ -90,6 +150,36 @@ struct TwiddleDst {
 };
 
 template <typename T, int E>
+__device__ __forceinline__ void fs_a_body(unsigned char* smem, const V_t<T>* src, V_t<T>* dst,
+                                          int N1, int N2, int C, int Tt, int Ls, int j20,
+                                          int sgn, int shift, int mask, const V_t<T>* T1,
+                                          const V_t<T>* T2, const PassTable& pt,
+                                          const V_t<T>* tw) {
+    const int cshift = __ffs(C) - 1;
+    const T sg = (T)sgn;
+    // Staged load: rows j1 of C contiguous columns (one run of C elements per row).
+    const int total = C * N1;
+    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+        const int c = idx & (C - 1);
+        const int e = idx >> cshift;
+        const int j2 = j20 + c;
+        V_t<T> v = V_t<T>{0, 0};
+        if (j2 < N2) v = src[(int64_t)e * N2 + j2];
+        v.y *= sg;
+        lds_line<T>(smem, C, Ls, c).put(e, v);
+    }
+    __syncthreads();
+    const int tid = threadIdx.x;
+    const int c = tid / Tt;
+    const int t = tid - c * Tt;
+    const int j2 = j20 + c;
+    const bool active = (c < C) && (j2 < N2);
+    const LdsLine<T> L = lds_line<T>(smem, C, Ls, c < C ? c : 0);
+    TwiddleDst<T> d{dst + (int64_t)(active ? j2 : 0) * N1, T1, T2, j2, shift, mask};
+    run_line<T, E>(pt, N1, t, Tt, active, tw, L, d, L, false, true);
+}
+
+template <typename T, int E>
 __global__ void __launch_bounds__(1024)
 k_fs_a(const V_t<T>* __restrict__ in, V_t<T>* __restrict__ out, FourStepAArgs a, PassTable pt,
        const V_t<T>* __restrict__ tw) {
@@ -97,36 +187,55 @@ k_fs_a(const V_t<T>* __restrict__ in, V_t<T>* __restrict__ out, FourStepAArgs a,
     const int tiles = (a.N2 + a.C - 1) / a.C;
     const int64_t bt = blockIdx.x / tiles;
     const int tile = blockIdx.x - (int)(bt * tiles);
-    const int j20 = tile * a.C;
-    const V_t<T>* src = in + bt * a.idist;
-    const int cshift = __ffs(a.C) - 1;
-    const T sg = (T)a.sgn_in;
-    // Staged load: rows j1 of C contiguous columns (one run of C elements per row).
-    const int total = a.C * a.N1;
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-        const int c = idx & (a.C - 1);
-        const int e = idx >> cshift;
-        const int j2 = j20 + c;
-        V_t<T> v = V_t<T>{0, 0};
-        if (j2 < a.N2) v = src[(int64_t)e * a.N2 + j2];
-        v.y *= sg;
-        lds_line<T>(smem, a.C, a.Ls, c).put(e, v);
-    }
-    __syncthreads();
-    const int tid = threadIdx.x;
-    const int c = tid / a.Tt;
-    const int t = tid - c * a.Tt;
-    const int j2 = j20 + c;
-    const bool active = (c < a.C) && (j2 < a.N2);
-    const LdsLine<T> L = lds_line<T>(smem, a.C, a.Ls, c < a.C ? c : 0);
-    TwiddleDst<T> dst{out + bt * a.odist + (int64_t)(active ? j2 : 0) * a.N1,
-                      (const V_t<T>*)a.T1, (const V_t<T>*)a.T2, j2, a.shift, a.mask};
-    run_line<T, E>(pt, a.N1, t, a.Tt, active, tw, L, dst, L, false, true);
+    fs_a_body<T, E>(smem, in + bt * a.idist, out + bt * a.odist, a.N1, a.N2, a.C, a.Tt, a.Ls,
+                    tile * a.C, a.sgn_in, a.shift, a.mask, (const V_t<T>*)a.T1,
+                    (const V_t<T>*)a.T2, pt, tw);
 }
 
 // ------------------------------------------------------------------------------------------
-// Four-step pass B.  Staged load and staged (transposing) store; in == out allowed.
+// Four-step pass B body: lines k10 .. k10+C-1, staged load and staged (transposing) store.
+//   out[k1 + N1*k2] = DFT_N2(in[j2*N1 + k1])[k2];  in == out allowed (same index set).
 // ------------------------------------------------------------------------------------------
+template <typename T, int E>
+__device__ __forceinline__ void fs_b_body(unsigned char* smem, const V_t<T>* src, V_t<T>* dst,
+                                          int N1, int N2, int C, int Tt, int Ls, int k10,
+                                          int sgn, double scale, const V_t<T>* K, V_t<T>* dc,
+                                          const PassTable& pt, const V_t<T>* tw) {
+    const int cshift = __ffs(C) - 1;
+    const int total = C * N2;
+    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+        const int c = idx & (C - 1);
+        const int e = idx >> cshift;
+        const int k1 = k10 + c;
+        V_t<T> v = V_t<T>{0, 0};
+        if (k1 < N1) v = src[(int64_t)e * N1 + k1];
+        lds_line<T>(smem, C, Ls, c).put(e, v);
+    }
+    __syncthreads();
+    const int tid = threadIdx.x;
+    const int c = tid / Tt;
+    const int t = tid - c * Tt;
+    const bool active = (c < C) && (k10 + c < N1);
+    const LdsLine<T> L = lds_line<T>(smem, C, Ls, c < C ? c : 0);
+    run_line<T, E>(pt, N2, t, Tt, active, tw, L, L, L, false, false);
+    // run_line ended with a barrier: the lines' outputs are in LDS.
+    const T sx = (T)scale, sy = (T)(scale * sgn);
+    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+        const int cc = idx & (C - 1);
+        const int e = idx >> cshift;
+        const int k1 = k10 + cc;
+        if (k1 < N1) {
+            V_t<T> v = lds_line<T>(smem, C, Ls, cc)(e);
+            const int64_t o = (int64_t)k1 + (int64_t)N1 * e;
+            if (K) {
+                if (dc && o == 0) *dc = v;
+                v = cconj(cmul(v, K[o]));
+            }
+            dst[o] = V_t<T>{v.x * sx, v.y * sy};
+        }
+    }
+}
+
 template <typename T, int E>
 __global__ void __launch_bounds__(1024)
 k_fs_b(const V_t<T>* in, V_t<T>* out, FourStepBArgs a, PassTable pt,
@@ -135,48 +244,13 @@ k_fs_b(const V_t<T>* in, V_t<T>* out, FourStepBArgs a, PassTable pt,
     const int tiles = (a.N1 + a.C - 1) / a.C;
     const int64_t bt = blockIdx.x / tiles;
     const int tile = blockIdx.x - (int)(bt * tiles);
-    const int k10 = tile * a.C;
-    const V_t<T>* src = in + bt * a.idist;
-    V_t<T>* dst = out + bt * a.odist;
-    const int cshift = __ffs(a.C) - 1;
-    const int total = a.C * a.N2;
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-        const int c = idx & (a.C - 1);
-        const int e = idx >> cshift;
-        const int k1 = k10 + c;
-        V_t<T> v = V_t<T>{0, 0};
-        if (k1 < a.N1) v = src[(int64_t)e * a.N1 + k1];
-        lds_line<T>(smem, a.C, a.Ls, c).put(e, v);
-    }
-    __syncthreads();
-    const int tid = threadIdx.x;
-    const int c = tid / a.Tt;
-    const int t = tid - c * a.Tt;
-    const bool active = (c < a.C) && (k10 + c < a.N1);
-    const LdsLine<T> L = lds_line<T>(smem, a.C, a.Ls, c < a.C ? c : 0);
-    run_line<T, E>(pt, a.N2, t, a.Tt, active, tw, L, L, L, false, false);
-    // run_line ended with a barrier: the lines' outputs are in LDS.
-    const T sx = (T)a.scale, sy = (T)(a.scale * a.sgn_out);
-    const V_t<T>* K = (const V_t<T>*)a.epi.K;
-    V_t<T>* dc = (V_t<T>*)a.epi.dc;
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-        const int cc = idx & (a.C - 1);
-        const int e = idx >> cshift;
-        const int k1 = k10 + cc;
-        if (k1 < a.N1) {
-            V_t<T> v = lds_line<T>(smem, a.C, a.Ls, cc)(e);
-            const int64_t o = (int64_t)k1 + (int64_t)a.N1 * e;
-            if (K) {
-                if (dc && o == 0) dc[bt] = v;
-                v = cconj(cmul(v, K[o]));
-            }
-            dst[o] = V_t<T>{v.x * sx, v.y * sy};
-        }
-    }
+    fs_b_body<T, E>(smem, in + bt * a.idist, out + bt * a.odist, a.N1, a.N2, a.C, a.Tt, a.Ls,
+                    tile * a.C, a.sgn_out, a.scale, (const V_t<T>*)a.epi.K,
+                    a.epi.dc ? (V_t<T>*)a.epi.dc + 2 * bt : nullptr, pt, tw);
 }
 
 // ------------------------------------------------------------------------------------------
-// k_conv_small: Rader / Bluestein, whole transform in LDS.
+// conv body: Rader / Bluestein, whole transform in LDS.
 // LDS: X (input, then output; C lines of LsX) | A (convolution work, C lines of LsA) | dc[C]
 // ------------------------------------------------------------------------------------------
 template <typename T>
@@ -224,122 +298,270 @@ struct ScatterDst {  // inverse result c = conj(v)/L into the output X
     }
 };
 
-template <typename T, int E>
-__global__ void __launch_bounds__(1024)
-k_conv_small(const V_t<T>* __restrict__ in, V_t<T>* __restrict__ out, ConvSmallArgs a,
-             PassTable pt, const V_t<T>* __restrict__ tw) {
-    extern __shared__ __align__(16) unsigned char smem[];
+template <typename T, int E, class LINES>
+__device__ __forceinline__ void conv_body(unsigned char* smem, const LINES& lines, int nl, int n,
+                                          int L, int C, int Tt, int LsX, int LsA, int mode,
+                                          int sgn, double scale, const int* perm,
+                                          const int* iperm, const V_t<T>* chirp,
+                                          const V_t<T>* K, const PassTable& pt,
+                                          const V_t<T>* tw) {
     T* base = reinterpret_cast<T*>(smem);
     T* xre = base;
-    T* xim = xre + a.C * a.LsX;
-    T* are = xim + a.C * a.LsX;
-    T* aim = are + a.C * a.LsA;
-    V_t<T>* sdc = reinterpret_cast<V_t<T>*>(aim + a.C * a.LsA);
-    const T sg = (T)a.sgn_in;
-    // 1. staged, coalesced load of the C input lines (flat over lines x elements)
-    const int64_t line0 = (int64_t)blockIdx.x * a.C;
-    const int nl = (int)((a.nlines - line0) < a.C ? (a.nlines - line0) : a.C);
-    for (int idx = threadIdx.x; idx < nl * a.n; idx += blockDim.x) {
-        const int c = idx / a.n;
-        const int e = idx - c * a.n;
-        V_t<T> v = in[(line0 + c) * a.idist + e];
+    T* xim = xre + C * LsX;
+    T* are = xim + C * LsX;
+    T* aim = are + C * LsA;
+    V_t<T>* sdc = reinterpret_cast<V_t<T>*>(aim + C * LsA);
+    const T sg = (T)sgn;
+    // 1. staged, coalesced load of the nl input lines (flat over lines x elements)
+    for (int idx = threadIdx.x; idx < nl * n; idx += blockDim.x) {
+        const int c = idx / n;
+        const int e = idx - c * n;
+        V_t<T> v = lines.in(c)[e];
         v.y *= sg;
-        LdsLine<T>{xre + c * a.LsX, xim + c * a.LsX}.put(e, v);
+        LdsLine<T>{xre + c * LsX, xim + c * LsX}.put(e, v);
     }
     __syncthreads();
     const int tid = threadIdx.x;
-    const int c = tid / a.Tt;
-    const int t = tid - c * a.Tt;
-    const int64_t line = (int64_t)blockIdx.x * a.C + c;
-    const bool active = (c < a.C) && (line < a.nlines);
-    const int cc = c < a.C ? c : 0;
-    const LdsLine<T> X{xre + cc * a.LsX, xim + cc * a.LsX};
-    const LdsLine<T> A{are + cc * a.LsA, aim + cc * a.LsA};
-    const V_t<T>* chirp = (const V_t<T>*)a.chirp;
+    const int c = tid / Tt;
+    const int t = tid - c * Tt;
+    const bool active = (c < C) && (c < nl);
+    const int cc = c < C ? c : 0;
+    const LdsLine<T> X{xre + cc * LsX, xim + cc * LsX};
+    const LdsLine<T> A{are + cc * LsA, aim + cc * LsA};
     // 2. forward FFT_L of the gathered / chirped sequence; pointwise product with K on the way out
-    GatherSrc<T> gsrc{X, a.perm, chirp, a.n, a.mode};
-    ConvEpiDst<T> edst{A, (const V_t<T>*)a.K, sdc + cc};
-    run_line<T, E>(pt, a.L, t, a.Tt, active, tw, gsrc, edst, A, true, true, true);
+    GatherSrc<T> gsrc{X, perm, chirp, n, mode};
+    ConvEpiDst<T> edst{A, K, sdc + cc};
+    run_line<T, E>(pt, L, t, Tt, active, tw, gsrc, edst, A, true, true, true);
     __syncthreads();
     // 3. second forward FFT_L of the conjugated product = conj of the inverse; scatter into X
     const V_t<T> x0 = X(0);
-    ScatterDst<T> sdst{X, a.iperm, chirp, x0, (T)(1.0 / a.L), a.n, a.mode};
-    run_line<T, E>(pt, a.L, t, a.Tt, active, tw, A, sdst, A, false, true);
+    ScatterDst<T> sdst{X, iperm, chirp, x0, (T)(1.0 / L), n, mode};
+    run_line<T, E>(pt, L, t, Tt, active, tw, A, sdst, A, false, true);
     __syncthreads();
-    if (a.mode == (int)ConvMode::Rader && active && t == 0) {
+    if (mode == (int)ConvMode::Rader && active && t == 0) {
         const V_t<T> d = sdc[cc];
         X.put(0, V_t<T>{x0.x + d.x, x0.y + d.y});
     }
     __syncthreads();
     // 4. staged, coalesced store
-    const T sx = (T)a.scale, sy = (T)(a.scale * a.sgn_out);
-    for (int idx = threadIdx.x; idx < nl * a.n; idx += blockDim.x) {
-        const int c2 = idx / a.n;
-        const int e = idx - c2 * a.n;
-        const V_t<T> v = LdsLine<T>{xre + c2 * a.LsX, xim + c2 * a.LsX}(e);
-        out[(line0 + c2) * a.odist + e] = V_t<T>{v.x * sx, v.y * sy};
+    const T sx = (T)scale, sy = (T)(scale * sgn);
+    for (int idx = threadIdx.x; idx < nl * n; idx += blockDim.x) {
+        const int c2 = idx / n;
+        const int e = idx - c2 * n;
+        const V_t<T> v = LdsLine<T>{xre + c2 * LsX, xim + c2 * LsX}(e);
+        lines.out(c2)[e] = V_t<T>{v.x * sx, v.y * sy};
     }
 }
 
+template <typename T, int E>
+__global__ void __launch_bounds__(1024)
+k_conv_small(const V_t<T>* __restrict__ in, V_t<T>* __restrict__ out, ConvSmallArgs a,
+             PassTable pt, const V_t<T>* __restrict__ tw) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int64_t line0 = (int64_t)blockIdx.x * a.C;
+    const int nl = (int)((a.nlines - line0) < a.C ? (a.nlines - line0) : a.C);
+    UniformLines<T> lines{in + line0 * a.idist, out + line0 * a.odist, a.idist, a.odist, nullptr};
+    conv_body<T, E>(smem, lines, nl, a.n, a.L, a.C, a.Tt, a.LsX, a.LsA, a.mode, a.sgn_in,
+                    a.scale, a.perm, a.iperm, (const V_t<T>*)a.chirp, (const V_t<T>*)a.K, pt, tw);
+}
+
 // ------------------------------------------------------------------------------------------
-// Large convolution gather / scatter (grid: x over elements, y over lines).
+// Large convolution gather / scatter bodies over elements [e0, e1) of one line.
 // ------------------------------------------------------------------------------------------
 template <typename Ti, typename Tc>
-__global__ void __launch_bounds__(256)
-k_conv_gather(const V_t<Ti>* __restrict__ x, V_t<Tc>* __restrict__ w, ConvGatherArgs a) {
-  for (int64_t line = blockIdx.y; line < a.nlines; line += gridDim.y) {
-    const V_t<Ti>* xs = x + line * a.idist;
-    V_t<Tc>* ws = w + line * a.wdist;
-    const V_t<Tc>* chirp = (const V_t<Tc>*)a.chirp;
-    const Tc sg = (Tc)a.sgn_in;
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < a.L;
-         e += (int64_t)gridDim.x * blockDim.x) {
+__device__ __forceinline__ void gather_body(const V_t<Ti>* xs, V_t<Tc>* ws, V_t<Tc>* slots,
+                                            int64_t e0, int64_t e1, int64_t estep, int64_t n,
+                                            int mode, int sgn, const int* perm,
+                                            const V_t<Tc>* chirp) {
+    const Tc sg = (Tc)sgn;
+    for (int64_t e = e0; e < e1; e += estep) {
         V_t<Tc> v = V_t<Tc>{0, 0};
-        if (a.mode == (int)ConvMode::Rader) {
-            if (e < a.n - 1) {
-                const V_t<Ti> u = xs[a.perm[e]];
+        if (mode == (int)ConvMode::Rader) {
+            if (e < n - 1) {
+                const V_t<Ti> u = xs[perm[e]];
                 v = V_t<Tc>{(Tc)u.x, (Tc)u.y * sg};
             }
-        } else if (e < a.n) {
+            if (e == 0) {  // keep x0 for the scatter (the input may be overwritten in place)
+                const V_t<Ti> u0 = xs[0];
+                slots[1] = V_t<Tc>{(Tc)u0.x, (Tc)u0.y * sg};
+            }
+        } else if (e < n) {
             const V_t<Ti> u = xs[e];
             v = cmul(V_t<Tc>{(Tc)u.x, (Tc)u.y * sg}, chirp[e]);
         }
         ws[e] = v;
     }
-  }
 }
 
+// slots = {A^[0], x0} of the line (Rader).
 template <typename Ti, typename Tc>
-__global__ void __launch_bounds__(256)
-k_conv_scatter(const V_t<Tc>* __restrict__ w, V_t<Ti>* __restrict__ y, ConvScatterArgs a) {
-  for (int64_t line = blockIdx.y; line < a.nlines; line += gridDim.y) {
-    const V_t<Tc>* ws = w + line * a.wdist;
-    V_t<Ti>* ys = y + line * a.odist;
-    const V_t<Tc>* chirp = (const V_t<Tc>*)a.chirp;
-    const Tc invL = (Tc)a.invL;
-    const Tc sx = (Tc)a.scale, sy = (Tc)(a.scale * a.sgn_out);
-    if (a.mode == (int)ConvMode::Rader) {
-        const V_t<Ti> u0 = ((const V_t<Ti>*)a.x)[line * a.idist];
-        const V_t<Tc> x0 = V_t<Tc>{(Tc)u0.x, (Tc)u0.y * (Tc)a.sgn_in};
-        for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < a.n - 1;
-             k += (int64_t)gridDim.x * blockDim.x) {
+__device__ __forceinline__ void scatter_body(const V_t<Tc>* ws, V_t<Ti>* ys,
+                                             const V_t<Tc>* slots, int64_t k0, int64_t k1,
+                                             int64_t kstep, int64_t n, int mode, int sgn,
+                                             double invL_, double scale, const int* iperm,
+                                             const V_t<Tc>* chirp) {
+    const Tc invL = (Tc)invL_;
+    const Tc sx = (Tc)scale, sy = (Tc)(scale * sgn);
+    if (mode == (int)ConvMode::Rader) {
+        const V_t<Tc> x0 = slots[1];
+        for (int64_t k = k0; k < k1 && k < n - 1; k += kstep) {
             const V_t<Tc> v = ws[k];
             const V_t<Tc> r = V_t<Tc>{x0.x + v.x * invL, x0.y - v.y * invL};
-            ys[a.iperm[k]] = V_t<Ti>{(Ti)(r.x * sx), (Ti)(r.y * sy)};
+            ys[iperm[k]] = V_t<Ti>{(Ti)(r.x * sx), (Ti)(r.y * sy)};
             if (k == 0) {
-                const V_t<Tc> d = ((const V_t<Tc>*)a.dc)[line];
+                const V_t<Tc> d = slots[0];
                 ys[0] = V_t<Ti>{(Ti)((x0.x + d.x) * sx), (Ti)((x0.y + d.y) * sy)};
             }
         }
     } else {
-        for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < a.n;
-             k += (int64_t)gridDim.x * blockDim.x) {
+        for (int64_t k = k0; k < k1 && k < n; k += kstep) {
             const V_t<Tc> v = ws[k];
             const V_t<Tc> r = cmul(V_t<Tc>{v.x * invL, -v.y * invL}, chirp[k]);
             ys[k] = V_t<Ti>{(Ti)(r.x * sx), (Ti)(r.y * sy)};
         }
     }
-  }
+}
+
+template <typename Ti, typename Tc>
+__global__ void __launch_bounds__(256)
+k_conv_gather(const V_t<Ti>* __restrict__ x, V_t<Tc>* __restrict__ w, ConvGatherArgs a) {
+    for (int64_t line = blockIdx.y; line < a.nlines; line += gridDim.y)
+        gather_body<Ti, Tc>(x + line * a.idist, w + line * a.wdist, (V_t<Tc>*)a.dc + 2 * line,
+                            (int64_t)blockIdx.x * blockDim.x + threadIdx.x, a.L,
+                            (int64_t)gridDim.x * blockDim.x, a.n, a.mode, a.sgn_in, a.perm,
+                            (const V_t<Tc>*)a.chirp);
+}
+
+template <typename Ti, typename Tc>
+__global__ void __launch_bounds__(256)
+k_conv_scatter(const V_t<Tc>* __restrict__ w, V_t<Ti>* __restrict__ y, ConvScatterArgs a) {
+    for (int64_t line = blockIdx.y; line < a.nlines; line += gridDim.y)
+        scatter_body<Ti, Tc>(w + line * a.wdist, y + line * a.odist, (const V_t<Tc>*)a.dc + 2 * line,
+                             (int64_t)blockIdx.x * blockDim.x + threadIdx.x, a.n,
+                             (int64_t)gridDim.x * blockDim.x, a.n, a.mode, a.sgn_in, a.invL,
+                             a.scale, a.iperm, (const V_t<Tc>*)a.chirp);
+}
+
+// ------------------------------------------------------------------------------------------
+// Work-list wrappers (one MItem per workgroup).
+// ------------------------------------------------------------------------------------------
+constexpr int kMultiE = 16;
+constexpr int kChunk = 4096;  // elements per gather / scatter item
+
+template <typename V>
+__device__ __forceinline__ V* mptr(uint64_t a, int sel, uint64_t ib, uint64_t ob) {
+    return reinterpret_cast<V*>(mresolve(a, sel, ib, ob));
+}
+
+__device__ __forceinline__ void item_direction(const MItem& it, int n, const MultiArgs& ma,
+                                               int& sgn, double& scale) {
+    if (it.flags & kLaunchDir) {
+        sgn = ma.sgn;
+        scale = ma.normalize ? 1.0 / (double)n : 1.0;
+    } else {
+        sgn = it.sgn;
+        scale = it.scale;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(1024)
+k_rows_multi(MultiArgs ma, const MItem* __restrict__ items, const MPlan* __restrict__ plans) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const MItem it = items[blockIdx.x];
+    const MPlan* P = plans + it.plan;
+    int sgn;
+    double scale;
+    item_direction(it, P->n, ma, sgn, scale);
+    ItemLines<T> lines{ma.in_base, ma.out_base, &it};
+    rows_body<T, kMultiE>(smem, lines, it.count, P->line.N, P->C, P->line.Tt, P->line.Ls,
+                          P->line.pt, (const V_t<T>*)P->line.tw, sgn, scale,
+                          (it.flags & kEpilogue) ? (const V_t<T>*)P->K : nullptr);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(1024)
+k_fs_a_multi(MultiArgs ma, const MItem* __restrict__ items, const MPlan* __restrict__ plans) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const MItem it = items[blockIdx.x];
+    const MPlan* P = plans + it.plan;
+    int sgn;
+    double scale;
+    item_direction(it, P->n, ma, sgn, scale);
+    fs_a_body<T, kMultiE>(smem, mptr<const V_t<T>>(it.in, it.flags & kSelInMask, ma.in_base, ma.out_base),
+                          mptr<V_t<T>>(it.out, (it.flags >> kSelOutShift) & 15, ma.in_base, ma.out_base),
+                          P->N1, P->N2, P->CA, P->la.Tt, P->la.Ls, it.count * P->CA, sgn,
+                          P->shift, P->mask, (const V_t<T>*)P->T1, (const V_t<T>*)P->T2, P->la.pt,
+                          (const V_t<T>*)P->la.tw);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(1024)
+k_fs_b_multi(MultiArgs ma, const MItem* __restrict__ items, const MPlan* __restrict__ plans) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const MItem it = items[blockIdx.x];
+    const MPlan* P = plans + it.plan;
+    int sgn;
+    double scale;
+    item_direction(it, P->n, ma, sgn, scale);
+    const bool epi = it.flags & kEpilogue;
+    fs_b_body<T, kMultiE>(smem, mptr<const V_t<T>>(it.in, it.flags & kSelInMask, ma.in_base, ma.out_base),
+                          mptr<V_t<T>>(it.out, (it.flags >> kSelOutShift) & 15, ma.in_base, ma.out_base),
+                          P->N1, P->N2, P->CB, P->lb.Tt, P->lb.Ls, it.count * P->CB, sgn, scale,
+                          epi ? (const V_t<T>*)P->K : nullptr,
+                          epi ? reinterpret_cast<V_t<T>*>(it.dc) : nullptr, P->lb.pt,
+                          (const V_t<T>*)P->lb.tw);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(1024)
+k_conv_multi(MultiArgs ma, const MItem* __restrict__ items, const MPlan* __restrict__ plans) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const MItem it = items[blockIdx.x];
+    const MPlan* P = plans + it.plan;
+    int sgn;
+    double scale;
+    item_direction(it, P->n, ma, sgn, scale);
+    ItemLines<T> lines{ma.in_base, ma.out_base, &it};
+    conv_body<T, kMultiE>(smem, lines, it.count, P->n, P->L, P->C, P->line.Tt, P->LsX, P->LsA,
+                          P->mode, sgn, scale, P->perm, P->iperm, (const V_t<T>*)P->chirp,
+                          (const V_t<T>*)P->K, P->line.pt, (const V_t<T>*)P->line.tw);
+}
+
+// gather: in = x line, out = work line; scatter: in = work line, out = y line.  dc = the
+// line's slots {A^[0], x0}.  One item covers kChunk elements of one line.
+template <typename Ti, typename Tc>
+__global__ void __launch_bounds__(256)
+k_gather_multi(MultiArgs ma, const MItem* __restrict__ items, const MPlan* __restrict__ plans) {
+    const MItem it = items[blockIdx.x];
+    const MPlan* P = plans + it.plan;
+    int sgn;
+    double scale;
+    item_direction(it, P->n, ma, sgn, scale);
+    const int64_t e0 = (int64_t)it.count * kChunk;
+    int64_t e1 = e0 + kChunk;
+    if (e1 > P->L) e1 = P->L;
+    gather_body<Ti, Tc>(mptr<const V_t<Ti>>(it.in, it.flags & kSelInMask, ma.in_base, ma.out_base),
+                        mptr<V_t<Tc>>(it.out, (it.flags >> kSelOutShift) & 15, ma.in_base, ma.out_base),
+                        reinterpret_cast<V_t<Tc>*>(it.dc), e0 + threadIdx.x, e1, blockDim.x, P->n,
+                        P->mode, sgn, P->perm, (const V_t<Tc>*)P->chirp);
+}
+
+template <typename Ti, typename Tc>
+__global__ void __launch_bounds__(256)
+k_scatter_multi(MultiArgs ma, const MItem* __restrict__ items, const MPlan* __restrict__ plans) {
+    const MItem it = items[blockIdx.x];
+    const MPlan* P = plans + it.plan;
+    int sgn;
+    double scale;
+    item_direction(it, P->n, ma, sgn, scale);
+    const int64_t k0 = (int64_t)it.count * kChunk;
+    const int64_t k1 = k0 + kChunk;
+    scatter_body<Ti, Tc>(mptr<const V_t<Tc>>(it.in, it.flags & kSelInMask, ma.in_base, ma.out_base),
+                         mptr<V_t<Ti>>(it.out, (it.flags >> kSelOutShift) & 15, ma.in_base, ma.out_base),
+                         reinterpret_cast<const V_t<Tc>*>(it.dc), k0 + threadIdx.x, k1, blockDim.x,
+                         P->n, P->mode, sgn, 1.0 / (double)P->L, scale, P->iperm,
+                         (const V_t<Tc>*)P->chirp);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -403,9 +625,6 @@ hipError_t conv_t(const void* in, void* out, const ConvSmallArgs& a, const PassT
     return hipGetLastError();
 }
 
-}  // namespace detail
-
-namespace detail {
 template <typename Ti, typename Tc>
 hipError_t gather_t(const void* x, void* w, const ConvGatherArgs& a, hipStream_t st) {
     const int64_t per = (a.L + 255) / 256;
@@ -415,6 +634,7 @@ hipError_t gather_t(const void* x, void* w, const ConvGatherArgs& a, hipStream_t
                        (const V_t<Ti>*)x, (V_t<Tc>*)w, a);
     return hipGetLastError();
 }
+
 template <typename Ti, typename Tc>
 hipError_t scatter_t(const void* w, void* y, const ConvScatterArgs& a, hipStream_t st) {
     const int64_t per = (a.n + 255) / 256;
@@ -424,6 +644,45 @@ hipError_t scatter_t(const void* w, void* y, const ConvScatterArgs& a, hipStream
                        (const V_t<Tc>*)w, (V_t<Ti>*)y, a);
     return hipGetLastError();
 }
-}  // namespace detail
 
+// Work-list launch for data precision T (rows, four-step, fused convolution).
+template <typename T>
+hipError_t multi_t(const MultiLaunch& L, const MultiArgs& ma, hipStream_t st) {
+    static bool once = (allow_big_lds(k_rows_multi<T>), allow_big_lds(k_fs_a_multi<T>),
+                        allow_big_lds(k_fs_b_multi<T>), allow_big_lds(k_conv_multi<T>), true);
+    (void)once;
+    const dim3 grid((unsigned)L.nitems), block(L.block);
+    switch ((MultiKernel)L.kernel) {
+        case MultiKernel::Rows:
+            hipLaunchKernelGGL(k_rows_multi<T>, grid, block, L.lds, st, ma, L.items, L.plans);
+            break;
+        case MultiKernel::FourStepA:
+            hipLaunchKernelGGL(k_fs_a_multi<T>, grid, block, L.lds, st, ma, L.items, L.plans);
+            break;
+        case MultiKernel::FourStepB:
+            hipLaunchKernelGGL(k_fs_b_multi<T>, grid, block, L.lds, st, ma, L.items, L.plans);
+            break;
+        case MultiKernel::ConvSmall:
+            hipLaunchKernelGGL(k_conv_multi<T>, grid, block, L.lds, st, ma, L.items, L.plans);
+            break;
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// Work-list gather / scatter of large convolutions (data Ti, work buffer Tc).
+template <typename Ti, typename Tc>
+hipError_t gs_multi_t(const MultiLaunch& L, const MultiArgs& ma, hipStream_t st) {
+    const dim3 grid((unsigned)L.nitems), block(256);
+    if ((MultiKernel)L.kernel == MultiKernel::Gather)
+        hipLaunchKernelGGL((k_gather_multi<Ti, Tc>), grid, block, 0, st, ma, L.items, L.plans);
+    else if ((MultiKernel)L.kernel == MultiKernel::Scatter)
+        hipLaunchKernelGGL((k_scatter_multi<Ti, Tc>), grid, block, 0, st, ma, L.items, L.plans);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+}  // namespace detail
 }  // namespace cfft

@@ -90,6 +90,16 @@ static int upload(DevBuf& buf, const std::vector<cld>& v, Prec pr) {
     return CFFT_OK;
 }
 
+int upload_cld(DevBuf& buf, const std::vector<cld>& v, Prec pr) { return upload(buf, v, pr); }
+int upload_ints(DevBuf& buf, const std::vector<int>& v);
+int upload_ints(DevBuf& buf, const std::vector<int>& v) {
+    if (v.empty()) return CFFT_OK;
+    int rc = buf.alloc(v.size() * sizeof(int));
+    if (rc) return rc;
+    CFFT_HIP(hipMemcpy(buf.p, v.data(), buf.bytes, hipMemcpyHostToDevice));
+    return CFFT_OK;
+}
+
 static int upload_int(DevBuf& buf, const std::vector<int>& v) {
     if (v.empty()) return CFFT_OK;
     int rc = buf.alloc(v.size() * sizeof(int));
@@ -182,6 +192,13 @@ int line_stride_for(Prec p, int64_t N) {
 // Builds the pass table, thread count and twiddles of one line FFT.  E = 0 picks the default.
 int make_line(LinePlan& lp, int64_t N, Prec pr, int E, int max_threads) {
     lp.N = (int)N;
+    if (N == 1) {  // no passes: the rows kernel copies
+        lp.E = E ? E : 16;
+        lp.Tt = 1;
+        lp.pt.npass = 0;
+        lp.Ls = line_stride_for(pr, 1);
+        return upload(lp.dtw, std::vector<cld>{cld(1, 0)}, pr);
+    }
     std::vector<int> rad = line_radices(N);
     if (rad.empty() || (int)rad.size() > kMaxPasses)
         return fail(CFFT_ERR_UNSUPPORTED, "line length " + std::to_string(N) + " is not smooth");
@@ -193,6 +210,7 @@ int make_line(LinePlan& lp, int64_t N, Prec pr, int E, int max_threads) {
                                               std::to_string(lp.Tt) + " threads");
     lp.pt.npass = (int)rad.size();
     std::vector<cld> tw;
+    const OmegaTable om(N);  // w_g^{pk} = w_N^{s p k}
     int64_t s = 1;
     for (size_t i = 0; i < rad.size(); ++i) {
         const int R = rad[i];
@@ -200,7 +218,7 @@ int make_line(LinePlan& lp, int64_t N, Prec pr, int E, int max_threads) {
         lp.pt.twoff[i] = (int)tw.size();
         const int64_t g = N / s, n1 = g / R;
         for (int64_t p = 0; p < n1; ++p)
-            for (int k = 1; k < R; ++k) tw.push_back(omega(p * k, g));
+            for (int k = 1; k < R; ++k) tw.push_back(om(s * p * k));
         s *= R;
     }
     if (tw.empty()) tw.push_back(cld(1, 0));
@@ -240,6 +258,8 @@ static int plan_direct(Ctx* ctx, Plan& p, int E, int target_threads) {
 }
 
 // Can a line of length N run as one workgroup (threads and LDS)?
+static bool line_fits(Ctx* ctx, int64_t N, Prec pr, int E);
+bool line_fits_lds(Ctx* ctx, int64_t N, Prec pr, int E) { return line_fits(ctx, N, pr, E); }
 static bool line_fits(Ctx* ctx, int64_t N, Prec pr, int E) {
     if (N < 2 || !is_smooth(N) || N > (1 << 24)) return false;
     std::vector<int> rad = line_radices(N);
@@ -307,49 +327,105 @@ static int plan_fourstep_factors(Ctx* ctx, FourStepPlan& fs, int64_t n, Prec pr,
 
 static int plan_fft_core(Ctx* ctx, Plan& p, const PlanOptions& o);
 
-// Rader / Bluestein planning.
-static int plan_conv(Ctx* ctx, Plan& p, const PlanOptions& o) {
-    const int64_t n = p.n;
+// Convolution data of a Rader (prime n) or Bluestein transform: permutations or chirp, and the
+// zero-padded kernel b whose spectrum the convolution multiplies by.
+int conv_data(int64_t n, ConvData& d) {
     const bool rader = is_prime(n);
-    p.mode = rader ? ConvMode::Rader : ConvMode::Bluestein;
-    std::vector<cld> b;  // convolution kernel, length L
-    std::vector<cld> chirp;
-    std::vector<int> perm, iperm;
+    d.mode = rader ? ConvMode::Rader : ConvMode::Bluestein;
+    d.perm.clear();
+    d.iperm.clear();
+    d.chirp.clear();
     if (rader) {
         const int64_t N = n - 1;
-        p.L = is_smooth(N) ? N : next_smooth(2 * N - 1);
+        d.L = is_smooth(N) ? N : next_smooth(2 * N - 1);
         const int64_t g = primitive_root(n);
         const int64_t ginv = powmod(g, n - 2, n);
-        perm.resize(N);
-        iperm.resize(N);
+        d.perm.resize(N);
+        d.iperm.resize(N);
+        const OmegaTable om(n);
         std::vector<cld> w(N);
         int64_t gp = 1, gi = 1;
         for (int64_t i = 0; i < N; ++i) {
-            perm[i] = (int)gp;
-            iperm[i] = (int)gi;
-            w[i] = omega(gi, n);  // w[i] = w_n^{(g^-1)^i}
+            d.perm[i] = (int)gp;
+            d.iperm[i] = (int)gi;
+            w[i] = om(gi);  // w[i] = w_n^{(g^-1)^i}  (prime.jl:36-37, rounded once)
             gp = gp * g % n;
             gi = gi * ginv % n;
         }
-        b.assign(p.L, cld(0, 0));
-        for (int64_t m = 0; m < N; ++m) b[m] = w[m];
-        for (int64_t m = p.L - N + 1; m < p.L; ++m) b[m] = w[m - (p.L - N)];
-        p.g = g;
+        d.b.assign(d.L, cld(0, 0));
+        for (int64_t m = 0; m < N; ++m) d.b[m] = w[m];
+        for (int64_t m = d.L - N + 1; m < d.L; ++m) d.b[m] = w[m - (d.L - N)];
+        d.g = g;
     } else {
-        p.L = next_smooth(2 * n - 1);
-        chirp.resize(n);
+        d.L = next_smooth(2 * n - 1);
+        d.chirp.resize(n);
+        const OmegaTable om(2 * n);
         for (int64_t m = 0; m < n; ++m) {
             const int64_t e = (int64_t)((unsigned __int128)m * m % (unsigned __int128)(2 * n));
-            chirp[m] = omega(e, 2 * n);  // e^{-pi i m^2 / n}
+            d.chirp[m] = om(e);  // e^{-pi i m^2 / n}
         }
-        b.assign(p.L, cld(0, 0));
-        for (int64_t m = 0; m < n; ++m) b[m] = std::conj(chirp[m]);
-        for (int64_t m = 1; m < n; ++m) b[p.L - m] = std::conj(chirp[m]);
+        d.b.assign(d.L, cld(0, 0));
+        for (int64_t m = 0; m < n; ++m) d.b[m] = std::conj(d.chirp[m]);
+        for (int64_t m = 1; m < n; ++m) d.b[d.L - m] = std::conj(d.chirp[m]);
+        d.g = 0;
     }
-    if (!host_fft_ld(b)) return fail(CFFT_ERR_UNSUPPORTED, "kernel spectrum length not smooth");
-    // precision of the convolution: FP64 when the I/O is FP32 and the prime exceeds 2^16
-    p.cprec = p.prec;
-    if (p.prec == Prec::F32 && n > 65536) p.cprec = Prec::F64;
+    return CFFT_OK;
+}
+
+// K = FFT_L(b), rounded once to cprec.  Small L: long double on the host; larger L: this
+// library's own FP64 FFT on the device (FP64 is kept for kernel generation only).
+int conv_spectrum(Ctx* ctx, const std::vector<cld>& b, Prec cprec, DevBuf& K) {
+    const int64_t L = (int64_t)b.size();
+    if (L <= 4096) {
+        std::vector<cld> h = b;
+        if (!host_fft_ld(h)) return fail(CFFT_ERR_UNSUPPORTED, "kernel spectrum length not smooth");
+        return upload(K, h, cprec);
+    }
+    Plan t;
+    t.ctx = ctx;
+    t.n = L;
+    t.batch = 1;
+    t.idist = t.odist = L;
+    t.prec = Prec::F64;
+    t.dtype = CFFT_C64;
+    PlanOptions o;
+    int rc = plan_build(ctx, t, o);
+    if (rc) return rc;
+    DevBuf in, out;
+    if ((rc = upload(in, b, Prec::F64))) return rc;
+    if ((rc = out.alloc((size_t)L * 16))) return rc;
+    if ((rc = plan_exec(t, in.p, out.p, CFFT_FORWARD, 0, ctx->stream))) return rc;
+    CFFT_HIP(hipStreamSynchronize(ctx->stream));
+    if (cprec == Prec::F64) {
+        std::swap(K.p, out.p);
+        std::swap(K.bytes, out.bytes);
+        return CFFT_OK;
+    }
+    std::vector<double> h((size_t)2 * L);
+    CFFT_HIP(hipMemcpy(h.data(), out.p, (size_t)L * 16, hipMemcpyDeviceToHost));
+    std::vector<float> f((size_t)2 * L);
+    for (size_t i = 0; i < f.size(); ++i) f[i] = (float)h[i];
+    if ((rc = K.alloc((size_t)L * 8))) return rc;
+    CFFT_HIP(hipMemcpy(K.p, f.data(), (size_t)L * 8, hipMemcpyHostToDevice));
+    return CFFT_OK;
+}
+
+// Precision of a convolution's work: FP64 when the I/O is FP32 and n exceeds 2^16.
+Prec conv_precision(int64_t n, Prec io) { return (io == Prec::F32 && n > 65536) ? Prec::F64 : io; }
+
+// Rader / Bluestein planning.
+static int plan_conv(Ctx* ctx, Plan& p, const PlanOptions& o) {
+    const int64_t n = p.n;
+    ConvData d;
+    conv_data(n, d);
+    p.mode = d.mode;
+    p.L = d.L;
+    p.g = d.g;
+    p.cprec = conv_precision(n, p.prec);
+    const std::vector<int>& perm = d.perm;
+    const std::vector<int>& iperm = d.iperm;
+    const std::vector<cld>& chirp = d.chirp;
+    const bool rader = p.mode == ConvMode::Rader;
     int rc;
     // fused small shape?
     const int E = o.E ? o.E : 16;
@@ -375,7 +451,7 @@ static int plan_conv(Ctx* ctx, Plan& p, const PlanOptions& o) {
     } else {
         if ((rc = upload(p.chirp, chirp, p.cprec))) return rc;
     }
-    if ((rc = upload(p.K, b, p.cprec))) return rc;
+    if ((rc = conv_spectrum(ctx, d.b, p.cprec, p.K))) return rc;
     if (small) {
         p.kind = Kind::ConvSmall;
         return CFFT_OK;
@@ -399,7 +475,7 @@ static int plan_conv(Ctx* ctx, Plan& p, const PlanOptions& o) {
     const size_t wbytes = (size_t)p.batch * p.L * esize(p.cprec);
     if ((rc = p.w1.alloc(wbytes))) return rc;
     if ((rc = p.w2.alloc(wbytes))) return rc;
-    if ((rc = p.dc.alloc((size_t)p.batch * esize(p.cprec)))) return rc;
+    if ((rc = p.dc.alloc((size_t)p.batch * 2 * esize(p.cprec)))) return rc;
     return CFFT_OK;
 }
 
@@ -571,6 +647,7 @@ static int run_fft(const Plan& p, const void* in, void* out, int sgn, double sca
             g.sgn_in = sgn;
             g.perm = (const int*)p.perm.p;
             g.chirp = p.chirp.p;
+            g.dc = p.dc.p;
             CFFT_HIP(launch_conv_gather(p.prec, p.cprec, in, p.w1.p, g, st));
             ConvEpilogue e{p.K.p, p.dc.p};
             int rc = run_fft(*p.sub, p.w1.p, p.w2.p, 1, 1.0, e, st);
@@ -592,7 +669,6 @@ static int run_fft(const Plan& p, const void* in, void* out, int sgn, double sca
             s.iperm = (const int*)p.iperm.p;
             s.chirp = p.chirp.p;
             s.dc = p.dc.p;
-            s.x = in;
             CFFT_HIP(launch_conv_scatter(p.prec, p.cprec, p.w1.p, out, s, st));
             return CFFT_OK;
         }
@@ -606,8 +682,6 @@ int plan_exec(const Plan& p, const void* in, void* out, int direction, int norma
     if (p.kind == Kind::Uneven) return uneven_exec(p, in, out, direction, normalize, st);
     if (p.batch == 0) return CFFT_OK;
     if (!in || !out) return fail(CFFT_ERR_INVALID, "null buffer");
-    if (p.kind == Kind::ConvLarge && in == out)
-        return fail(CFFT_ERR_INVALID, "in-place execution is not supported for this length");
     const int sgn = direction > 0 ? -1 : 1;
     const double scale = normalize ? 1.0 / (double)p.n : 1.0;
     return run_fft(p, in, out, sgn, scale, ConvEpilogue{nullptr, nullptr}, st);
@@ -691,9 +765,6 @@ void ctx_destroy(Ctx* c) {
 // =============================================================================== C ABI
 using namespace cfft;
 
-struct cfft_ctx_s {
-    Ctx* c;
-};
 struct cfft_plan_s {
     std::shared_ptr<Plan> p;
     cfft_ctx_s* ctx;

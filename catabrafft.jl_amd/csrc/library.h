@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <complex>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -116,6 +117,19 @@ int plan_exec(const Plan& p, const void* in, void* out, int direction, int norma
 size_t plan_device_bytes(const Plan& p);
 std::string plan_description(const Plan& p);
 
+struct ConvData {
+    ConvMode mode = ConvMode::Rader;
+    int64_t L = 0, g = 0;
+    std::vector<int> perm, iperm;
+    std::vector<std::complex<long double>> chirp, b;
+};
+int conv_data(int64_t n, ConvData& d);
+int conv_spectrum(Ctx* ctx, const std::vector<std::complex<long double>>& b, Prec cprec, DevBuf& K);
+Prec conv_precision(int64_t n, Prec io);
+int upload_cld(DevBuf& buf, const std::vector<std::complex<long double>>& v, Prec pr);
+int upload_ints(DevBuf& buf, const std::vector<int>& v);
+bool line_fits_lds(Ctx* ctx, int64_t N, Prec pr, int E);
+
 // autotune.cpp: MEASURE / ENCHANT planning (times candidate PlanOptions on the device).
 int autotune(Ctx* ctx, Plan& p, int flags, const PlanOptions& base);
 
@@ -126,3 +140,8 @@ int uneven_exec(const Plan& p, const void* in, void* out, int direction, int nor
                 hipStream_t st);
 
 }  // namespace cfft
+
+// The C ABI's context handle (include/catabrafft.h).
+struct cfft_ctx_s {
+    cfft::Ctx* c;
+};

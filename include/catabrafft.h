@@ -117,10 +117,14 @@ CFFT_API int cfft_synchronize(cfft_ctx ctx);
 /* A server with its own worker thread and streams; admission keeps the resident device bytes
  * of all in-flight requests (their buffers + plan tables) under hbm_cap_bytes. */
 CFFT_API int cfft_server_create(cfft_ctx ctx, int64_t hbm_cap_bytes, int nstreams, cfft_server* srv);
-/* Submits `batch` transforms of length n, host input `in`, host output `out` (both stay owned
- * by the caller and must live until the ticket completes). */
-CFFT_API int cfft_server_submit(cfft_server srv, int64_t n, int64_t batch, int dtype, int direction,
-                       int normalize, const void* in, void* out, int64_t* ticket);
+/* Submits `batch` contiguous transforms of length n.  on_device = 0: `in` / `out` are host
+ * buffers (the server stages them through device memory it admits under the cap);
+ * on_device = 1: they are device buffers already resident in HBM (their bytes still count
+ * toward the cap while the request is in flight).  Both stay owned by the caller and must live
+ * until the ticket completes. */
+CFFT_API int cfft_server_submit(cfft_server srv, int64_t n, int64_t batch, int dtype,
+                                int direction, int normalize, const void* in, void* out,
+                                int on_device, int64_t* ticket);
 /* 0 when done, CFFT_ERR_NOTREADY when still queued / running, < 0 on failure. */
 CFFT_API int cfft_server_poll(cfft_server srv, int64_t ticket);
 /* Blocks until the ticket is done. */

@@ -69,12 +69,17 @@ __host__ __device__ constexpr int bfly_per_thread(int E, int R) { return E / R >
 // One Stockham pass.  SRC(e) returns element e of the pass input, DST.put(e, v) stores element e
 // of its output.  mid_barrier separates the reads from the writes when both go to the same LDS.
 template <typename T, int R, int MB, class SRC, class DST>
-__device__ __forceinline__ void stockham_pass(const int N, const int s, const int t, const int Tt,
+__device__ __forceinline__ void stockham_pass(const int N, const int s, const int t_in, const int Tt,
                                               const bool active,
                                               const typename Vec2<T>::type* __restrict__ tw,
                                               const SRC& src, const DST& dst,
                                               const bool mid_barrier) {
     using V = typename Vec2<T>::type;
+    // Keep per-pass address arithmetic inside the pass: an opaque copy of the lane index stops
+    // the compiler from hoisting every radix's addresses out of the pass loop.
+    int t_ = t_in;
+    asm volatile("" : "+v"(t_));
+    const int t = t_;
     const int nb = N / R;
     V v[MB][R];
 #pragma unroll

@@ -75,20 +75,7 @@ def cpu_baseline(budget_s: float = 12.0):
     }
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cases", default="all", help="comma list of case indices")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py: this FFT engine runs on one GPU; the batched workload is measured at "
-              "--gpus 1 only (transforms shard by rows across GPUs with no communication).",
-              file=sys.stderr)
-        sys.exit(2)
-
+def bench_batched(args):
     import torch
 
     import catabrafft as C
@@ -190,6 +177,284 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))
+
+
+# ----------------------------------------------------------------------------- other configs
+def _time_ms(fn, steps, warmup):
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(steps):
+        fn()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / steps
+
+
+def bench_single(args):
+    """n = 1024 ComplexF64, one transform: forward + inverse latency on the device and through
+    the host-pointer ABI (the Julia drop-in's path, PCIe included)."""
+    import torch
+
+    import catabrafft as C
+    import oracle as O
+
+    n = 1024
+    rng = np.random.Generator(np.random.PCG64(1))
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    xd = torch.from_numpy(x).cuda()
+    yd, zd = torch.empty_like(xd), torch.empty_like(xd)
+    p = C.Plan(n, 1, np.complex128)
+    s = torch.cuda.current_stream().cuda_stream
+
+    def fwd_inv():
+        p.exec_ptr(xd.data_ptr(), yd.data_ptr(), -1, False, s)
+        p.exec_ptr(yd.data_ptr(), zd.data_ptr(), 1, True, s)
+
+    dev_us = _time_ms(fwd_inv, args.steps * 50, args.warmup) * 1e3
+    t0 = time.perf_counter()
+    reps = 200
+    for _ in range(reps):
+        y = C.fft(x)
+        z = C.ifft(y)
+    host_us = (time.perf_counter() - t0) / reps * 1e6
+    err = float(np.linalg.norm(y - O.fft_l(x)) / np.linalg.norm(O.fft_l(x)))
+    rt = float(np.linalg.norm(z - x) / np.linalg.norm(x))
+    plan = O.Plan(n, "d")
+    t0 = time.perf_counter()
+    for _ in range(2000):
+        plan.exec(x, -1)
+        plan.exec(x, 1, True)
+    cpu_us = (time.perf_counter() - t0) / 2000 * 1e6
+    print(json.dumps({"workload": "n=1024 ComplexF64 forward+inverse, single transform",
+                      "device_us_per_fwd_inv": round(dev_us, 2),
+                      "host_abi_us_per_fwd_inv": round(host_us, 1),
+                      "cpu_oracle_us_per_fwd_inv_1thread": round(cpu_us, 2),
+                      "rel_l2_vs_oracle": err, "round_trip": rt, "plan": p.describe()}))
+
+
+def bench_rader(args):
+    """Batched Rader primes, ComplexF32 (FP64 convolution for p > 2^16), 2^24 elements each."""
+    import torch
+
+    import catabrafft as C
+    import oracle as O
+
+    rows = []
+    for p_ in (17, 257, 4099, 65537, 1048573):
+        batch = (1 << 24) // p_
+        g = torch.Generator(device="cuda")
+        g.manual_seed(3)
+        x = torch.complex(torch.rand(batch, p_, generator=g, device="cuda") * 2 - 1,
+                          torch.rand(batch, p_, generator=g, device="cuda") * 2 - 1)
+        y = torch.empty_like(x)
+        plan = C.Plan(p_, batch, np.complex64)
+        s = torch.cuda.current_stream().cuda_stream
+        ms = _time_ms(lambda: plan.exec_ptr(x.data_ptr(), y.data_ptr(), -1, False, s),
+                      args.steps, args.warmup)
+        sample = [0, batch - 1]
+        xr = x[sample].cpu().numpy().astype(np.complex128)
+        ref = O.fft(xr, "d", threads=8) if p_ > 4096 else O.fft_l(xr)
+        err = float(np.linalg.norm(y[sample].cpu().numpy() - ref) / np.linalg.norm(ref))
+        gbs = 2 * p_ * batch * 8 / (ms * 1e-3) / 1e9
+        rows.append({"p": p_, "batch": batch, "ms": round(ms, 4),
+                     "gflops": round(flops(p_, batch) / (ms * 1e-3) / 1e9, 1),
+                     "algo_GBs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+                     "rel_l2_vs_oracle": err, "plan": plan.describe()})
+        del x, y, plan
+        torch.cuda.empty_cache()
+    print(json.dumps({"workload": "batched Rader primes, ComplexF32, 2^24 elements per prime",
+                      "cases": rows}))
+
+
+def cfg4_lengths(count=10000, seed=4, lo=64, hi=1 << 18):
+    """L = round(exp(U(ln lo, ln hi))), snapped to the nearest 7-smooth (70%), non-7-smooth
+    composite (20%) or prime (10%) length."""
+    import oracle as O
+
+    rng = np.random.Generator(np.random.PCG64(seed))
+
+    def smooth7(m):
+        for q in (2, 3, 5, 7):
+            while m % q == 0:
+                m //= q
+        return m == 1
+
+    def nearest(L, pred):
+        d = 0
+        while True:
+            for c in (L - d, L + d):
+                if 2 <= c and pred(c):
+                    return c
+            d += 1
+
+    out = []
+    for _ in range(count):
+        L = int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
+        u = rng.uniform()
+        if u < 0.7:
+            out.append(nearest(L, smooth7))
+        elif u < 0.9:
+            out.append(nearest(L, lambda m: not smooth7(m) and not O.isprime(m)))
+        else:
+            out.append(nearest(L, O.isprime))
+    return out
+
+
+def bench_uneven(args):
+    """10,000 transforms of uneven lengths in one plan (work-list launches, hipGraph replay)."""
+    import torch
+
+    import catabrafft as C
+    import oracle as O
+
+    lengths = cfg4_lengths()
+    offs = np.concatenate([[0], np.cumsum(lengths)[:-1]]).astype(np.int64)
+    total = int(sum(lengths))
+    g = torch.Generator(device="cuda")
+    g.manual_seed(4)
+    x = torch.complex(torch.randn(total, generator=g, device="cuda"),
+                      torch.randn(total, generator=g, device="cuda"))
+    y = torch.empty_like(x)
+    t0 = time.perf_counter()
+    plan = C.UnevenPlan(lengths, offs, np.complex64)
+    plan_s = time.perf_counter() - t0
+    s = torch.cuda.current_stream().cuda_stream
+    ms = _time_ms(lambda: plan.exec_ptr(x.data_ptr(), y.data_ptr(), -1, False, s),
+                  args.steps, args.warmup)
+    fl = sum(5.0 * n * math.log2(n) for n in lengths)
+    gbs = 2 * total * 8 / (ms * 1e-3) / 1e9
+    rng = np.random.Generator(np.random.PCG64(7))
+    errs = []
+    xs, ys = x.cpu().numpy(), y.cpu().numpy()
+    for i in rng.choice(len(lengths), 12, replace=False):
+        n, o = lengths[i], offs[i]
+        ref = O.fft(xs[o:o + n].astype(np.complex128), "d")
+        errs.append(float(np.linalg.norm(ys[o:o + n] - ref) / np.linalg.norm(ref)))
+    print(json.dumps({"workload": "uneven batch: 10,000 lengths log-uniform in [64, 2^18] "
+                                  "(70% 7-smooth, 20% other composite, 10% prime), ComplexF32",
+                      "elements": total, "ms": round(ms, 4),
+                      "gflops": round(fl / (ms * 1e-3) / 1e9, 1), "algo_GBs": round(gbs, 1),
+                      "hbm_frac": round(gbs / HBM_PEAK_GBS, 4), "plan_seconds": round(plan_s, 2),
+                      "max_rel_l2_sampled": max(errs), "plan": plan.describe()}))
+
+
+def bench_serve(args):
+    """Poisson arrivals at 20k req/s, lengths as the uneven batch, batch per request
+    Geometric(p = 1/16) truncated to 1..256, 10% ComplexF64, 200 GB admission cap; requests
+    read device-resident inputs.  Baseline: every transform of the same trace as its own FP64
+    launch (plan per length, batch 1), one after another on one stream."""
+    import torch
+
+    import catabrafft as C
+
+    rng = np.random.Generator(np.random.PCG64(5))
+    nreq = args.requests
+    lengths = cfg4_lengths(count=nreq, seed=5)
+    batches = np.minimum(np.maximum(rng.geometric(1 / 16, nreq), 1), 256)
+    f64 = rng.uniform(size=nreq) < 0.1
+    gaps = rng.exponential(1 / 20000.0, nreq)
+    pool_elems = int(max(n * b for n, b in zip(lengths, batches)))
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+    pool32 = torch.complex(torch.randn(pool_elems, generator=g, device="cuda"),
+                           torch.randn(pool_elems, generator=g, device="cuda"))
+    pool64 = pool32.to(torch.complex128)
+    out32 = torch.empty(pool_elems * 4, dtype=torch.complex64, device="cuda")
+    out64 = torch.empty(pool_elems * 4, dtype=torch.complex128, device="cuda")
+    srv = C.Server(hbm_cap_bytes=200 * 10 ** 9, nstreams=4)
+    # warm the server's resident plans with one request per (length, dtype), then forget them
+    warm = []
+    for n, d in sorted(set(zip(lengths, f64.tolist()))):
+        src = pool64 if d else pool32
+        warm.append(srv.submit(src[:n], out=(out64 if d else out32)[:n]))
+    for t in warm:
+        srv.wait(t)
+    srv.reset_stats()
+    # replay the trace
+    tickets = []
+    t_next = time.perf_counter()
+    ooff = 0
+    for i in range(nreq):
+        t_next += gaps[i]
+        while time.perf_counter() < t_next:
+            pass
+        n, b = lengths[i], int(batches[i])
+        src = (pool64 if f64[i] else pool32)[:n * b].view(b, n)
+        dst_pool = out64 if f64[i] else out32
+        if ooff + n * b > dst_pool.numel():
+            ooff = 0
+        dst = dst_pool[ooff:ooff + n * b].view(b, n)
+        ooff += n * b
+        tickets.append(srv.submit(src, out=dst))
+    for t in tickets:
+        srv.wait(t)
+    st = srv.stats()
+    srv.close()
+    transforms = int(batches.sum())
+    # baseline: one FP64 transform per launch
+    plans = {}
+    for n in set(lengths):
+        plans[n] = C.Plan(n, 1, np.complex128)
+    s = torch.cuda.current_stream().cuda_stream
+    base_n = min(nreq, 2000)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    done = 0
+    for i in range(base_n):
+        n, b = lengths[i], int(batches[i])
+        src = pool64[:n * b]
+        for r in range(b):
+            plans[n].exec_ptr(src[r * n:].data_ptr(), out64[r * n:].data_ptr(), -1, False, s)
+        done += b
+    torch.cuda.synchronize()
+    base_tps = done / (time.perf_counter() - t0)
+    print(json.dumps({"workload": f"serving: {nreq} Poisson requests at 20k req/s, lengths "
+                                  "log-uniform [64, 2^18], batch Geometric(1/16) in 1..256, "
+                                  "10% ComplexF64, 200 GB cap, device-resident I/O",
+                      "p50_ms": round(st["p50_s"] * 1e3, 3), "p99_ms": round(st["p99_s"] * 1e3, 3),
+                      "transforms_per_s": round(st["transforms_per_s"], 1),
+                      "gflops": round(st["gflops"], 1), "transforms": transforms,
+                      "baseline_fp64_one_per_launch_transforms_per_s": round(base_tps, 1),
+                      "baseline_sample_requests": base_n,
+                      "speedup_vs_baseline": round(st["transforms_per_s"] / base_tps, 2)}))
+
+
+def bench_copy(args):
+    """A plain device-to-device copy of 1 GiB (torch): the reference kernel whose FETCH_SIZE /
+    WRITE_SIZE counters calibrate the HBM-traffic figures of the FFT kernels."""
+    import torch
+
+    nbytes = 1 << 30
+    x = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").uniform_()
+    y = torch.empty_like(x)
+    ms = _time_ms(lambda: y.copy_(x), args.steps, args.warmup)
+    print(json.dumps({"workload": "device copy, 1 GiB read + 1 GiB written", "ms": round(ms, 4),
+                      "GBs_read_plus_write": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cases", default="all", help="comma list of case indices (batched)")
+    ap.add_argument("--workload", default="batched",
+                    choices=["batched", "single", "rader", "uneven", "serve", "copy"])
+    ap.add_argument("--requests", type=int, default=4000, help="serve: trace length")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this FFT engine runs on one GPU; the batched workload is measured at "
+              "--gpus 1 only (transforms shard by rows across GPUs with no communication).",
+              file=sys.stderr)
+        sys.exit(2)
+    {"batched": bench_batched, "single": bench_single, "rader": bench_rader,
+     "uneven": bench_uneven, "serve": bench_serve, "copy": bench_copy}[args.workload](args)
 
 
 if __name__ == "__main__":

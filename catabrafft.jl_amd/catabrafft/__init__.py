@@ -242,6 +242,9 @@ class Server:
         return {"p50_s": v[0].value, "p99_s": v[1].value, "transforms_per_s": v[2].value,
                 "gflops": v[3].value}
 
+    def reset_stats(self):
+        check(lib().cfft_server_reset_stats(self.h))
+
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
             check(lib().cfft_server_destroy(self.h))

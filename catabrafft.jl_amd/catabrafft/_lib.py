@@ -80,6 +80,7 @@ def _sig(L):
     _set(L, "cfft_server_poll", [vp, i64])
     _set(L, "cfft_server_wait", [vp, i64])
     _set(L, "cfft_server_stats", [vp, P(dbl), P(dbl), P(dbl), P(dbl)])
+    _set(L, "cfft_server_reset_stats", [vp])
     _set(L, "cfft_server_destroy", [vp])
     return L
 

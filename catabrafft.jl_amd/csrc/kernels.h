@@ -84,6 +84,9 @@ struct ConvScatterArgs {
     const void* dc;        // per line slots {A^[0], x0} (Tc precision)
 };
 
+// Sets per-kernel attributes (large LDS) once; called when a context is created.
+void init_kernels();
+
 hipError_t launch_rows(Prec p, const void* in, void* out, const RowsArgs& a, const PassTable& pt,
                        const void* tw, hipStream_t st);
 hipError_t launch_fourstep_a(Prec p, const void* in, void* out, const FourStepAArgs& a,

@@ -5,6 +5,11 @@
 
 namespace cfft {
 
+void init_kernels() {
+    init_kernels_f32();
+    init_kernels_f64();
+}
+
 hipError_t launch_rows(Prec p, const void* in, void* out, const RowsArgs& a, const PassTable& pt,
                        const void* tw, hipStream_t st) {
     return p == Prec::F32 ? rows_f32(in, out, a, pt, tw, st) : rows_f64(in, out, a, pt, tw, st);

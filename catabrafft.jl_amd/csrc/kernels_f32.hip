@@ -33,4 +33,19 @@ hipError_t multi_f32(const MultiLaunch& L, const MultiArgs& ma, hipStream_t st) 
 hipError_t gs_multi_f32_f32(const MultiLaunch& L, const MultiArgs& ma, hipStream_t st) {
     return gs_multi_t<float, float>(L, ma, st);
 }
+// Lets every kernel of this precision use up to 160 KiB of LDS (called once per context).
+void init_kernels_f32() {
+    allow_big_lds(k_rows<float, 8>);
+    allow_big_lds(k_rows<float, 16>);
+    allow_big_lds(k_fs_a<float, 8>);
+    allow_big_lds(k_fs_a<float, 16>);
+    allow_big_lds(k_fs_b<float, 8>);
+    allow_big_lds(k_fs_b<float, 16>);
+    allow_big_lds(k_conv_small<float, 8>);
+    allow_big_lds(k_conv_small<float, 16>);
+    allow_big_lds(k_rows_multi<float>);
+    allow_big_lds(k_fs_a_multi<float>);
+    allow_big_lds(k_fs_b_multi<float>);
+    allow_big_lds(k_conv_multi<float>);
+}
 }  // namespace cfft

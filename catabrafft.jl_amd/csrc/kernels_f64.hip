@@ -43,4 +43,19 @@ hipError_t gs_multi_f32_f64(const MultiLaunch& L, const MultiArgs& ma, hipStream
 hipError_t gs_multi_f64_f64(const MultiLaunch& L, const MultiArgs& ma, hipStream_t st) {
     return gs_multi_t<double, double>(L, ma, st);
 }
+// Lets every kernel of this precision use up to 160 KiB of LDS (called once per context).
+void init_kernels_f64() {
+    allow_big_lds(k_rows<double, 8>);
+    allow_big_lds(k_rows<double, 16>);
+    allow_big_lds(k_fs_a<double, 8>);
+    allow_big_lds(k_fs_a<double, 16>);
+    allow_big_lds(k_fs_b<double, 8>);
+    allow_big_lds(k_fs_b<double, 16>);
+    allow_big_lds(k_conv_small<double, 8>);
+    allow_big_lds(k_conv_small<double, 16>);
+    allow_big_lds(k_rows_multi<double>);
+    allow_big_lds(k_fs_a_multi<double>);
+    allow_big_lds(k_fs_b_multi<double>);
+    allow_big_lds(k_conv_multi<double>);
+}
 }  // namespace cfft

@@ -579,8 +579,6 @@ void allow_big_lds(K kernel) {
 template <typename T, int E>
 hipError_t rows_t(const void* in, void* out, const RowsArgs& a, const PassTable& pt,
                   const void* tw, hipStream_t st) {
-    static bool once = (allow_big_lds(k_rows<T, E>), true);
-    (void)once;
     const size_t lds = 2 * sizeof(T) * (size_t)a.C * a.Ls;
     const int64_t blocks = (a.nlines + a.C - 1) / a.C;
     hipLaunchKernelGGL((k_rows<T, E>), dim3((unsigned)blocks), dim3(a.C * a.Tt), lds, st,
@@ -591,8 +589,6 @@ hipError_t rows_t(const void* in, void* out, const RowsArgs& a, const PassTable&
 template <typename T, int E>
 hipError_t fsa_t(const void* in, void* out, const FourStepAArgs& a, const PassTable& pt,
                  const void* tw, hipStream_t st) {
-    static bool once = (allow_big_lds(k_fs_a<T, E>), true);
-    (void)once;
     const size_t lds = 2 * sizeof(T) * (size_t)a.C * a.Ls;
     const int64_t blocks = (int64_t)((a.N2 + a.C - 1) / a.C) * a.batch;
     hipLaunchKernelGGL((k_fs_a<T, E>), dim3((unsigned)blocks), dim3(a.C * a.Tt), lds, st,
@@ -603,8 +599,6 @@ hipError_t fsa_t(const void* in, void* out, const FourStepAArgs& a, const PassTa
 template <typename T, int E>
 hipError_t fsb_t(const void* in, void* out, const FourStepBArgs& a, const PassTable& pt,
                  const void* tw, hipStream_t st) {
-    static bool once = (allow_big_lds(k_fs_b<T, E>), true);
-    (void)once;
     const size_t lds = 2 * sizeof(T) * (size_t)a.C * a.Ls;
     const int64_t blocks = (int64_t)((a.N1 + a.C - 1) / a.C) * a.batch;
     hipLaunchKernelGGL((k_fs_b<T, E>), dim3((unsigned)blocks), dim3(a.C * a.Tt), lds, st,
@@ -615,8 +609,6 @@ hipError_t fsb_t(const void* in, void* out, const FourStepBArgs& a, const PassTa
 template <typename T, int E>
 hipError_t conv_t(const void* in, void* out, const ConvSmallArgs& a, const PassTable& pt,
                   const void* tw, hipStream_t st) {
-    static bool once = (allow_big_lds(k_conv_small<T, E>), true);
-    (void)once;
     const size_t lds =
         2 * sizeof(T) * (size_t)a.C * (a.LsX + a.LsA) + sizeof(V_t<T>) * (size_t)a.C;
     const int64_t blocks = (a.nlines + a.C - 1) / a.C;
@@ -648,9 +640,6 @@ hipError_t scatter_t(const void* w, void* y, const ConvScatterArgs& a, hipStream
 // Work-list launch for data precision T (rows, four-step, fused convolution).
 template <typename T>
 hipError_t multi_t(const MultiLaunch& L, const MultiArgs& ma, hipStream_t st) {
-    static bool once = (allow_big_lds(k_rows_multi<T>), allow_big_lds(k_fs_a_multi<T>),
-                        allow_big_lds(k_fs_b_multi<T>), allow_big_lds(k_conv_multi<T>), true);
-    (void)once;
     const dim3 grid((unsigned)L.nitems), block(L.block);
     switch ((MultiKernel)L.kernel) {
         case MultiKernel::Rows:

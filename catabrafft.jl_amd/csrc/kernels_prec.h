@@ -19,6 +19,9 @@ CFFT_DECLARE_LAUNCHERS(f32)
 CFFT_DECLARE_LAUNCHERS(f64)
 #undef CFFT_DECLARE_LAUNCHERS
 
+void init_kernels_f32();
+void init_kernels_f64();
+
 hipError_t gather_f32_f32(const void* x, void* w, const ConvGatherArgs& a, hipStream_t st);
 hipError_t gather_f32_f64(const void* x, void* w, const ConvGatherArgs& a, hipStream_t st);
 hipError_t gather_f64_f64(const void* x, void* w, const ConvGatherArgs& a, hipStream_t st);

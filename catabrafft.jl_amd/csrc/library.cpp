@@ -472,6 +472,7 @@ static int plan_conv(Ctx* ctx, Plan& p, const PlanOptions& o) {
     if (rc) return rc;
     if (s.kind != Kind::Direct && s.kind != Kind::FourStep)
         return fail(CFFT_ERR_UNSUPPORTED, "convolution length not plannable");
+    if (!o.alloc_work) return CFFT_OK;  // the caller supplies per-transform work buffers
     const size_t wbytes = (size_t)p.batch * p.L * esize(p.cprec);
     if ((rc = p.w1.alloc(wbytes))) return rc;
     if ((rc = p.w2.alloc(wbytes))) return rc;
@@ -746,6 +747,7 @@ int ctx_create(int device, Ctx** out) {
     CFFT_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, device));
     c->num_cu = v;
     CFFT_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    init_kernels();
     *out = c.release();
     return CFFT_OK;
 }

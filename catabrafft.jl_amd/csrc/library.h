@@ -82,6 +82,7 @@ struct PlanOptions {
     int ca = 0, cb = 0;        // four-step column tiles (0 = default)
     int force_conv = 0;
     bool explicit_opts = false;
+    bool alloc_work = true;    // large convolutions: allocate work buffers for `batch` lines
 };
 
 using PlanKey = std::tuple<int64_t, int64_t, int64_t, int64_t, int, int>;

@@ -110,12 +110,9 @@ static int srv_plan(Server* s, int64_t n, int dtype, Plan** out, size_t* new_byt
         p->C = 256;
     } else {
         PlanOptions o;
+        o.alloc_work = false;  // a large convolution's work buffers come with each request
         if ((rc = plan_build(s->ctx, *p, o))) return rc;
     }
-    // a large convolution's per-transform work comes from the request, not from the plan
-    p->w1.reset();
-    p->w2.reset();
-    p->dc.reset();
     *new_bytes = plan_device_bytes(*p);
     *out = p.get();
     s->plans[key] = std::move(p);
@@ -407,6 +404,18 @@ int cfft_server_stats(cfft_server srv, double* p50, double* p99, double* transfo
         *transforms_per_s = (double)s->transforms / span;
         *gflops = s->flops / span / 1e9;
     }
+    return CFFT_OK;
+}
+
+int cfft_server_reset_stats(cfft_server srv) {
+    if (!srv) return fail(CFFT_ERR_INVALID, "null server");
+    Server* s = srv->s;
+    std::lock_guard<std::mutex> g(s->mu);
+    s->lat.clear();
+    s->first_submit = -1;
+    s->last_done = 0;
+    s->flops = 0;
+    s->transforms = 0;
     return CFFT_OK;
 }
 

@@ -133,6 +133,8 @@ CFFT_API int cfft_server_wait(cfft_server srv, int64_t ticket);
  * GFLOP/s at 5 n log2 n) over every completed request so far. */
 CFFT_API int cfft_server_stats(cfft_server srv, double* p50, double* p99, double* transforms_per_s,
                       double* gflops);
+/* Forgets the latency / throughput history (resident plans stay). */
+CFFT_API int cfft_server_reset_stats(cfft_server srv);
 CFFT_API int cfft_server_destroy(cfft_server srv);
 
 #ifdef __cplusplus

@@ -16,6 +16,16 @@
 
 #include "codelets.h"
 
+// Dynamic LDS of a kernel, and an opaque register copy (an empty asm the compiler cannot see
+// through).  Both are macros so that the kernel sources also compile in the CPU emulation
+// harness of the tests (tests/emu), which supplies its own definitions.
+#ifndef CFFT_DYN_LDS
+#define CFFT_DYN_LDS(name) extern __shared__ __align__(16) unsigned char name[]
+#endif
+#ifndef CFFT_OPAQUE
+#define CFFT_OPAQUE(x) asm volatile("" : "+v"(x))
+#endif
+
 namespace cfft {
 
 template <typename T> struct Vec2;
@@ -78,7 +88,7 @@ __device__ __forceinline__ void stockham_pass(const int N, const int s, const in
     // Keep per-pass address arithmetic inside the pass: an opaque copy of the lane index stops
     // the compiler from hoisting every radix's addresses out of the pass loop.
     int t_ = t_in;
-    asm volatile("" : "+v"(t_));
+    CFFT_OPAQUE(t_);
     const int t = t_;
     const int nb = N / R;
     V v[MB][R];

@@ -122,7 +122,7 @@ template <typename T, int E>
 __global__ void __launch_bounds__(1024)
 k_rows(const V_t<T>* __restrict__ in, V_t<T>* __restrict__ out, RowsArgs a, PassTable pt,
        const V_t<T>* __restrict__ tw) {
-    extern __shared__ __align__(16) unsigned char smem[];
+    CFFT_DYN_LDS(smem);
     const int64_t line0 = (int64_t)blockIdx.x * a.C;
     const int64_t left = a.nlines - line0;
     const int count = (int)(left < a.C ? left : a.C);
@@ -183,7 +183,7 @@ template <typename T, int E>
 __global__ void __launch_bounds__(1024)
 k_fs_a(const V_t<T>* __restrict__ in, V_t<T>* __restrict__ out, FourStepAArgs a, PassTable pt,
        const V_t<T>* __restrict__ tw) {
-    extern __shared__ __align__(16) unsigned char smem[];
+    CFFT_DYN_LDS(smem);
     const int tiles = (a.N2 + a.C - 1) / a.C;
     const int64_t bt = blockIdx.x / tiles;
     const int tile = blockIdx.x - (int)(bt * tiles);
@@ -240,7 +240,7 @@ template <typename T, int E>
 __global__ void __launch_bounds__(1024)
 k_fs_b(const V_t<T>* in, V_t<T>* out, FourStepBArgs a, PassTable pt,
        const V_t<T>* __restrict__ tw) {
-    extern __shared__ __align__(16) unsigned char smem[];
+    CFFT_DYN_LDS(smem);
     const int tiles = (a.N1 + a.C - 1) / a.C;
     const int64_t bt = blockIdx.x / tiles;
     const int tile = blockIdx.x - (int)(bt * tiles);
@@ -357,7 +357,7 @@ template <typename T, int E>
 __global__ void __launch_bounds__(1024)
 k_conv_small(const V_t<T>* __restrict__ in, V_t<T>* __restrict__ out, ConvSmallArgs a,
              PassTable pt, const V_t<T>* __restrict__ tw) {
-    extern __shared__ __align__(16) unsigned char smem[];
+    CFFT_DYN_LDS(smem);
     const int64_t line0 = (int64_t)blockIdx.x * a.C;
     const int nl = (int)((a.nlines - line0) < a.C ? (a.nlines - line0) : a.C);
     UniformLines<T> lines{in + line0 * a.idist, out + line0 * a.odist, a.idist, a.odist, nullptr};
@@ -467,7 +467,7 @@ __device__ __forceinline__ void item_direction(const MItem& it, int n, const Mul
 template <typename T>
 __global__ void __launch_bounds__(1024)
 k_rows_multi(MultiArgs ma, const MItem* __restrict__ items, const MPlan* __restrict__ plans) {
-    extern __shared__ __align__(16) unsigned char smem[];
+    CFFT_DYN_LDS(smem);
     const MItem it = items[blockIdx.x];
     const MPlan* P = plans + it.plan;
     int sgn;
@@ -482,7 +482,7 @@ k_rows_multi(MultiArgs ma, const MItem* __restrict__ items, const MPlan* __restr
 template <typename T>
 __global__ void __launch_bounds__(1024)
 k_fs_a_multi(MultiArgs ma, const MItem* __restrict__ items, const MPlan* __restrict__ plans) {
-    extern __shared__ __align__(16) unsigned char smem[];
+    CFFT_DYN_LDS(smem);
     const MItem it = items[blockIdx.x];
     const MPlan* P = plans + it.plan;
     int sgn;
@@ -498,7 +498,7 @@ k_fs_a_multi(MultiArgs ma, const MItem* __restrict__ items, const MPlan* __restr
 template <typename T>
 __global__ void __launch_bounds__(1024)
 k_fs_b_multi(MultiArgs ma, const MItem* __restrict__ items, const MPlan* __restrict__ plans) {
-    extern __shared__ __align__(16) unsigned char smem[];
+    CFFT_DYN_LDS(smem);
     const MItem it = items[blockIdx.x];
     const MPlan* P = plans + it.plan;
     int sgn;
@@ -516,7 +516,7 @@ k_fs_b_multi(MultiArgs ma, const MItem* __restrict__ items, const MPlan* __restr
 template <typename T>
 __global__ void __launch_bounds__(1024)
 k_conv_multi(MultiArgs ma, const MItem* __restrict__ items, const MPlan* __restrict__ plans) {
-    extern __shared__ __align__(16) unsigned char smem[];
+    CFFT_DYN_LDS(smem);
     const MItem it = items[blockIdx.x];
     const MPlan* P = plans + it.plan;
     int sgn;

@@ -1,0 +1,49 @@
+#!/usr/bin/env python3
+"""Builds tests/emu/_build/libcatabrafft_emu.so: the library's own sources (planner, C ABI,
+kernels) compiled with g++ against the CPU emulation runtime in this folder.  TEST
+INFRASTRUCTURE ONLY: loaded by tests/test_emu.py, never by the product."""
+import concurrent.futures as cf
+import glob
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+CSRC = os.path.join(ROOT, "catabrafft.jl_amd", "csrc")
+OUT = os.path.join(HERE, "_build")
+LIB = os.path.join(OUT, "libcatabrafft_emu.so")
+FLAGS = ["-std=c++20", "-O1", "-fPIC", "-ffp-contract=off", "-w",
+         "-I", os.path.join(HERE, "include"), "-I", CSRC, "-I", os.path.join(ROOT, "include")]
+
+
+def sources():
+    return ([os.path.join(HERE, "emu.cpp")] + sorted(glob.glob(os.path.join(CSRC, "*.cpp"))) +
+            sorted(glob.glob(os.path.join(CSRC, "*.hip"))))
+
+
+def deps():
+    return (glob.glob(os.path.join(CSRC, "*.h")) + glob.glob(os.path.join(ROOT, "include", "*.h"))
+            + [os.path.join(HERE, "include", "hip", "hip_runtime.h")])
+
+
+def compile_one(src):
+    obj = os.path.join(OUT, os.path.basename(src) + ".o")
+    newest = max([os.path.getmtime(src)] + [os.path.getmtime(d) for d in deps()])
+    if os.path.exists(obj) and os.path.getmtime(obj) >= newest:
+        return obj
+    cmd = ["g++"] + FLAGS + ["-x", "c++", "-c", src, "-o", obj]
+    subprocess.run(cmd, check=True)
+    return obj
+
+
+def build():
+    os.makedirs(OUT, exist_ok=True)
+    with cf.ThreadPoolExecutor(max_workers=8) as ex:
+        objs = list(ex.map(compile_one, sources()))
+    subprocess.run(["g++", "-shared", "-o", LIB] + objs + ["-lpthread"], check=True)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build())

@@ -1,0 +1,185 @@
+"""The library's own sources (planner, C ABI, HIP kernel bodies) built for the CPU emulation
+runtime in tests/emu and checked against the oracle on small cases.
+
+This runs the same kernel code paths as the GPU build (pass sequences, LDS layouts, four-step
+tiling, Rader / Bluestein, work lists, hipGraph capture and the server's rounds) without a GPU, so
+indexing and planning mistakes are caught on the CPU.  It says nothing about GPU timing or about
+compiler behaviour on gfx950; the GPU suite covers that.
+"""
+import ctypes
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import ROOT, rel_l2, seeded_complex
+
+EMU = os.path.join(ROOT, "tests", "emu")
+
+
+@pytest.fixture(scope="module")
+def E():
+    subprocess.run([sys.executable, os.path.join(EMU, "build_emu.py")], check=True,
+                   capture_output=True)
+    import catabrafft._lib as L
+
+    lib = L._sig(ctypes.CDLL(os.path.join(EMU, "_build", "libcatabrafft_emu.so")))
+    ctx = ctypes.c_void_p()
+    assert lib.cfft_init(0, ctypes.byref(ctx)) == 0
+    return lib, ctx
+
+
+def _plan(lib, ctx, n, batch, dt, idist=None, odist=None, flags=0):
+    p = ctypes.c_void_p()
+    rc = lib.cfft_plan_1d(ctx, n, batch, idist or n, odist or n, 0 if dt == np.complex64 else 1,
+                          flags, ctypes.byref(p))
+    assert rc == 0, lib.cfft_last_error()
+    return p
+
+
+def _desc(lib, p):
+    buf = ctypes.create_string_buffer(1024)
+    lib.cfft_plan_describe(p, buf, len(buf))
+    return buf.value.decode()
+
+
+def _tol(n, dt):
+    return 1e-6 if dt == np.complex64 else 4 * np.finfo(float).eps * max(1, np.log2(n))
+
+
+CASES = [  # (n, batch, dtype, expected strategy)
+    (1, 3, np.complex64, "trivial"), (2, 4, np.complex128, "direct"), (16, 5, np.complex64, "direct"),
+    (60, 5, np.complex64, "direct"), (1000, 3, np.complex64, "direct"),
+    (1024, 2, np.complex128, "direct"), (2520, 2, np.complex64, "direct"),
+    (2520, 2, np.complex128, "direct"), (11 * 13 * 8, 2, np.complex64, "direct"),
+    (17, 7, np.complex64, "rader-fused"), (97, 3, np.complex128, "rader-fused"),
+    (257, 2, np.complex64, "rader-fused"), (4099, 1, np.complex64, "rader-fused"),
+    (17 * 19, 2, np.complex64, "bluestein-fused"), (1 << 16, 1, np.complex64, "fourstep"),
+    (1 << 17, 1, np.complex128, "fourstep"), (65537, 1, np.complex64, "rader-large"),
+    (3 * 8191, 1, np.complex64, "bluestein-large"),
+]
+
+
+@pytest.mark.parametrize("n,batch,dt,kind", CASES)
+def test_emulated_kernels_match_oracle(E, n, batch, dt, kind):
+    lib, ctx = E
+    x = seeded_complex(n, batch=batch, seed=n, dtype=dt)
+    p = _plan(lib, ctx, n, batch, dt)
+    assert kind in _desc(lib, p)
+    for direction, norm in ((-1, 0), (1, 1), (1, 0)):
+        y = np.zeros_like(x)
+        assert lib.cfft_exec(p, x.ctypes.data, y.ctypes.data, direction, norm, None) == 0
+        ref = O.fft(x.astype(np.complex128), "d", direction, bool(norm))
+        assert rel_l2(y, ref) < _tol(n, dt), (direction, norm)
+    lib.cfft_plan_destroy(p)
+
+
+@pytest.mark.parametrize("n", [2520, 4099, 1 << 16, 65537])
+def test_emulated_in_place_and_strided(E, n):
+    lib, ctx = E
+    batch, idist, odist = 2, n + 3, n + 7
+    x = seeded_complex(idist * batch, seed=3, dtype=np.complex64)
+    y = np.zeros(odist * batch, dtype=np.complex64)
+    p = _plan(lib, ctx, n, batch, np.complex64, idist, odist)
+    assert lib.cfft_exec(p, x.ctypes.data, y.ctypes.data, -1, 0, None) == 0
+    for b in range(batch):
+        ref = O.fft(x[b * idist:b * idist + n].astype(np.complex128), "d")
+        assert rel_l2(y[b * odist:b * odist + n], ref) < 1e-6
+        assert np.all(y[b * odist + n:(b + 1) * odist] == 0)
+    z = seeded_complex(n, batch=2, seed=4, dtype=np.complex64)
+    z0 = z.copy()
+    q = _plan(lib, ctx, n, 2, np.complex64)
+    assert lib.cfft_exec(q, z.ctypes.data, z.ctypes.data, -1, 0, None) == 0
+    assert rel_l2(z, O.fft(z0.astype(np.complex128), "d")) < 1e-6
+
+
+def test_emulated_uneven_plan_and_alone_equality(E):
+    lib, ctx = E
+    lengths = [64, 1000, 4099, 2520, 1 << 16, 12345, 7, 1, 2, 17, 17, 60, 60, 60, 3 * 8191, 65537]
+    offs = np.cumsum([0] + [n + 3 for n in lengths[:-1]]).astype(np.int64)
+    span = int(offs[-1] + lengths[-1])
+    le = np.array(lengths, dtype=np.int64)
+    P = ctypes.POINTER(ctypes.c_int64)
+    p = ctypes.c_void_p()
+    assert lib.cfft_plan_many_uneven(ctx, len(lengths), le.ctypes.data_as(P),
+                                     offs.ctypes.data_as(P), 0, 0, ctypes.byref(p)) == 0
+    x = seeded_complex(span, seed=1, dtype=np.complex64)
+    for direction, norm in ((-1, 0), (1, 1)):
+        y = np.zeros_like(x)
+        assert lib.cfft_exec(p, x.ctypes.data, y.ctypes.data, direction, norm, None) == 0
+        for n, o in zip(lengths, offs):
+            ref = O.fft(x[o:o + n].astype(np.complex128), "d", direction, bool(norm))
+            assert rel_l2(y[o:o + n], ref) < 1e-6, n
+            # alone (single-length plan) and in the mix give identical bits
+            q = _plan(lib, ctx, n, 1, np.complex64)
+            alone = np.zeros(n, dtype=np.complex64)
+            assert lib.cfft_exec(q, x[o:].ctypes.data, alone.ctypes.data, direction, norm, None) == 0
+            assert np.array_equal(alone, y[o:o + n]), n
+        assert np.all(y[offs[0] + lengths[0]:offs[1]] == 0)
+
+
+def test_emulated_server_rounds(E):
+    lib, ctx = E
+    srv = ctypes.c_void_p()
+    assert lib.cfft_server_create(ctx, 1 << 30, 2, ctypes.byref(srv)) == 0
+    reqs = []
+    for i, (n, b) in enumerate([(60, 3), (1024, 2), (17, 4), (4099, 1), (1 << 16, 1), (60, 2),
+                                 (3 * 8191, 1)]):
+        dt = np.complex128 if i == 1 else np.complex64
+        x = seeded_complex(n, batch=b, seed=i, dtype=dt)
+        y = np.zeros_like(x)
+        t = ctypes.c_int64()
+        direction = -1 if i % 2 == 0 else 1
+        assert lib.cfft_server_submit(srv, n, b, 0 if dt == np.complex64 else 1, direction,
+                                      int(direction > 0), x.ctypes.data, y.ctypes.data, i % 2,
+                                      ctypes.byref(t)) == 0
+        reqs.append((t.value, x, y, direction))
+    for t, x, y, direction in reqs:
+        assert lib.cfft_server_wait(srv, t) == 0
+        ref = O.fft(x.astype(np.complex128), "d", direction, direction > 0)
+        assert rel_l2(y, ref) < _tol(x.shape[1], x.dtype)
+    v = [ctypes.c_double() for _ in range(4)]
+    assert lib.cfft_server_stats(srv, *[ctypes.byref(a) for a in v]) == 0
+    assert v[0].value > 0 and v[1].value >= v[0].value and v[2].value > 0
+    # a request above the cap is refused
+    big = seeded_complex(1 << 16, batch=8, seed=9, dtype=np.complex64)
+    out = np.zeros_like(big)
+    srv2 = ctypes.c_void_p()
+    assert lib.cfft_server_create(ctx, big.nbytes, 1, ctypes.byref(srv2)) == 0
+    t = ctypes.c_int64()
+    assert lib.cfft_server_submit(srv2, 1 << 16, 8, 0, -1, 0, big.ctypes.data, out.ctypes.data, 0,
+                                  ctypes.byref(t)) == 0
+    assert lib.cfft_server_wait(srv2, t.value) == -5
+    assert lib.cfft_server_destroy(srv2) == 0
+    assert lib.cfft_server_destroy(srv) == 0
+
+
+def test_emulated_measure_flags(E):
+    lib, ctx = E
+    x = seeded_complex(1024, batch=2, seed=2, dtype=np.complex64)
+    for flag in (1, 2):
+        p = _plan(lib, ctx, 1024, 2, np.complex64, flags=flag)
+        assert "tuned" in _desc(lib, p)
+        y = np.zeros_like(x)
+        assert lib.cfft_exec(p, x.ctypes.data, y.ctypes.data, -1, 0, None) == 0
+        assert rel_l2(y, O.fft(x.astype(np.complex128), "d")) < 1e-6
+
+
+def test_emulated_argument_errors(E):
+    lib, ctx = E
+    p = ctypes.c_void_p()
+    assert lib.cfft_plan_1d(ctx, 0, 1, 1, 1, 0, 0, ctypes.byref(p)) == -1
+    assert lib.cfft_plan_1d(ctx, 16, 1, 8, 16, 0, 0, ctypes.byref(p)) == -1
+    assert lib.cfft_plan_1d(ctx, 16, 1, 16, 16, 5, 0, ctypes.byref(p)) == -1
+    assert lib.cfft_plan_1d(ctx, 16, 1, 16, 16, 0, 9, ctypes.byref(p)) == -1
+    q = _plan(lib, ctx, 16, 1, np.complex64)
+    assert lib.cfft_exec(q, None, None, -1, 0, None) == -1
+    assert lib.cfft_exec(q, 1, 1, 0, 0, None) == -1
+    le = np.array([8, 8], dtype=np.int64)
+    of = np.array([0, 4], dtype=np.int64)
+    P = ctypes.POINTER(ctypes.c_int64)
+    assert lib.cfft_plan_many_uneven(ctx, 2, le.ctypes.data_as(P), of.ctypes.data_as(P), 0, 0,
+                                     ctypes.byref(p)) == -1

@@ -34,7 +34,8 @@ NO_FLAG, MEASURE, ENCHANT = 0, 1, 2
 
 __all__ = ["fft", "ifft", "bfft", "plan_fft", "plan_bfft", "plan_ifft", "mul", "empty_cache",
            "Spell", "Plan", "UnevenPlan", "Server", "Context", "context", "NO_FLAG", "MEASURE",
-           "ENCHANT", "CatabraFFTError", "CatabraFFTArgumentError"]
+           "ENCHANT", "CatabraFFTError", "CatabraFFTArgumentError", "export_wisdom",
+           "import_wisdom"]
 
 
 # ------------------------------------------------------------------------------ context
@@ -390,6 +391,16 @@ def mul(y, p: Spell, x):
     out = p._plan.exec_host(np.asarray(x), None, p.direction, p.normalize)
     y[...] = out
     return y
+
+
+def export_wisdom(path: str, device: int = 0):
+    """Writes the MEASURE / ENCHANT choices made so far to `path`."""
+    check(lib().cfft_wisdom_export(context(device).h, str(path).encode()))
+
+
+def import_wisdom(path: str, device: int = 0):
+    """Loads choices written by export_wisdom; later MEASURE / ENCHANT plans reuse them."""
+    check(lib().cfft_wisdom_import(context(device).h, str(path).encode()))
 
 
 def empty_cache():

@@ -70,6 +70,8 @@ def _sig(L):
     _set(L, "cfft_plan_describe", [vp, ctypes.c_char_p, i64])
     _set(L, "cfft_plan_destroy", [vp])
     _set(L, "cfft_empty_cache", [vp])
+    _set(L, "cfft_wisdom_export", [vp, ctypes.c_char_p])
+    _set(L, "cfft_wisdom_import", [vp, ctypes.c_char_p])
     _set(L, "cfft_malloc", [vp, i64, P(vp)])
     _set(L, "cfft_free", [vp, vp])
     _set(L, "cfft_memcpy_h2d", [vp, vp, vp, i64])

@@ -10,6 +10,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
 #include <cmath>
 #include <sstream>
 #include <vector>
@@ -86,7 +89,72 @@ static void copy_shape(const Plan& from, Plan& to) {
     to.prec = from.prec;
 }
 
+// ------------------------------------------------------------------------------- wisdom
+// Tuned choices keyed by (n, batch, idist, odist, dtype, flags); kept per context and written /
+// read as text lines "n batch idist odist dtype flags : E target_threads n1 ca cb".
+static std::string wisdom_key(const Plan& p, int flags) {
+    std::ostringstream k;
+    k << p.n << " " << p.batch << " " << p.idist << " " << p.odist << " " << p.dtype << " " << flags;
+    return k.str();
+}
+
+int wisdom_export(Ctx* ctx, const char* path) {
+    std::lock_guard<std::mutex> g(ctx->mu);
+    FILE* f = std::fopen(path, "w");
+    if (!f) return fail(CFFT_ERR_INVALID, std::string("cannot write ") + path);
+    std::fprintf(f, "# catabrafft wisdom v1 (gfx950)\n");
+    for (const auto& kv : ctx->wisdom) {
+        const PlanOptions& o = kv.second;
+        std::fprintf(f, "%s : %d %d %lld %d %d\n", kv.first.c_str(), o.E, o.target_threads,
+                     (long long)o.n1, o.ca, o.cb);
+    }
+    std::fclose(f);
+    return CFFT_OK;
+}
+
+int wisdom_import(Ctx* ctx, const char* path) {
+    FILE* f = std::fopen(path, "r");
+    if (!f) return fail(CFFT_ERR_INVALID, std::string("cannot read ") + path);
+    char line[512];
+    int n = 0;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    while (std::fgets(line, sizeof line, f)) {
+        if (line[0] == '#') continue;
+        char* colon = std::strchr(line, ':');
+        if (!colon) continue;
+        *colon = 0;
+        std::string key(line);
+        while (!key.empty() && key.back() == ' ') key.pop_back();
+        PlanOptions o;
+        long long n1 = 0;
+        if (std::sscanf(colon + 1, "%d %d %lld %d %d", &o.E, &o.target_threads, &n1, &o.ca,
+                        &o.cb) != 5) {
+            std::fclose(f);
+            return fail(CFFT_ERR_INVALID, "malformed wisdom line");
+        }
+        o.n1 = n1;
+        o.explicit_opts = true;
+        ctx->wisdom[key] = o;
+        ++n;
+    }
+    std::fclose(f);
+    return n;
+}
+
 int autotune(Ctx* ctx, Plan& p, int flags, const PlanOptions& base) {
+    const std::string wkey = wisdom_key(p, flags);
+    {
+        std::unique_lock<std::mutex> g(ctx->mu);
+        auto it = ctx->wisdom.find(wkey);
+        if (it != ctx->wisdom.end()) {
+            const PlanOptions o = it->second;
+            g.unlock();
+            if (plan_build(ctx, p, o) == CFFT_OK) {
+                p.desc = plan_description(p) + " (wisdom)";
+                return CFFT_OK;
+            }
+        }
+    }
     std::vector<PlanOptions> cands = candidates(ctx, p, flags);
     if (cands.size() <= 1 || p.batch == 0) return plan_build(ctx, p, base);
     const size_t es = p.prec == Prec::F32 ? 8 : 16;
@@ -142,6 +210,10 @@ int autotune(Ctx* ctx, Plan& p, int flags, const PlanOptions& base) {
     if (!found) return plan_build(ctx, p, base);
     rc = plan_build(ctx, p, bestopt);
     if (rc) return rc;
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        ctx->wisdom[wkey] = bestopt;
+    }
     std::ostringstream os;
     os << " tuned(" << (flags >= CFFT_ENCHANT ? "enchant" : "measure") << ", " << cands.size()
        << " candidates, " << best / 3.0 << " ms)";

@@ -901,6 +901,17 @@ int cfft_empty_cache(cfft_ctx ctx) {
     return CFFT_OK;
 }
 
+int cfft_wisdom_export(cfft_ctx ctx, const char* path) {
+    if (!ctx || !path) return fail(CFFT_ERR_INVALID, "null argument");
+    return wisdom_export(ctx->c, path);
+}
+
+int cfft_wisdom_import(cfft_ctx ctx, const char* path) {
+    if (!ctx || !path) return fail(CFFT_ERR_INVALID, "null argument");
+    const int n = wisdom_import(ctx->c, path);
+    return n < 0 ? n : CFFT_OK;
+}
+
 int cfft_malloc(cfft_ctx ctx, int64_t bytes, void** ptr) {
     if (!ctx || !ptr || bytes < 0) return fail(CFFT_ERR_INVALID, "bad argument");
     CFFT_HIP(hipSetDevice(ctx->c->device));

@@ -97,6 +97,7 @@ struct Ctx {
     std::map<PlanKey, std::shared_ptr<Plan>> cache;
     std::mutex stage_mu;
     DevBuf stage_in, stage_out;
+    std::map<std::string, PlanOptions> wisdom;  // tuned choices (autotune.cpp), guarded by mu
 };
 
 int ctx_create(int device, Ctx** out);
@@ -133,6 +134,8 @@ bool line_fits_lds(Ctx* ctx, int64_t N, Prec pr, int E);
 
 // autotune.cpp: MEASURE / ENCHANT planning (times candidate PlanOptions on the device).
 int autotune(Ctx* ctx, Plan& p, int flags, const PlanOptions& base);
+int wisdom_export(Ctx* ctx, const char* path);
+int wisdom_import(Ctx* ctx, const char* path);  // returns the number of entries read
 
 // uneven.cpp: many transforms of different lengths in one plan.
 int uneven_create(Ctx* ctx, int64_t count, const int64_t* lengths, const int64_t* offsets,

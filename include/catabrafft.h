@@ -106,6 +106,11 @@ CFFT_API int cfft_plan_destroy(cfft_plan plan);
  * Plans still held by callers stay valid. */
 CFFT_API int cfft_empty_cache(cfft_ctx ctx);
 
+/* Wisdom: the choices MEASURE / ENCHANT planning made, as a text file, so a later process
+ * reuses them without timing again (the plan serialisation drafted in spells.jl:180-374). */
+CFFT_API int cfft_wisdom_export(cfft_ctx ctx, const char* path);
+CFFT_API int cfft_wisdom_import(cfft_ctx ctx, const char* path);
+
 /* ---- device memory helpers for hosts without their own allocator (Julia shim, C drivers) */
 CFFT_API int cfft_malloc(cfft_ctx ctx, int64_t bytes, void** ptr);
 CFFT_API int cfft_free(cfft_ctx ctx, void* ptr);

@@ -183,3 +183,24 @@ def test_emulated_argument_errors(E):
     P = ctypes.POINTER(ctypes.c_int64)
     assert lib.cfft_plan_many_uneven(ctx, 2, le.ctypes.data_as(P), of.ctypes.data_as(P), 0, 0,
                                      ctypes.byref(p)) == -1
+
+
+def test_emulated_wisdom_round_trip(E, tmp_path):
+    lib, ctx = E
+    p = _plan(lib, ctx, 2048, 4, np.complex64, flags=1)
+    assert "tuned" in _desc(lib, p)
+    path = str(tmp_path / "wisdom.txt").encode()
+    assert lib.cfft_wisdom_export(ctx, path) == 0
+    text = open(path.decode()).read()
+    assert "2048 4 2048 2048 0 1 :" in text
+    ctx2 = ctypes.c_void_p()
+    assert lib.cfft_init(0, ctypes.byref(ctx2)) == 0
+    assert lib.cfft_wisdom_import(ctx2, path) == 0
+    q = _plan(lib, ctx2, 2048, 4, np.complex64, flags=1)
+    assert "(wisdom)" in _desc(lib, q)
+    x = seeded_complex(2048, batch=4, seed=8, dtype=np.complex64)
+    y = np.zeros_like(x)
+    assert lib.cfft_exec(q, x.ctypes.data, y.ctypes.data, -1, 0, None) == 0
+    assert rel_l2(y, O.fft(x.astype(np.complex128), "d")) < 1e-6
+    assert lib.cfft_wisdom_import(ctx2, b"/nonexistent/wisdom") == -1
+    lib.cfft_finalize(ctx2)

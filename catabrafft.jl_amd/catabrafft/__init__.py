@@ -35,7 +35,7 @@ NO_FLAG, MEASURE, ENCHANT = 0, 1, 2
 __all__ = ["fft", "ifft", "bfft", "plan_fft", "plan_bfft", "plan_ifft", "mul", "empty_cache",
            "Spell", "Plan", "UnevenPlan", "Server", "Context", "context", "NO_FLAG", "MEASURE",
            "ENCHANT", "CatabraFFTError", "CatabraFFTArgumentError", "export_wisdom",
-           "import_wisdom"]
+           "import_wisdom", "rfft", "irfft", "brfft", "RealPlan"]
 
 
 # ------------------------------------------------------------------------------ context
@@ -155,6 +155,61 @@ class Plan:
         v = ctypes.c_int64()
         check(lib().cfft_plan_device_bytes(self.h, ctypes.byref(v)))
         return v.value
+
+
+class RealPlan(Plan):
+    """cfft_plan_1d_real: rows of n reals <-> half spectra of n/2+1 complex values."""
+
+    def __init__(self, n: int, batch: int = 1, dtype=np.float32, flags: int = NO_FLAG,
+                 rdist: int | None = None, cdist: int | None = None, device: int = 0):
+        self.ctx = context(device)
+        self.n, self.batch = int(n), int(batch)
+        self.idist = int(rdist if rdist is not None else n)
+        self.odist = int(cdist if cdist is not None else n // 2 + 1)
+        dt = np.dtype(dtype)
+        if dt not in (np.dtype(np.float32), np.dtype(np.float64)):
+            raise TypeError(f"real transforms take float32 / float64 rows, got {dt}")
+        self.dtype_code = C32 if dt == np.float32 else C64
+        self.rdtype = dt.type
+        self.dtype = np.complex64 if self.dtype_code == C32 else np.complex128
+        self.flags = int(flags)
+        h = ctypes.c_void_p()
+        check(lib().cfft_plan_1d_real(self.ctx.h, self.n, self.batch, self.idist, self.odist,
+                                      self.dtype_code, self.flags, ctypes.byref(h)))
+        self.h = h
+
+    def forward_host(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=self.rdtype)
+        out = np.empty(x.shape[:-1] + (self.odist,), dtype=self.dtype)
+        check(lib().cfft_exec_host(self.h, x.ctypes.data_as(ctypes.c_void_p),
+                                   out.ctypes.data_as(ctypes.c_void_p), FORWARD, 0))
+        return out[..., :self.n // 2 + 1]
+
+    def backward_host(self, X: np.ndarray, normalize: bool) -> np.ndarray:
+        X = np.ascontiguousarray(X, dtype=self.dtype)
+        out = np.empty(X.shape[:-1] + (self.idist,), dtype=self.rdtype)
+        check(lib().cfft_exec_host(self.h, X.ctypes.data_as(ctypes.c_void_p),
+                                   out.ctypes.data_as(ctypes.c_void_p), BACKWARD,
+                                   int(normalize)))
+        return out[..., :self.n]
+
+    def forward_torch(self, x):
+        t = _lib.torch
+        x = x.contiguous()
+        out = t.empty(x.shape[:-1] + (self.odist,), dtype=t.complex64 if self.dtype_code == C32
+                      else t.complex128, device=x.device)
+        self.exec_ptr(x.data_ptr(), out.data_ptr(), FORWARD, False,
+                      t.cuda.current_stream(x.device).cuda_stream)
+        return out
+
+    def backward_torch(self, X, normalize: bool):
+        t = _lib.torch
+        X = X.contiguous()
+        out = t.empty(X.shape[:-1] + (self.idist,), dtype=t.float32 if self.dtype_code == C32
+                      else t.float64, device=X.device)
+        self.exec_ptr(X.data_ptr(), out.data_ptr(), BACKWARD, normalize,
+                      t.cuda.current_stream(X.device).cuda_stream)
+        return out
 
 
 class UnevenPlan:
@@ -393,6 +448,57 @@ def mul(y, p: Spell, x):
     return y
 
 
+_rplans: dict = {}
+
+
+def _real_plan(n: int, batch: int, rdt, device: int = 0) -> RealPlan:
+    key = (n, batch, np.dtype(rdt).name, device)
+    with _plan_lock:
+        p = _rplans.get(key)
+        if p is None:
+            p = _rplans[key] = RealPlan(n, batch, rdt, device=device)
+        return p
+
+
+def rfft(x):
+    """Half spectrum (n//2 + 1 values) of real rows (AbstractFFTs.rfft)."""
+    if _is_torch(x):
+        rdt = np.float32 if x.dtype == _lib.torch.float32 else np.float64
+        n, batch = _shape(x)
+        return _real_plan(n, batch, rdt, x.device.index or 0).forward_torch(x)
+    x = np.asarray(x)
+    if x.dtype not in (np.float32, np.float64):
+        raise TypeError(f"rfft takes real float32 / float64 input, got {x.dtype}")
+    n, batch = _shape(x)
+    return _real_plan(n, batch, x.dtype).forward_host(x)
+
+
+def brfft(X, n: int):
+    """Unnormalised inverse of rfft to n reals (AbstractFFTs.brfft)."""
+    return _c2r(X, n, False)
+
+
+def irfft(X, n: int):
+    """Inverse of rfft to n reals (AbstractFFTs.irfft)."""
+    return _c2r(X, n, True)
+
+
+def _c2r(X, n: int, normalize: bool):
+    if _is_torch(X):
+        code = _torch_dtype_code(X)
+        m, batch = _shape(X)
+        if m != n // 2 + 1:
+            raise ValueError(f"a half spectrum of {m} values does not match n={n}")
+        p = _real_plan(n, batch, np.float32 if code == C32 else np.float64, X.device.index or 0)
+        return p.backward_torch(X, normalize)
+    X = np.asarray(X)
+    code = _dtype_code(X.dtype)
+    m, batch = _shape(X)
+    if m != n // 2 + 1:
+        raise ValueError(f"a half spectrum of {m} values does not match n={n}")
+    return _real_plan(n, batch, np.float32 if code == C32 else np.float64).backward_host(X, normalize)
+
+
 def export_wisdom(path: str, device: int = 0):
     """Writes the MEASURE / ENCHANT choices made so far to `path`."""
     check(lib().cfft_wisdom_export(context(device).h, str(path).encode()))
@@ -407,6 +513,7 @@ def empty_cache():
     """Drops cached plans and staging buffers (CatabraFFT.jl:39-43)."""
     with _plan_lock:
         _plans.clear()
+        _rplans.clear()
     with _ctx_lock:
         for c in _contexts.values():
             c.empty_cache()

@@ -84,6 +84,25 @@ struct ConvScatterArgs {
     const void* dc;        // per line slots {A^[0], x0} (Tc precision)
 };
 
+// Real-input transforms (n = 2m even), fused into one rows launch: forward reads `rdist`-spaced
+// rows of n reals and writes `cdist`-spaced rows of m+1 complex; backward the reverse.
+struct RealArgs {
+    int m, C, Tt, Ls;
+    int64_t nlines, rdist, cdist;  // rdist in reals (even), cdist in complex elements
+    int forward;
+    double scale;
+    const void* w;                 // w[k] = e^{-2 pi i k / n}, k < m
+};
+
+// Elementwise steps of the general real path (see k_real_aux): mode 0 r2c post, 1 c2r pre,
+// 2 reals -> complex, 3 half of a full spectrum, 4 half -> full Hermitian, 5 real part.
+struct RealAuxArgs {
+    int mode;
+    int64_t n, m, nlines, count, idist, odist;
+    double scale;
+    const void* w;
+};
+
 // Sets per-kernel attributes (large LDS) once; called when a context is created.
 void init_kernels();
 
@@ -95,6 +114,10 @@ hipError_t launch_fourstep_b(Prec p, const void* in, void* out, const FourStepBA
                              const PassTable& pt, const void* tw, hipStream_t st);
 hipError_t launch_conv_small(Prec p, const void* in, void* out, const ConvSmallArgs& a,
                              const PassTable& pt, const void* tw, hipStream_t st);
+hipError_t launch_real_rows(Prec p, const void* in, void* out, const RealArgs& a,
+                            const PassTable& pt, const void* tw, int E, hipStream_t st);
+hipError_t launch_real_aux(Prec p, const void* in, void* out, const RealAuxArgs& a,
+                           hipStream_t st);
 hipError_t launch_conv_gather(Prec pio, Prec pc, const void* x, void* w, const ConvGatherArgs& a,
                               hipStream_t st);
 hipError_t launch_conv_scatter(Prec pio, Prec pc, const void* w, void* y,

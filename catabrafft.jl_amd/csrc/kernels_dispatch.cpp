@@ -26,6 +26,15 @@ hipError_t launch_conv_small(Prec p, const void* in, void* out, const ConvSmallA
                              const PassTable& pt, const void* tw, hipStream_t st) {
     return p == Prec::F32 ? conv_f32(in, out, a, pt, tw, st) : conv_f64(in, out, a, pt, tw, st);
 }
+hipError_t launch_real_rows(Prec p, const void* in, void* out, const RealArgs& a,
+                            const PassTable& pt, const void* tw, int E, hipStream_t st) {
+    return p == Prec::F32 ? real_rows_f32(in, out, a, pt, tw, E, st)
+                          : real_rows_f64(in, out, a, pt, tw, E, st);
+}
+hipError_t launch_real_aux(Prec p, const void* in, void* out, const RealAuxArgs& a,
+                           hipStream_t st) {
+    return p == Prec::F32 ? real_aux_f32(in, out, a, st) : real_aux_f64(in, out, a, st);
+}
 hipError_t launch_conv_gather(Prec pio, Prec pc, const void* x, void* w, const ConvGatherArgs& a,
                               hipStream_t st) {
     if (pio == Prec::F32) return pc == Prec::F32 ? gather_f32_f32(x, w, a, st) : gather_f32_f64(x, w, a, st);

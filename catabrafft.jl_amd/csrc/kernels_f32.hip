@@ -33,6 +33,13 @@ hipError_t multi_f32(const MultiLaunch& L, const MultiArgs& ma, hipStream_t st) 
 hipError_t gs_multi_f32_f32(const MultiLaunch& L, const MultiArgs& ma, hipStream_t st) {
     return gs_multi_t<float, float>(L, ma, st);
 }
+hipError_t real_rows_f32(const void* in, void* out, const RealArgs& a, const PassTable& pt,
+                         const void* tw, int E, hipStream_t st) {
+    return CFFT_E(real_rows_t, float, E, in, out, a, pt, tw, st);
+}
+hipError_t real_aux_f32(const void* in, void* out, const RealAuxArgs& a, hipStream_t st) {
+    return real_aux_t<float>(in, out, a, st);
+}
 // Lets every kernel of this precision use up to 160 KiB of LDS (called once per context).
 void init_kernels_f32() {
     allow_big_lds(k_rows<float, 8>);
@@ -47,5 +54,7 @@ void init_kernels_f32() {
     allow_big_lds(k_fs_a_multi<float>);
     allow_big_lds(k_fs_b_multi<float>);
     allow_big_lds(k_conv_multi<float>);
+    allow_big_lds(k_real_rows<float, 8>);
+    allow_big_lds(k_real_rows<float, 16>);
 }
 }  // namespace cfft

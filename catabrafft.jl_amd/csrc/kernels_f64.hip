@@ -43,6 +43,13 @@ hipError_t gs_multi_f32_f64(const MultiLaunch& L, const MultiArgs& ma, hipStream
 hipError_t gs_multi_f64_f64(const MultiLaunch& L, const MultiArgs& ma, hipStream_t st) {
     return gs_multi_t<double, double>(L, ma, st);
 }
+hipError_t real_rows_f64(const void* in, void* out, const RealArgs& a, const PassTable& pt,
+                         const void* tw, int E, hipStream_t st) {
+    return CFFT_E(real_rows_t, double, E, in, out, a, pt, tw, st);
+}
+hipError_t real_aux_f64(const void* in, void* out, const RealAuxArgs& a, hipStream_t st) {
+    return real_aux_t<double>(in, out, a, st);
+}
 // Lets every kernel of this precision use up to 160 KiB of LDS (called once per context).
 void init_kernels_f64() {
     allow_big_lds(k_rows<double, 8>);
@@ -57,5 +64,7 @@ void init_kernels_f64() {
     allow_big_lds(k_fs_a_multi<double>);
     allow_big_lds(k_fs_b_multi<double>);
     allow_big_lds(k_conv_multi<double>);
+    allow_big_lds(k_real_rows<double, 8>);
+    allow_big_lds(k_real_rows<double, 16>);
 }
 }  // namespace cfft

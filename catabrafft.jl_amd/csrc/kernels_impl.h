@@ -443,6 +443,139 @@ k_conv_scatter(const V_t<Tc>* __restrict__ w, V_t<Ti>* __restrict__ y, ConvScatt
 }
 
 // ------------------------------------------------------------------------------------------
+// Real-input transforms, n = 2m even: the reals are read as m complex z[j] = x[2j] + i x[2j+1].
+//   forward:  Z = FFT_m(z);  X[k] = (Z[k] + conj Z[m-k])/2 - i w^k (Z[k] - conj Z[m-k])/2
+//   backward: Z'[k] = (X[k] + conj X[m-k]) + i conj(w^k) (X[k] - conj X[m-k]);  z = bfft_m(Z')
+// with w = e^{-2 pi i / n} (table w[k], k < m) and Z[m] = Z[0].
+// ------------------------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void r2c_pair(const V_t<T> zk, const V_t<T> zj, const V_t<T> wk,
+                                         V_t<T>& xk) {
+    const V_t<T> czj = cconj(zj);
+    const V_t<T> a = V_t<T>{(T)0.5 * (zk.x + czj.x), (T)0.5 * (zk.y + czj.y)};
+    const V_t<T> d = V_t<T>{(T)0.5 * (zk.x - czj.x), (T)0.5 * (zk.y - czj.y)};
+    const V_t<T> wd = cmul(wk, d);
+    xk = V_t<T>{a.x + wd.y, a.y - wd.x};  // a - i (w d)
+}
+
+// Half spectrum X[0..m] of one line from Z in LDS (or any Z(k) source), threads t of Tt.
+template <typename T, class ZSRC>
+__device__ __forceinline__ void r2c_post(const ZSRC& Z, V_t<T>* X, const V_t<T>* w, int m, int t,
+                                         int Tt, bool active, T scale) {
+    if (!active) return;
+    for (int k = t; k <= m / 2; k += Tt) {
+        const int j = m - k;
+        const V_t<T> zk = Z(k), zj = Z(j == m ? 0 : j);
+        V_t<T> xk, xj;
+        r2c_pair<T>(zk, zj, w[k], xk);
+        const V_t<T> wj = V_t<T>{-w[k].x, w[k].y};  // w^{m-k} = -conj(w^k)
+        r2c_pair<T>(zj, zk, wj, xj);
+        X[k] = V_t<T>{xk.x * scale, xk.y * scale};
+        X[j] = V_t<T>{xj.x * scale, xj.y * scale};
+    }
+}
+
+// conj(Z'[k]) from the half spectrum in global memory (the engine's first pass reads it).
+template <typename T>
+struct C2RSrc {
+    const V_t<T>* X;
+    const V_t<T>* w;
+    int m;
+    __device__ __forceinline__ V_t<T> operator()(int k) const {
+        const V_t<T> xk = X[k], cxj = cconj(X[m - k]);
+        const V_t<T> a = V_t<T>{xk.x + cxj.x, xk.y + cxj.y};
+        const V_t<T> d = V_t<T>{xk.x - cxj.x, xk.y - cxj.y};
+        const V_t<T> wd = cmul(cconj(w[k]), d);
+        const V_t<T> z = V_t<T>{a.x - wd.y, a.y + wd.x};  // a + i (conj(w) d)
+        return cconj(z);
+    }
+};
+
+// Fused rows: up to C lines; dir > 0 is r2c (reals -> half spectra), dir < 0 is c2r.
+template <typename T, int E>
+__global__ void __launch_bounds__(1024)
+k_real_rows(const void* __restrict__ in, void* __restrict__ out, RealArgs a, PassTable pt,
+            const V_t<T>* __restrict__ tw) {
+    CFFT_DYN_LDS(smem);
+    const int tid = threadIdx.x;
+    const int c = tid / a.Tt;
+    const int t = tid - c * a.Tt;
+    const int64_t line = (int64_t)blockIdx.x * a.C + c;
+    const bool active = (c < a.C) && (line < a.nlines);
+    const int64_t li = active ? line : 0;
+    const LdsLine<T> L = lds_line<T>(smem, a.C, a.Ls, c < a.C ? c : 0);
+    const V_t<T>* w = (const V_t<T>*)a.w;
+    if (a.forward) {
+        const V_t<T>* z = reinterpret_cast<const V_t<T>*>((const T*)in + li * a.rdist);
+        V_t<T>* X = (V_t<T>*)out + li * a.cdist;
+        GSrc<T> gs{z, (T)1};
+        run_line<T, E>(pt, a.m, t, a.Tt, active, tw, gs, L, L, true, false);
+        r2c_post<T>(L, X, w, a.m, t, a.Tt, active, (T)a.scale);
+    } else {
+        const V_t<T>* X = (const V_t<T>*)in + li * a.cdist;
+        V_t<T>* z = reinterpret_cast<V_t<T>*>((T*)out + li * a.rdist);
+        C2RSrc<T> src{X, w, a.m};
+        GRowDst<T> gd{z, (T)a.scale, (T)(-a.scale), nullptr, nullptr};
+        run_line<T, E>(pt, a.m, t, a.Tt, active, tw, src, gd, L, true, true);
+    }
+}
+
+// Elementwise helpers of the general real path (one thread per element, grid.y over lines).
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_real_aux(const void* __restrict__ in, void* __restrict__ out, RealAuxArgs a) {
+    const V_t<T>* w = (const V_t<T>*)a.w;
+    for (int64_t line = blockIdx.y; line < a.nlines; line += gridDim.y) {
+        for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < a.count;
+             k += (int64_t)gridDim.x * blockDim.x) {
+            const T s = (T)a.scale;
+            switch (a.mode) {
+                case 0: {  // r2c post: Z (m per line) -> X (m+1 per line), k <= m/2
+                    const V_t<T>* Z = (const V_t<T>*)in + line * a.idist;
+                    V_t<T>* X = (V_t<T>*)out + line * a.odist;
+                    const int64_t j = a.m - k;
+                    V_t<T> xk, xj;
+                    r2c_pair<T>(Z[k], Z[j == a.m ? 0 : j], w[k], xk);
+                    r2c_pair<T>(Z[j == a.m ? 0 : j], Z[k], V_t<T>{-w[k].x, w[k].y}, xj);
+                    X[k] = V_t<T>{xk.x * s, xk.y * s};
+                    X[j] = V_t<T>{xj.x * s, xj.y * s};
+                    break;
+                }
+                case 1: {  // c2r pre: X (m+1) -> Z' (m) scaled, k < m
+                    const V_t<T>* X = (const V_t<T>*)in + line * a.idist;
+                    V_t<T>* Z = (V_t<T>*)out + line * a.odist;
+                    const V_t<T> z = cconj(C2RSrc<T>{X, w, (int)a.m}(k));
+                    Z[k] = V_t<T>{z.x * s, z.y * s};
+                    break;
+                }
+                case 2: {  // reals -> complex with zero imaginary part, k < n
+                    const T* x = (const T*)in + line * a.idist;
+                    ((V_t<T>*)out)[line * a.odist + k] = V_t<T>{x[k] * s, (T)0};
+                    break;
+                }
+                case 3: {  // first n/2 + 1 of a full spectrum, k <= n/2
+                    ((V_t<T>*)out)[line * a.odist + k] = ((const V_t<T>*)in)[line * a.idist + k];
+                    break;
+                }
+                case 4: {  // half spectrum -> full Hermitian spectrum, k < n
+                    const V_t<T>* X = (const V_t<T>*)in + line * a.idist;
+                    const int64_t h = k <= a.n / 2 ? k : a.n - k;
+                    const V_t<T> v = X[h];
+                    ((V_t<T>*)out)[line * a.odist + k] = k <= a.n / 2 ? V_t<T>{v.x * s, v.y * s}
+                                                                      : V_t<T>{v.x * s, -v.y * s};
+                    break;
+                }
+                default: {  // 5: real part of a complex line, k < n
+                    const V_t<T> v = ((const V_t<T>*)in)[line * a.idist + k];
+                    ((T*)out)[line * a.odist + k] = v.x * s;
+                    break;
+                }
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
 // Work-list wrappers (one MItem per workgroup).
 // ------------------------------------------------------------------------------------------
 constexpr int kMultiE = 16;
@@ -634,6 +767,25 @@ hipError_t scatter_t(const void* w, void* y, const ConvScatterArgs& a, hipStream
     const unsigned gy = (unsigned)(a.nlines < 65535 ? a.nlines : 65535);
     hipLaunchKernelGGL((k_conv_scatter<Ti, Tc>), dim3(gx, gy), dim3(256), 0, st,
                        (const V_t<Tc>*)w, (V_t<Ti>*)y, a);
+    return hipGetLastError();
+}
+
+template <typename T, int E>
+hipError_t real_rows_t(const void* in, void* out, const RealArgs& a, const PassTable& pt,
+                       const void* tw, hipStream_t st) {
+    const size_t lds = 2 * sizeof(T) * (size_t)a.C * a.Ls;
+    const int64_t blocks = (a.nlines + a.C - 1) / a.C;
+    hipLaunchKernelGGL((k_real_rows<T, E>), dim3((unsigned)blocks), dim3(a.C * a.Tt), lds, st, in,
+                       out, a, pt, (const V_t<T>*)tw);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t real_aux_t(const void* in, void* out, const RealAuxArgs& a, hipStream_t st) {
+    const int64_t per = (a.count + 255) / 256;
+    const unsigned gx = (unsigned)(per < 4096 ? (per > 0 ? per : 1) : 4096);
+    const unsigned gy = (unsigned)(a.nlines < 65535 ? a.nlines : 65535);
+    hipLaunchKernelGGL(k_real_aux<T>, dim3(gx, gy), dim3(256), 0, st, in, out, a);
     return hipGetLastError();
 }
 

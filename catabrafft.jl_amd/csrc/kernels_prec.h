@@ -19,6 +19,12 @@ CFFT_DECLARE_LAUNCHERS(f32)
 CFFT_DECLARE_LAUNCHERS(f64)
 #undef CFFT_DECLARE_LAUNCHERS
 
+hipError_t real_rows_f32(const void* in, void* out, const RealArgs& a, const PassTable& pt,
+                         const void* tw, int E, hipStream_t st);
+hipError_t real_rows_f64(const void* in, void* out, const RealArgs& a, const PassTable& pt,
+                         const void* tw, int E, hipStream_t st);
+hipError_t real_aux_f32(const void* in, void* out, const RealAuxArgs& a, hipStream_t st);
+hipError_t real_aux_f64(const void* in, void* out, const RealAuxArgs& a, hipStream_t st);
 void init_kernels_f32();
 void init_kernels_f64();
 

@@ -678,9 +678,13 @@ static int run_fft(const Plan& p, const void* in, void* out, int sgn, double sca
     return fail(CFFT_ERR_UNSUPPORTED, "plan kind not executable");
 }
 
+static int real_exec(const Plan& p, const void* in, void* out, int direction, int normalize,
+                     hipStream_t st);
+
 int plan_exec(const Plan& p, const void* in, void* out, int direction, int normalize,
               hipStream_t st) {
     if (p.kind == Kind::Uneven) return uneven_exec(p, in, out, direction, normalize, st);
+    if (p.real) return real_exec(p, in, out, direction, normalize, st);
     if (p.batch == 0) return CFFT_OK;
     if (!in || !out) return fail(CFFT_ERR_INVALID, "null buffer");
     const int sgn = direction > 0 ? -1 : 1;
@@ -688,11 +692,149 @@ int plan_exec(const Plan& p, const void* in, void* out, int direction, int norma
     return run_fft(p, in, out, sgn, scale, ConvEpilogue{nullptr, nullptr}, st);
 }
 
+// ------------------------------------------------------------------------------- real input
+// rfft / brfft / irfft (AbstractFFTs' real transforms; the reference's plan_rfft / plan_brfft
+// stubs at CatabraFFT.jl:199-208 build nothing usable).  Even n reads the reals as n/2 complex
+// values and untangles the spectrum with w^k = e^{-2 pi i k/n}; odd n goes through a full
+// complex transform.
+int plan_real_create(Ctx* ctx, int64_t n, int64_t batch, int64_t rdist, int64_t cdist, int dtype,
+                     int flags, std::shared_ptr<Plan>* out) {
+    if (n < 2) return fail(CFFT_ERR_INVALID, "real transforms need n >= 2");
+    if (batch < 0) return fail(CFFT_ERR_INVALID, "batch must be >= 0");
+    if (dtype != CFFT_C32 && dtype != CFFT_C64) return fail(CFFT_ERR_INVALID, "bad dtype");
+    if (flags < CFFT_NO_FLAG || flags > CFFT_ENCHANT) return fail(CFFT_ERR_INVALID, "bad flags");
+    if (rdist < n || cdist < n / 2 + 1)
+        return fail(CFFT_ERR_INVALID, "rdist must be >= n and cdist >= n/2 + 1");
+    CFFT_HIP(hipSetDevice(ctx->device));
+    auto p = std::make_shared<Plan>();
+    p->ctx = ctx;
+    p->n = n;
+    p->batch = batch;
+    p->idist = rdist;
+    p->odist = cdist;
+    p->dtype = dtype;
+    p->flags = flags;
+    p->prec = dtype == CFFT_C32 ? Prec::F32 : Prec::F64;
+    const int64_t m = n / 2;
+    const size_t es = esize(p->prec);
+    int rc;
+    auto sub = [&](int64_t len, int64_t idist, int64_t odist, std::unique_ptr<Plan>& s) {
+        s.reset(new Plan());
+        s->ctx = ctx;
+        s->n = len;
+        s->batch = batch;
+        s->idist = idist;
+        s->odist = odist;
+        s->prec = p->prec;
+        s->dtype = dtype;
+        s->flags = flags;
+        PlanOptions o;
+        return plan_build(ctx, *s, o);
+    };
+    if (n % 2 == 0 && rdist % 2 == 0) {
+        std::vector<cld> w(m);
+        const OmegaTable om(n);
+        for (int64_t k = 0; k < m; ++k) w[k] = om(k);
+        if ((rc = upload(p->rw, w, p->prec))) return rc;
+        if (m >= 2 && line_fits(ctx, m, p->prec, 16)) {
+            if ((rc = make_line(p->line, m, p->prec, 16, 1024))) return rc;
+            p->C = rows_lines_per_block(p->line, p->prec, batch, ctx->lds_rows_budget, 256);
+            p->real = 1;
+        } else {
+            if ((rc = sub(m, rdist / 2, m, p->rsub_fwd))) return rc;
+            if ((rc = sub(m, m, rdist / 2, p->rsub_bwd))) return rc;
+            if ((rc = p->rs1.alloc((size_t)std::max<int64_t>(batch, 1) * m * es))) return rc;
+            p->real = 2;
+        }
+    } else {
+        if ((rc = sub(n, n, n, p->rsub_fwd))) return rc;
+        if ((rc = p->rs1.alloc((size_t)std::max<int64_t>(batch, 1) * n * es))) return rc;
+        if ((rc = p->rs2.alloc((size_t)std::max<int64_t>(batch, 1) * n * es))) return rc;
+        p->real = 3;
+    }
+    p->kind = Kind::Direct;
+    std::ostringstream os;
+    os << "real n=" << n << " batch=" << batch << " prec=" << (p->prec == Prec::F32 ? "f32" : "f64")
+       << (p->real == 1 ? " fused half-length " : p->real == 2 ? " half-length via " : " full-length via ");
+    if (p->real == 1) {
+        os << "[";
+        for (int i = 0; i < p->line.pt.npass; ++i) os << (i ? "," : "") << p->line.pt.radix[i];
+        os << "] C=" << p->C;
+    } else {
+        os << "{" << plan_description(*p->rsub_fwd) << "}";
+    }
+    p->desc = os.str();
+    *out = p;
+    return CFFT_OK;
+}
+
+static int real_aux(const Plan& p, int mode, const void* in, void* out, int64_t count,
+                    int64_t idist, int64_t odist, double scale, hipStream_t st) {
+    RealAuxArgs a{};
+    a.mode = mode;
+    a.n = p.n;
+    a.m = p.n / 2;
+    a.nlines = p.batch;
+    a.count = count;
+    a.idist = idist;
+    a.odist = odist;
+    a.scale = scale;
+    a.w = p.rw.p;
+    CFFT_HIP(launch_real_aux(p.prec, in, out, a, st));
+    return CFFT_OK;
+}
+
+static int real_exec(const Plan& p, const void* in, void* out, int direction, int normalize,
+                     hipStream_t st) {
+    if (p.batch == 0) return CFFT_OK;
+    if (!in || !out) return fail(CFFT_ERR_INVALID, "null buffer");
+    if (in == out) return fail(CFFT_ERR_INVALID, "real transforms are out of place");
+    const bool fwd = direction < 0;
+    const double scale = (!fwd && normalize) ? 1.0 / (double)p.n : 1.0;
+    const int64_t n = p.n, m = n / 2, rdist = p.idist, cdist = p.odist;
+    const ConvEpilogue none{nullptr, nullptr};
+    if (p.real == 1) {
+        RealArgs a{};
+        a.m = (int)m;
+        a.C = p.C;
+        a.Tt = p.line.Tt;
+        a.Ls = p.line.Ls;
+        a.nlines = p.batch;
+        a.rdist = rdist;
+        a.cdist = cdist;
+        a.forward = fwd ? 1 : 0;
+        a.scale = scale;
+        a.w = p.rw.p;
+        CFFT_HIP(launch_real_rows(p.prec, in, out, a, p.line.pt, p.line.dtw.p, p.line.E, st));
+        return CFFT_OK;
+    }
+    int rc;
+    if (p.real == 2) {
+        if (fwd) {
+            if ((rc = run_fft(*p.rsub_fwd, in, p.rs1.p, 1, 1.0, none, st))) return rc;
+            return real_aux(p, 0, p.rs1.p, out, m / 2 + 1, m, cdist, 1.0, st);
+        }
+        if ((rc = real_aux(p, 1, in, p.rs1.p, m, cdist, m, scale, st))) return rc;
+        return run_fft(*p.rsub_bwd, p.rs1.p, out, -1, 1.0, none, st);
+    }
+    if (fwd) {
+        if ((rc = real_aux(p, 2, in, p.rs1.p, n, rdist, n, 1.0, st))) return rc;
+        if ((rc = run_fft(*p.rsub_fwd, p.rs1.p, p.rs2.p, 1, 1.0, none, st))) return rc;
+        return real_aux(p, 3, p.rs2.p, out, n / 2 + 1, n, cdist, 1.0, st);
+    }
+    if ((rc = real_aux(p, 4, in, p.rs1.p, n, cdist, n, scale, st))) return rc;
+    if ((rc = run_fft(*p.rsub_fwd, p.rs1.p, p.rs2.p, -1, 1.0, none, st))) return rc;
+    return real_aux(p, 5, p.rs2.p, out, n, n, rdist, 1.0, st);
+}
+
 size_t plan_device_bytes(const Plan& p) {
     size_t b = p.line.dtw.bytes + p.fs.a.dtw.bytes + p.fs.b.dtw.bytes + p.fs.T1.bytes +
                p.fs.T2.bytes + p.scratch.bytes + p.perm.bytes + p.iperm.bytes + p.chirp.bytes +
                p.K.bytes + p.dc.bytes + p.w1.bytes + p.w2.bytes;
     if (p.sub) b += plan_device_bytes(*p.sub);
+    b += p.rw.bytes + p.rs1.bytes + p.rs2.bytes;
+    if (p.rsub_fwd) b += plan_device_bytes(*p.rsub_fwd);
+    if (p.rsub_bwd) b += plan_device_bytes(*p.rsub_bwd);
     for (const auto& s : p.parts) b += plan_device_bytes(*s);
     b += p.uneven_bytes;
     return b;
@@ -817,6 +959,16 @@ int cfft_plan_1d(cfft_ctx ctx, int64_t n, int64_t batch, int64_t idist, int64_t 
     return CFFT_OK;
 }
 
+int cfft_plan_1d_real(cfft_ctx ctx, int64_t n, int64_t batch, int64_t rdist, int64_t cdist,
+                      int dtype, int flags, cfft_plan* plan) {
+    if (!ctx || !plan) return fail(CFFT_ERR_INVALID, "null argument");
+    std::shared_ptr<Plan> p;
+    int rc = plan_real_create(ctx->c, n, batch, rdist, cdist, dtype, flags, &p);
+    if (rc) return rc;
+    *plan = new cfft_plan_s{p, ctx};
+    return CFFT_OK;
+}
+
 int cfft_plan_many_uneven(cfft_ctx ctx, int64_t count, const int64_t* lengths,
                           const int64_t* offsets, int dtype, int flags, cfft_plan* plan) {
     if (!ctx || !plan || (count > 0 && (!lengths || !offsets)))
@@ -848,6 +1000,12 @@ int cfft_exec_host(cfft_plan plan, const void* in, void* out, int direction, int
     size_t ib, ob;
     if (p.kind == Kind::Uneven) {
         ib = ob = (size_t)p.uneven_span * es;
+    } else if (p.real) {
+        if (p.batch == 0) return CFFT_OK;
+        const size_t rb = (size_t)((p.batch - 1) * p.idist + p.n) * (es / 2);
+        const size_t cb = (size_t)((p.batch - 1) * p.odist + p.n / 2 + 1) * es;
+        ib = direction < 0 ? rb : cb;
+        ob = direction < 0 ? cb : rb;
     } else {
         if (p.batch == 0) return CFFT_OK;
         ib = (size_t)((p.batch - 1) * p.idist + p.n) * es;

@@ -66,6 +66,12 @@ struct Plan {
     Prec cprec = Prec::F32;
     DevBuf perm, iperm, chirp, K, dc, w1, w2;
     std::unique_ptr<Plan> sub;
+    // real-input transforms: 1 = fused (n even, n/2 fits a workgroup), 2 = even via a complex
+    // sub-plan of n/2, 3 = odd via a complex sub-plan of n; idist / odist hold rdist (reals) and
+    // cdist (complex elements)
+    int real = 0;
+    DevBuf rw, rs1, rs2;
+    std::unique_ptr<Plan> rsub_fwd, rsub_bwd;
     // uneven batch
     std::vector<std::unique_ptr<Plan>> parts;
     std::shared_ptr<void> uneven;  // opaque state owned by uneven.cpp
@@ -116,6 +122,8 @@ int plan_create(Ctx* ctx, int64_t n, int64_t batch, int64_t idist, int64_t odist
                 int flags, const PlanOptions& opts, std::shared_ptr<Plan>* out);
 int plan_exec(const Plan& p, const void* in, void* out, int direction, int normalize,
               hipStream_t st);
+int plan_real_create(Ctx* ctx, int64_t n, int64_t batch, int64_t rdist, int64_t cdist, int dtype,
+                     int flags, std::shared_ptr<Plan>* out);
 size_t plan_device_bytes(const Plan& p);
 std::string plan_description(const Plan& p);
 

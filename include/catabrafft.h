@@ -80,6 +80,16 @@ CFFT_API int cfft_finalize(cfft_ctx ctx);
 CFFT_API int cfft_plan_1d(cfft_ctx ctx, int64_t n, int64_t batch, int64_t idist, int64_t odist,
                  int dtype, int flags, cfft_plan* plan);
 
+/* Real-input transforms (rfft / brfft / irfft): `batch` rows of n reals (row b at element
+ * b*rdist of a float / double array, rdist >= n) and their half spectra of n/2+1 complex
+ * values (row b at element b*cdist, cdist >= n/2+1).  cfft_exec with CFFT_FORWARD maps reals
+ * to half spectra (rfft); CFFT_BACKWARD maps half spectra to reals (brfft, and irfft with
+ * normalize = 1).  Out of place only.  dtype CFFT_C32 / CFFT_C64 selects float / double.
+ * No counterpart in the reference: its plan_rfft / plan_brfft stubs (CatabraFFT.jl:199-208)
+ * construct nothing usable. */
+CFFT_API int cfft_plan_1d_real(cfft_ctx ctx, int64_t n, int64_t batch, int64_t rdist,
+                               int64_t cdist, int dtype, int flags, cfft_plan* plan);
+
 /* A plan for `count` transforms of uneven lengths: transform i has lengths[i] elements at
  * element offset offsets[i] of both the input and the output buffer. */
 CFFT_API int cfft_plan_many_uneven(cfft_ctx ctx, int64_t count, const int64_t* lengths,

@@ -204,3 +204,29 @@ def test_emulated_wisdom_round_trip(E, tmp_path):
     assert rel_l2(y, O.fft(x.astype(np.complex128), "d")) < 1e-6
     assert lib.cfft_wisdom_import(ctx2, b"/nonexistent/wisdom") == -1
     lib.cfft_finalize(ctx2)
+
+
+@pytest.mark.parametrize("n,batch,rt", [(16, 3, np.float32), (2520, 2, np.float64), (1000, 2, np.float32),
+                                         (17, 3, np.float32), (1 << 16, 1, np.float32),
+                                         (2 * 4099, 1, np.float32), (99, 2, np.float64), (2, 2, np.float32),
+                                         (3, 2, np.float64)])
+def test_emulated_real_transforms(E, n, batch, rt):
+    lib, ctx = E
+    rng = np.random.Generator(np.random.PCG64(n))
+    rdist, cdist = n + 2, n // 2 + 3
+    x = rng.standard_normal((batch, rdist)).astype(rt)
+    X = np.zeros((batch, cdist), dtype=np.complex64 if rt == np.float32 else np.complex128)
+    p = ctypes.c_void_p()
+    assert lib.cfft_plan_1d_real(ctx, n, batch, rdist, cdist, 0 if rt == np.float32 else 1, 0,
+                                 ctypes.byref(p)) == 0, lib.cfft_last_error()
+    assert lib.cfft_exec(p, x.ctypes.data, X.ctypes.data, -1, 0, None) == 0, lib.cfft_last_error()
+    tol = 1e-6 if rt == np.float32 else 4 * np.finfo(float).eps * max(1, np.log2(n))
+    ref = O.fft(x[:, :n].astype(np.complex128), "d")[:, :n // 2 + 1]
+    assert rel_l2(X[:, :n // 2 + 1], ref) < tol
+    assert np.all(X[:, n // 2 + 1:] == 0)
+    for norm in (0, 1):
+        y = np.zeros((batch, rdist), dtype=rt)
+        assert lib.cfft_exec(p, X.ctypes.data, y.ctypes.data, 1, norm, None) == 0
+        expect = x[:, :n] * (1.0 if norm else n)
+        assert rel_l2(y[:, :n], expect) < 2 * tol
+        assert np.all(y[:, n:] == 0)

@@ -35,7 +35,8 @@ NO_FLAG, MEASURE, ENCHANT = 0, 1, 2
 __all__ = ["fft", "ifft", "bfft", "plan_fft", "plan_bfft", "plan_ifft", "mul", "empty_cache",
            "Spell", "Plan", "UnevenPlan", "Server", "Context", "context", "NO_FLAG", "MEASURE",
            "ENCHANT", "CatabraFFTError", "CatabraFFTArgumentError", "export_wisdom",
-           "import_wisdom", "rfft", "irfft", "brfft", "RealPlan"]
+           "import_wisdom", "rfft", "irfft", "brfft", "RealPlan", "NDPlan", "fftn", "ifftn",
+           "bfftn"]
 
 
 # ------------------------------------------------------------------------------ context
@@ -210,6 +211,29 @@ class RealPlan(Plan):
         self.exec_ptr(X.data_ptr(), out.data_ptr(), BACKWARD, normalize,
                       t.cuda.current_stream(X.device).cuda_stream)
         return out
+
+
+class NDPlan(Plan):
+    """cfft_plan_nd: a multi-dimensional transform of one row-major array over `axes`."""
+
+    def __init__(self, shape, axes=None, dtype=np.complex64, flags: int = NO_FLAG,
+                 device: int = 0):
+        self.ctx = context(device)
+        self.shape = tuple(int(d) for d in shape)
+        rank = len(self.shape)
+        axes = range(rank) if axes is None else [a % rank for a in axes]
+        mask = 0
+        for a in axes:
+            mask |= 1 << a
+        self.dtype_code = dtype if isinstance(dtype, int) else _dtype_code(dtype)
+        self.dtype = np.complex64 if self.dtype_code == C32 else np.complex128
+        self.n = int(np.prod([self.shape[a] for a in set(axes)])) if axes else 1
+        self.batch, self.flags = 1, int(flags)
+        dims = (ctypes.c_int64 * rank)(*self.shape)
+        h = ctypes.c_void_p()
+        check(lib().cfft_plan_nd(self.ctx.h, rank, dims, mask, self.dtype_code, self.flags,
+                                 ctypes.byref(h)))
+        self.h = h
 
 
 class UnevenPlan:
@@ -499,6 +523,43 @@ def _c2r(X, n: int, normalize: bool):
     return _real_plan(n, batch, np.float32 if code == C32 else np.float64).backward_host(X, normalize)
 
 
+_ndplans: dict = {}
+
+
+def _nd(x, axes, direction: int, normalize: bool):
+    rank = x.ndim
+    ax = tuple(range(rank)) if axes is None else tuple(sorted(a % rank for a in axes))
+    if _is_torch(x):
+        code = _torch_dtype_code(x)
+        key = (tuple(x.shape), ax, code, x.device.index or 0)
+    else:
+        x = np.asarray(x)
+        code = _dtype_code(x.dtype)
+        key = (x.shape, ax, code, 0)
+    with _plan_lock:
+        p = _ndplans.get(key)
+        if p is None:
+            p = _ndplans[key] = NDPlan(key[0], ax, code, device=key[3])
+    if _is_torch(x):
+        return p.exec_torch(x, None, direction, normalize)
+    return p.exec_host(x, None, direction, normalize)
+
+
+def fftn(x, axes=None):
+    """Multi-dimensional forward DFT over `axes` (all by default) of a row-major array."""
+    return _nd(x, axes, FORWARD, False)
+
+
+def ifftn(x, axes=None):
+    """Multi-dimensional inverse DFT (normalised by the transformed lengths)."""
+    return _nd(x, axes, BACKWARD, True)
+
+
+def bfftn(x, axes=None):
+    """Multi-dimensional unnormalised backward DFT."""
+    return _nd(x, axes, BACKWARD, False)
+
+
 def export_wisdom(path: str, device: int = 0):
     """Writes the MEASURE / ENCHANT choices made so far to `path`."""
     check(lib().cfft_wisdom_export(context(device).h, str(path).encode()))
@@ -514,6 +575,7 @@ def empty_cache():
     with _plan_lock:
         _plans.clear()
         _rplans.clear()
+        _ndplans.clear()
     with _ctx_lock:
         for c in _contexts.values():
             c.empty_cache()

@@ -63,6 +63,7 @@ def _sig(L):
     _set(L, "cfft_init", [c_int, P(vp)])
     _set(L, "cfft_finalize", [vp])
     _set(L, "cfft_plan_1d", [vp, i64, i64, i64, i64, c_int, c_int, P(vp)])
+    _set(L, "cfft_plan_nd", [vp, c_int, P(i64), i64, c_int, c_int, P(vp)])
     _set(L, "cfft_plan_1d_real", [vp, i64, i64, i64, i64, c_int, c_int, P(vp)])
     _set(L, "cfft_plan_many_uneven", [vp, i64, P(i64), P(i64), c_int, c_int, P(vp)])
     _set(L, "cfft_exec", [vp, vp, vp, c_int, c_int, vp])

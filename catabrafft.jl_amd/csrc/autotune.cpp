@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <cmath>
@@ -199,6 +200,8 @@ int autotune(Ctx* ctx, Plan& p, int flags, const PlanOptions& base) {
         }
         std::sort(ms.begin(), ms.end());
         const double med = ms[ms.size() / 2];
+        if (std::getenv("CFFT_TUNE_VERBOSE"))
+            std::fprintf(stderr, "tune %s : %.4f ms\n", plan_description(t).c_str(), med / 3.0);
         if (med < best) {
             best = med;
             bestopt = o;

@@ -45,6 +45,7 @@ struct FourStepAArgs {
 struct FourStepBArgs {
     int N1, N2, C, Tt, Ls, E;
     int64_t batch, idist, odist;
+    int sgn_in;            // -1 conjugates on load (a strided-axis step of an N-D transform)
     int sgn_out;
     double scale;
     ConvEpilogue epi;

@@ -199,16 +199,18 @@ k_fs_a(const V_t<T>* __restrict__ in, V_t<T>* __restrict__ out, FourStepAArgs a,
 template <typename T, int E>
 __device__ __forceinline__ void fs_b_body(unsigned char* smem, const V_t<T>* src, V_t<T>* dst,
                                           int N1, int N2, int C, int Tt, int Ls, int k10,
-                                          int sgn, double scale, const V_t<T>* K, V_t<T>* dc,
-                                          const PassTable& pt, const V_t<T>* tw) {
+                                          int sgn_in, int sgn, double scale, const V_t<T>* K,
+                                          V_t<T>* dc, const PassTable& pt, const V_t<T>* tw) {
     const int cshift = __ffs(C) - 1;
     const int total = C * N2;
+    const T sgi = (T)sgn_in;
     for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
         const int c = idx & (C - 1);
         const int e = idx >> cshift;
         const int k1 = k10 + c;
         V_t<T> v = V_t<T>{0, 0};
         if (k1 < N1) v = src[(int64_t)e * N1 + k1];
+        v.y *= sgi;
         lds_line<T>(smem, C, Ls, c).put(e, v);
     }
     __syncthreads();
@@ -245,7 +247,7 @@ k_fs_b(const V_t<T>* in, V_t<T>* out, FourStepBArgs a, PassTable pt,
     const int64_t bt = blockIdx.x / tiles;
     const int tile = blockIdx.x - (int)(bt * tiles);
     fs_b_body<T, E>(smem, in + bt * a.idist, out + bt * a.odist, a.N1, a.N2, a.C, a.Tt, a.Ls,
-                    tile * a.C, a.sgn_out, a.scale, (const V_t<T>*)a.epi.K,
+                    tile * a.C, a.sgn_in, a.sgn_out, a.scale, (const V_t<T>*)a.epi.K,
                     a.epi.dc ? (V_t<T>*)a.epi.dc + 2 * bt : nullptr, pt, tw);
 }
 
@@ -640,7 +642,7 @@ k_fs_b_multi(MultiArgs ma, const MItem* __restrict__ items, const MPlan* __restr
     const bool epi = it.flags & kEpilogue;
     fs_b_body<T, kMultiE>(smem, mptr<const V_t<T>>(it.in, it.flags & kSelInMask, ma.in_base, ma.out_base),
                           mptr<V_t<T>>(it.out, (it.flags >> kSelOutShift) & 15, ma.in_base, ma.out_base),
-                          P->N1, P->N2, P->CB, P->lb.Tt, P->lb.Ls, it.count * P->CB, sgn, scale,
+                          P->N1, P->N2, P->CB, P->lb.Tt, P->lb.Ls, it.count * P->CB, 1, sgn, scale,
                           epi ? (const V_t<T>*)P->K : nullptr,
                           epi ? reinterpret_cast<V_t<T>*>(it.dc) : nullptr, P->lb.pt,
                           (const V_t<T>*)P->lb.tw);

@@ -566,6 +566,7 @@ static int run_fourstep(const Plan& p, const void* in, void* mid, void* out, int
     b.batch = p.batch;
     b.idist = mdist;
     b.odist = odist;
+    b.sgn_in = 1;
     b.sgn_out = sgn;
     b.scale = scale;
     b.epi = epi;
@@ -613,6 +614,34 @@ static int run_fft(const Plan& p, const void* in, void* out, int sgn, double sca
                                     epi, st);
             }
             return run_fourstep(p, in, out, out, p.idist, p.odist, p.odist, sgn, scale, epi, st);
+        }
+        case Kind::Cols: {  // fs.N1 adjacent lines of length fs.N2 (stride N1), batch blocks
+            FourStepBArgs b{};
+            b.N1 = p.fs.N1;
+            b.N2 = p.fs.N2;
+            b.C = p.fs.CB;
+            b.Tt = p.fs.b.Tt;
+            b.Ls = p.fs.b.Ls;
+            b.E = p.fs.b.E;
+            b.batch = p.batch;
+            b.idist = p.idist;
+            b.odist = p.odist;
+            b.sgn_in = sgn;
+            b.sgn_out = sgn;
+            b.scale = scale;
+            b.epi = epi;
+            CFFT_HIP(launch_fourstep_b(p.prec, in, out, b, p.fs.b.pt, p.fs.b.dtw.p, st));
+            return CFFT_OK;
+        }
+        case Kind::ND: {
+            const size_t nparts = p.parts.size();
+            for (size_t i = 0; i < nparts; ++i) {
+                const Plan& s = *p.parts[i];
+                const bool last = i + 1 == nparts;
+                int rc = run_fft(s, i == 0 ? in : out, out, sgn, last ? scale : 1.0, epi, st);
+                if (rc) return rc;
+            }
+            return CFFT_OK;
         }
         case Kind::ConvSmall: {
             ConvSmallArgs a{};
@@ -690,6 +719,87 @@ int plan_exec(const Plan& p, const void* in, void* out, int direction, int norma
     const int sgn = direction > 0 ? -1 : 1;
     const double scale = normalize ? 1.0 / (double)p.n : 1.0;
     return run_fft(p, in, out, sgn, scale, ConvEpilogue{nullptr, nullptr}, st);
+}
+
+// ------------------------------------------------------------------------------- N-D
+// Multi-dimensional transforms over the axes in axes_mask (bit a = axis a, row-major dims, the
+// last axis contiguous): the role of plan_fft's `region` (CatabraFFT.jl:169, accepted there and
+// ignored).  The contiguous axis runs as a batch of 1-D transforms; every other axis runs as
+// four-step pass B over its adjacent strided lines, in place on the output.
+int plan_nd_create(Ctx* ctx, int rank, const int64_t* dims, int64_t axes_mask, int dtype,
+                   int flags, std::shared_ptr<Plan>* out) {
+    if (rank < 1 || rank > 8 || !dims) return fail(CFFT_ERR_INVALID, "rank must be 1..8");
+    if (dtype != CFFT_C32 && dtype != CFFT_C64) return fail(CFFT_ERR_INVALID, "bad dtype");
+    if (flags < CFFT_NO_FLAG || flags > CFFT_ENCHANT) return fail(CFFT_ERR_INVALID, "bad flags");
+    int64_t total = 1, ntrans = 1;
+    for (int a = 0; a < rank; ++a) {
+        if (dims[a] < 1) return fail(CFFT_ERR_INVALID, "dims must be >= 1");
+        total *= dims[a];
+        if ((axes_mask >> a) & 1) ntrans *= dims[a];
+    }
+    CFFT_HIP(hipSetDevice(ctx->device));
+    auto p = std::make_shared<Plan>();
+    p->ctx = ctx;
+    p->kind = Kind::ND;
+    p->n = ntrans;  // normalisation divides by the product of the transformed lengths
+    p->batch = 1;
+    p->idist = p->odist = total;
+    p->dtype = dtype;
+    p->flags = flags;
+    p->prec = dtype == CFFT_C32 ? Prec::F32 : Prec::F64;
+    std::ostringstream os;
+    os << "nd dims=[";
+    for (int a = 0; a < rank; ++a) os << (a ? "," : "") << dims[a];
+    os << "] axes=0x" << std::hex << axes_mask << std::dec;
+    int rc;
+    for (int a = rank - 1; a >= 0; --a) {
+        if (!((axes_mask >> a) & 1) || dims[a] == 1) continue;
+        int64_t inner = 1, outer = 1;
+        for (int b = a + 1; b < rank; ++b) inner *= dims[b];
+        for (int b = 0; b < a; ++b) outer *= dims[b];
+        std::unique_ptr<Plan> s(new Plan());
+        s->ctx = ctx;
+        s->prec = p->prec;
+        s->dtype = dtype;
+        s->flags = flags;
+        if (inner == 1) {  // contiguous axis: any 1-D plan
+            s->n = dims[a];
+            s->batch = outer;
+            s->idist = s->odist = dims[a];
+            PlanOptions o;
+            if ((rc = plan_build(ctx, *s, o))) return rc;
+            os << " | axis " << a << ": " << describe(*s);
+        } else {
+            const int64_t N2 = dims[a];
+            if (!line_fits(ctx, N2, p->prec, 16) || inner > (int64_t)1 << 30)
+                return fail(CFFT_ERR_UNSUPPORTED, "strided axis of length " + std::to_string(N2) +
+                                                      " must be smooth and fit one workgroup");
+            s->kind = Kind::Cols;
+            s->n = N2;
+            s->batch = outer;
+            s->idist = s->odist = inner * N2;
+            s->fs.N1 = (int)inner;
+            s->fs.N2 = (int)N2;
+            if ((rc = make_line(s->fs.b, N2, p->prec, 16, 1024))) return rc;
+            s->fs.CB = pick_tile(ctx, s->fs.b, p->prec, p->prec == Prec::F32 ? 16 : 8);
+            os << " | axis " << a << ": strided lines N=" << N2 << " x " << inner << " C="
+               << s->fs.CB;
+        }
+        p->parts.push_back(std::move(s));
+    }
+    if (p->parts.empty()) {  // nothing to transform: a copy
+        std::unique_ptr<Plan> s(new Plan());
+        s->ctx = ctx;
+        s->prec = p->prec;
+        s->kind = Kind::Trivial;
+        s->n = 1;
+        s->batch = total;
+        s->idist = s->odist = 1;
+        p->parts.push_back(std::move(s));
+    }
+    p->desc = os.str();
+    *out = p;
+    return CFFT_OK;
 }
 
 // ------------------------------------------------------------------------------- real input
@@ -959,6 +1069,16 @@ int cfft_plan_1d(cfft_ctx ctx, int64_t n, int64_t batch, int64_t idist, int64_t 
     return CFFT_OK;
 }
 
+int cfft_plan_nd(cfft_ctx ctx, int rank, const int64_t* dims, int64_t axes_mask, int dtype,
+                 int flags, cfft_plan* plan) {
+    if (!ctx || !plan) return fail(CFFT_ERR_INVALID, "null argument");
+    std::shared_ptr<Plan> p;
+    int rc = plan_nd_create(ctx->c, rank, dims, axes_mask, dtype, flags, &p);
+    if (rc) return rc;
+    *plan = new cfft_plan_s{p, ctx};
+    return CFFT_OK;
+}
+
 int cfft_plan_1d_real(cfft_ctx ctx, int64_t n, int64_t batch, int64_t rdist, int64_t cdist,
                       int dtype, int flags, cfft_plan* plan) {
     if (!ctx || !plan) return fail(CFFT_ERR_INVALID, "null argument");
@@ -1000,6 +1120,8 @@ int cfft_exec_host(cfft_plan plan, const void* in, void* out, int direction, int
     size_t ib, ob;
     if (p.kind == Kind::Uneven) {
         ib = ob = (size_t)p.uneven_span * es;
+    } else if (p.kind == Kind::ND) {
+        ib = ob = (size_t)p.idist * es;
     } else if (p.real) {
         if (p.batch == 0) return CFFT_OK;
         const size_t rb = (size_t)((p.batch - 1) * p.idist + p.n) * (es / 2);

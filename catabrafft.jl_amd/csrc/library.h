@@ -42,7 +42,7 @@ struct FourStepPlan {
     int shift = 0, mask = 0;
 };
 
-enum class Kind : int { Trivial, Direct, FourStep, ConvSmall, ConvLarge, Uneven };
+enum class Kind : int { Trivial, Direct, FourStep, ConvSmall, ConvLarge, Uneven, Cols, ND };
 
 struct Ctx;
 
@@ -122,6 +122,8 @@ int plan_create(Ctx* ctx, int64_t n, int64_t batch, int64_t idist, int64_t odist
                 int flags, const PlanOptions& opts, std::shared_ptr<Plan>* out);
 int plan_exec(const Plan& p, const void* in, void* out, int direction, int normalize,
               hipStream_t st);
+int plan_nd_create(Ctx* ctx, int rank, const int64_t* dims, int64_t axes_mask, int dtype,
+                   int flags, std::shared_ptr<Plan>* out);
 int plan_real_create(Ctx* ctx, int64_t n, int64_t batch, int64_t rdist, int64_t cdist, int dtype,
                      int flags, std::shared_ptr<Plan>* out);
 size_t plan_device_bytes(const Plan& p);

@@ -127,6 +127,46 @@ end
 Base.:*(p::Spell, x::AbstractVector{Complex{T}}) where {T} = mul!(similar(x), p, x)
 (p::Spell{T})(x::AbstractVector{Complex{T}}) where {T} = fft(x)
 
+# ---- real-input transforms (rfft / brfft / irfft): cfft_plan_1d_real; the reference's
+# plan_rfft / plan_brfft stubs (CatabraFFT.jl:199-208) construct nothing usable.
+const RPLANS = Dict{Tuple{Int,DataType},Ptr{Cvoid}}()
+
+function real_plan(n::Int, ::Type{T}) where {T<:AbstractFloat}
+    lock(LOCK) do
+        get!(RPLANS, (n, T)) do
+            h = Ref{Ptr{Cvoid}}(C_NULL)
+            check(ccall((:cfft_plan_1d_real, libcfft), Cint,
+                        (Ptr{Cvoid}, Int64, Int64, Int64, Int64, Cint, Cint, Ptr{Ptr{Cvoid}}),
+                        context(), n, 1, n, n ÷ 2 + 1, cfft_dtype(T), Cint(NO_FLAG), h))
+            h[]
+        end
+    end
+end
+
+"""rfft(x): the n÷2+1 non-negative-frequency terms of the DFT of real x."""
+function rfft(x::AbstractVector{T}) where {T<:Union{Float32,Float64}}
+    xd = x isa Vector{T} ? x : Vector{T}(x)
+    y = Vector{Complex{T}}(undef, length(x) ÷ 2 + 1)
+    GC.@preserve xd y check(ccall((:cfft_exec_host, libcfft), Cint,
+                                  (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint),
+                                  real_plan(length(x), T), pointer(xd), pointer(y), FORWARD, 0))
+    return y
+end
+
+function _c2r(X::AbstractVector{Complex{T}}, d::Int, normalize::Bool) where {T<:Union{Float32,Float64}}
+    length(X) == d ÷ 2 + 1 || throw(DimensionMismatch("$(length(X)) values do not match d=$d"))
+    Xd = X isa Vector{Complex{T}} ? X : Vector{Complex{T}}(X)
+    y = Vector{T}(undef, d)
+    GC.@preserve Xd y check(ccall((:cfft_exec_host, libcfft), Cint,
+                                  (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint),
+                                  real_plan(d, T), pointer(Xd), pointer(y), BACKWARD, Cint(normalize)))
+    return y
+end
+"""brfft(X, d): unnormalised inverse of rfft to d reals."""
+brfft(X::AbstractVector{Complex{T}}, d::Integer) where {T} = _c2r(X, Int(d), false)
+"""irfft(X, d): inverse of rfft to d reals."""
+irfft(X::AbstractVector{Complex{T}}, d::Integer) where {T} = _c2r(X, Int(d), true)
+
 """empty_cache(): drops plans and staging buffers (CatabraFFT.jl:39-43)."""
 function empty_cache()
     lock(LOCK) do
@@ -134,6 +174,10 @@ function empty_cache()
             ccall((:cfft_plan_destroy, libcfft), Cint, (Ptr{Cvoid},), h)
         end
         empty!(PLANS)
+        for h in values(RPLANS)
+            ccall((:cfft_plan_destroy, libcfft), Cint, (Ptr{Cvoid},), h)
+        end
+        empty!(RPLANS)
         CTX[] == C_NULL || check(ccall((:cfft_empty_cache, libcfft), Cint, (Ptr{Cvoid},), CTX[]))
     end
     nothing

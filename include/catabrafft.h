@@ -80,6 +80,14 @@ CFFT_API int cfft_finalize(cfft_ctx ctx);
 CFFT_API int cfft_plan_1d(cfft_ctx ctx, int64_t n, int64_t batch, int64_t idist, int64_t odist,
                  int dtype, int flags, cfft_plan* plan);
 
+/* Multi-dimensional transform of one row-major array dims[0] x ... x dims[rank-1] (the last
+ * axis contiguous) over the axes whose bit is set in axes_mask (bit a = axis a); normalize
+ * divides by the product of the transformed lengths.  Replaces plan_fft's `region` argument
+ * (CatabraFFT.jl:169, accepted and ignored there).  Every strided axis must be smooth and fit
+ * one workgroup (<= 16384 points in C32, <= 8192 in C64). */
+CFFT_API int cfft_plan_nd(cfft_ctx ctx, int rank, const int64_t* dims, int64_t axes_mask,
+                          int dtype, int flags, cfft_plan* plan);
+
 /* Real-input transforms (rfft / brfft / irfft): `batch` rows of n reals (row b at element
  * b*rdist of a float / double array, rdist >= n) and their half spectra of n/2+1 complex
  * values (row b at element b*cdist, cdist >= n/2+1).  cfft_exec with CFFT_FORWARD maps reals

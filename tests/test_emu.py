@@ -230,3 +230,31 @@ def test_emulated_real_transforms(E, n, batch, rt):
         expect = x[:, :n] * (1.0 if norm else n)
         assert rel_l2(y[:, :n], expect) < 2 * tol
         assert np.all(y[:, n:] == 0)
+
+
+@pytest.mark.parametrize("shape,axes", [((8, 12), (0, 1)), ((16, 60), (0,)), ((16, 60), (1,)),
+                                        ((6, 10, 14), (0, 1, 2)), ((6, 10, 14), (0, 2)),
+                                        ((5, 1, 97), (0, 1, 2)), ((4, 2520), (0, 1)),
+                                        ((3, 4, 5), ())])
+def test_emulated_nd(E, shape, axes):
+    lib, ctx = E
+    x = seeded_complex(int(np.prod(shape)), seed=len(shape), dtype=np.complex64).reshape(shape)
+    mask = 0
+    for a in axes:
+        mask |= 1 << a
+    dims = (ctypes.c_int64 * len(shape))(*shape)
+    p = ctypes.c_void_p()
+    assert lib.cfft_plan_nd(ctx, len(shape), dims, mask, 0, 0, ctypes.byref(p)) == 0, lib.cfft_last_error()
+    for direction, norm in ((-1, 0), (1, 1)):
+        y = np.zeros_like(x)
+        assert lib.cfft_exec(p, x.ctypes.data, y.ctypes.data, direction, norm, None) == 0
+        xr = x.astype(np.complex128)
+        ref = np.fft.fftn(xr, axes=axes) if direction < 0 else np.fft.ifftn(xr, axes=axes)
+        if not axes:
+            ref = xr
+        assert rel_l2(y, ref) < 1e-6, (direction, norm)
+    # in place
+    z = x.copy()
+    assert lib.cfft_exec(p, z.ctypes.data, z.ctypes.data, -1, 0, None) == 0
+    ref = np.fft.fftn(x.astype(np.complex128), axes=axes) if axes else x
+    assert rel_l2(z, ref) < 1e-6

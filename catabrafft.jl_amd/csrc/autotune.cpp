@@ -55,17 +55,27 @@ static std::vector<PlanOptions> candidates(Ctx* ctx, const Plan& p, int flags) {
         }
         const std::vector<int> tiles =
             enchant ? std::vector<int>{4, 8, 16, 32} : std::vector<int>{8, 16};
+        // chunks: whole batch, and transforms per round whose middle is ~32 / ~96 MB
+        std::vector<int64_t> chunks{0};
+        const int64_t tb = p.n * (p.prec == Prec::F32 ? 8 : 16);
+        for (int64_t mb : {32, 96}) {
+            const int64_t c = std::max<int64_t>(1, (mb << 20) / tb);
+            if (c < p.batch && std::find(chunks.begin(), chunks.end(), c) == chunks.end())
+                chunks.push_back(c);
+        }
         for (int64_t a : splits)
             for (int E : Es)
                 for (int ca : tiles)
-                    for (int cb : tiles) {
-                        PlanOptions o = base;
-                        o.E = E;
-                        o.n1 = a;
-                        o.ca = ca;
-                        o.cb = cb;
-                        out.push_back(o);
-                    }
+                    for (int cb : tiles)
+                        for (int64_t ch : chunks) {
+                            PlanOptions o = base;
+                            o.E = E;
+                            o.n1 = a;
+                            o.ca = ca;
+                            o.cb = cb;
+                            o.chunk = ch;
+                            out.push_back(o);
+                        }
     } else {
         for (int E : Es)
             for (int tt : tts) {
@@ -92,7 +102,7 @@ static void copy_shape(const Plan& from, Plan& to) {
 
 // ------------------------------------------------------------------------------- wisdom
 // Tuned choices keyed by (n, batch, idist, odist, dtype, flags); kept per context and written /
-// read as text lines "n batch idist odist dtype flags : E target_threads n1 ca cb".
+// read as text lines "n batch idist odist dtype flags : E target_threads n1 ca cb chunk".
 static std::string wisdom_key(const Plan& p, int flags) {
     std::ostringstream k;
     k << p.n << " " << p.batch << " " << p.idist << " " << p.odist << " " << p.dtype << " " << flags;
@@ -106,8 +116,8 @@ int wisdom_export(Ctx* ctx, const char* path) {
     std::fprintf(f, "# catabrafft wisdom v1 (gfx950)\n");
     for (const auto& kv : ctx->wisdom) {
         const PlanOptions& o = kv.second;
-        std::fprintf(f, "%s : %d %d %lld %d %d\n", kv.first.c_str(), o.E, o.target_threads,
-                     (long long)o.n1, o.ca, o.cb);
+        std::fprintf(f, "%s : %d %d %lld %d %d %lld\n", kv.first.c_str(), o.E, o.target_threads,
+                     (long long)o.n1, o.ca, o.cb, (long long)o.chunk);
     }
     std::fclose(f);
     return CFFT_OK;
@@ -127,13 +137,14 @@ int wisdom_import(Ctx* ctx, const char* path) {
         std::string key(line);
         while (!key.empty() && key.back() == ' ') key.pop_back();
         PlanOptions o;
-        long long n1 = 0;
-        if (std::sscanf(colon + 1, "%d %d %lld %d %d", &o.E, &o.target_threads, &n1, &o.ca,
-                        &o.cb) != 5) {
+        long long n1 = 0, chunk = -1;
+        if (std::sscanf(colon + 1, "%d %d %lld %d %d %lld", &o.E, &o.target_threads, &n1, &o.ca,
+                        &o.cb, &chunk) != 6) {
             std::fclose(f);
             return fail(CFFT_ERR_INVALID, "malformed wisdom line");
         }
         o.n1 = n1;
+        o.chunk = chunk;
         o.explicit_opts = true;
         ctx->wisdom[key] = o;
         ++n;

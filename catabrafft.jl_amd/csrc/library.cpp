@@ -16,6 +16,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -278,7 +279,8 @@ static int pick_tile(Ctx* ctx, const LinePlan& lp, Prec pr, int want) {
 }
 
 static int plan_fourstep_factors(Ctx* ctx, FourStepPlan& fs, int64_t n, Prec pr, int E,
-                                 int64_t n1_force, int ca, int cb) {
+                                 int64_t n1_force, int ca, int cb, int64_t chunk) {
+    fs.chunk = chunk;
     int64_t best1 = 0;
     if (n1_force > 0) {
         best1 = n1_force;
@@ -480,6 +482,15 @@ static int plan_conv(Ctx* ctx, Plan& p, const PlanOptions& o) {
     return CFFT_OK;
 }
 
+// Default four-step chunk: transforms per chunk whose middle stays within ~96 MB of the 256 MB
+// Infinity Cache (the whole batch when it is small already).
+static int64_t default_chunk(const Plan& p) {
+    if (const char* e = std::getenv("CFFT_FOURSTEP_CHUNK")) return std::max<int64_t>(0, std::atoll(e));
+    const int64_t bytes = p.n * esize(p.prec);
+    const int64_t c = std::max<int64_t>(1, ((int64_t)96 << 20) / std::max<int64_t>(bytes, 1));
+    return c >= p.batch ? 0 : c;
+}
+
 static int plan_fft_core(Ctx* ctx, Plan& p, const PlanOptions& o) {
     if (p.n == 1) {
         p.kind = Kind::Trivial;
@@ -488,7 +499,8 @@ static int plan_fft_core(Ctx* ctx, Plan& p, const PlanOptions& o) {
     const int E = o.E ? o.E : 16;
     if (o.force_conv == 0 && is_smooth(p.n)) {
         if (o.n1 == 0 && line_fits(ctx, p.n, p.prec, E)) return plan_direct(ctx, p, E, o.target_threads);
-        int rc = plan_fourstep_factors(ctx, p.fs, p.n, p.prec, E, o.n1, o.ca, o.cb);
+        int rc = plan_fourstep_factors(ctx, p.fs, p.n, p.prec, E, o.n1, o.ca, o.cb,
+                                       o.chunk >= 0 ? o.chunk : default_chunk(p));
         if (rc) return rc;
         p.kind = Kind::FourStep;
         return CFFT_OK;
@@ -515,6 +527,7 @@ static std::string describe(const Plan& p) {
         case Kind::FourStep:
             os << "fourstep N1=" << p.fs.N1 << " " << rad(p.fs.a) << " CA=" << p.fs.CA
                << " N2=" << p.fs.N2 << " " << rad(p.fs.b) << " CB=" << p.fs.CB;
+            if (p.fs.chunk) os << " chunk=" << p.fs.chunk;
             break;
         case Kind::ConvSmall:
             os << (p.mode == ConvMode::Rader ? "rader" : "bluestein") << "-fused L=" << p.L << " "
@@ -536,41 +549,54 @@ static std::string describe(const Plan& p) {
 static int run_fft(const Plan& p, const void* in, void* out, int sgn, double scale,
                    ConvEpilogue epi, hipStream_t st);
 
+// Runs the two passes over the batch in chunks of fs.chunk transforms (0 = whole batch).  Pass B
+// overwrites the middle in place with the output, so when a chunk's middle fits the Infinity
+// Cache its dirty lines are replaced before they are written back and HBM sees about one read
+// and one write per element.
 static int run_fourstep(const Plan& p, const void* in, void* mid, void* out, int64_t idist,
                         int64_t mdist, int64_t odist, int sgn, double scale, ConvEpilogue epi,
                         hipStream_t st) {
     const FourStepPlan& fs = p.fs;
-    FourStepAArgs a{};
-    a.N1 = fs.N1;
-    a.N2 = fs.N2;
-    a.C = fs.CA;
-    a.Tt = fs.a.Tt;
-    a.Ls = fs.a.Ls;
-    a.E = fs.a.E;
-    a.batch = p.batch;
-    a.idist = idist;
-    a.odist = mdist;
-    a.sgn_in = sgn;
-    a.shift = fs.shift;
-    a.mask = fs.mask;
-    a.T1 = fs.T1.p;
-    a.T2 = fs.T2.p;
-    CFFT_HIP(launch_fourstep_a(p.prec, in, mid, a, fs.a.pt, fs.a.dtw.p, st));
-    FourStepBArgs b{};
-    b.N1 = fs.N1;
-    b.N2 = fs.N2;
-    b.C = fs.CB;
-    b.Tt = fs.b.Tt;
-    b.Ls = fs.b.Ls;
-    b.E = fs.b.E;
-    b.batch = p.batch;
-    b.idist = mdist;
-    b.odist = odist;
-    b.sgn_in = 1;
-    b.sgn_out = sgn;
-    b.scale = scale;
-    b.epi = epi;
-    CFFT_HIP(launch_fourstep_b(p.prec, mid, out, b, fs.b.pt, fs.b.dtw.p, st));
+    const int64_t chunk = fs.chunk > 0 ? std::min<int64_t>(fs.chunk, p.batch) : p.batch;
+    const size_t es = esize(p.prec);
+    for (int64_t b0 = 0; b0 < p.batch; b0 += chunk) {
+        const int64_t nb = std::min<int64_t>(chunk, p.batch - b0);
+        const char* inb = (const char*)in + (size_t)b0 * idist * es;
+        char* midb = (char*)mid + (size_t)b0 * mdist * es;
+        char* outb = (char*)out + (size_t)b0 * odist * es;
+        FourStepAArgs a{};
+        a.N1 = fs.N1;
+        a.N2 = fs.N2;
+        a.C = fs.CA;
+        a.Tt = fs.a.Tt;
+        a.Ls = fs.a.Ls;
+        a.E = fs.a.E;
+        a.batch = nb;
+        a.idist = idist;
+        a.odist = mdist;
+        a.sgn_in = sgn;
+        a.shift = fs.shift;
+        a.mask = fs.mask;
+        a.T1 = fs.T1.p;
+        a.T2 = fs.T2.p;
+        CFFT_HIP(launch_fourstep_a(p.prec, inb, midb, a, fs.a.pt, fs.a.dtw.p, st));
+        FourStepBArgs b{};
+        b.N1 = fs.N1;
+        b.N2 = fs.N2;
+        b.C = fs.CB;
+        b.Tt = fs.b.Tt;
+        b.Ls = fs.b.Ls;
+        b.E = fs.b.E;
+        b.batch = nb;
+        b.idist = mdist;
+        b.odist = odist;
+        b.sgn_in = 1;
+        b.sgn_out = sgn;
+        b.scale = scale;
+        b.epi = epi;
+        if (epi.dc) b.epi.dc = (char*)epi.dc + (size_t)b0 * 2 * es;
+        CFFT_HIP(launch_fourstep_b(p.prec, midb, outb, b, fs.b.pt, fs.b.dtw.p, st));
+    }
     return CFFT_OK;
 }
 

@@ -40,6 +40,7 @@ struct FourStepPlan {
     LinePlan a, b;
     DevBuf T1, T2;
     int shift = 0, mask = 0;
+    int64_t chunk = 0;  // transforms per pass-A/pass-B round (0 = the whole batch)
 };
 
 enum class Kind : int { Trivial, Direct, FourStep, ConvSmall, ConvLarge, Uneven, Cols, ND };
@@ -86,6 +87,7 @@ struct PlanOptions {
     int target_threads = 256;  // threads per workgroup the rows kernels aim for
     int64_t n1 = 0;            // four-step split (0 = balanced / direct when it fits)
     int ca = 0, cb = 0;        // four-step column tiles (0 = default)
+    int64_t chunk = -1;        // four-step transforms per round (-1 = default, 0 = whole batch)
     int force_conv = 0;
     bool explicit_opts = false;
     bool alloc_work = true;    // large convolutions: allocate work buffers for `batch` lines

@@ -258,3 +258,25 @@ def test_emulated_nd(E, shape, axes):
     assert lib.cfft_exec(p, z.ctypes.data, z.ctypes.data, -1, 0, None) == 0
     ref = np.fft.fftn(x.astype(np.complex128), axes=axes) if axes else x
     assert rel_l2(z, ref) < 1e-6
+
+
+def test_emulated_fourstep_chunks(E, monkeypatch):
+    """Four-step in rounds of 2 transforms (the Infinity-Cache chunking) equals one round."""
+    lib, ctx = E
+    n, batch = 1 << 15, 5
+    x = seeded_complex(n, batch=batch, seed=12, dtype=np.complex64)
+    monkeypatch.setenv("CFFT_FOURSTEP_CHUNK", "2")
+    p = _plan(lib, ctx, n, batch, np.complex64, idist=n + 1, odist=n)
+    assert "chunk=2" in _desc(lib, p)
+    xs = np.zeros((batch, n + 1), dtype=np.complex64)
+    xs[:, :n] = x
+    y = np.zeros_like(x)
+    assert lib.cfft_exec(p, xs.ctypes.data, y.ctypes.data, -1, 0, None) == 0
+    assert rel_l2(y, O.fft(x.astype(np.complex128), "d")) < 1e-6
+    monkeypatch.setenv("CFFT_FOURSTEP_CHUNK", "0")
+    assert lib.cfft_empty_cache(ctx) == 0  # the cached plan was made with chunks of 2
+    q = _plan(lib, ctx, n, batch, np.complex64, idist=n + 1, odist=n)
+    assert "chunk" not in _desc(lib, q)
+    z = np.zeros_like(x)
+    assert lib.cfft_exec(q, xs.ctypes.data, z.ctypes.data, -1, 0, None) == 0
+    assert np.array_equal(y, z)

@@ -73,6 +73,13 @@ struct LdsLine {
     }
 };
 
+// Registers per lane that a pass holding `data` registers of butterfly inputs may spend on
+// twiddles loaded ahead of its butterflies: up to 32, less as the pass's own data grows, so
+// that with 1024-thread workgroups (128 registers per lane) nothing spills.
+__host__ __device__ constexpr int tw_prefetch_vgprs(int data) {
+    return 96 - 2 * data < 0 ? 0 : (96 - 2 * data > 32 ? 32 : 96 - 2 * data);
+}
+
 // Per-thread butterfly capacity: MB butterflies of radix R with E complex registers.
 __host__ __device__ constexpr int bfly_per_thread(int E, int R) { return E / R > 0 ? E / R : 1; }
 
@@ -91,6 +98,26 @@ __device__ __forceinline__ void stockham_pass(const int N, const int s, const in
     CFFT_OPAQUE(t_);
     const int t = t_;
     const int nb = N / R;
+    // Twiddles first, from a clamped (always valid) butterfly index: their memory latency then
+    // overlaps the source reads and the butterflies instead of being paid once per small group
+    // of loads.  tw_prefetch_vgprs bounds the registers per lane that hold them (KP twiddles per
+    // butterfly); the rest are loaded after the butterflies.  The table has rows for p = 0 (all
+    // ones), so every butterfly multiplies.  The last pass (s = N / R) has no twiddles.
+    constexpr int VW = (int)(sizeof(V) / 4);
+    constexpr int KP0 = tw_prefetch_vgprs(MB * R * VW) / (MB * VW);
+    constexpr int KP = (R - 1) < KP0 ? (R - 1) : KP0;
+    const bool twiddled = s * R < N;
+    V w[MB][KP > 0 ? KP : 1];
+    if (twiddled) {
+#pragma unroll
+        for (int ib = 0; ib < MB; ++ib) {
+            const int bt = t + ib * Tt;
+            const int b = bt < nb ? bt : nb - 1;
+            const V* wp = tw + (b / s) * (R - 1);
+#pragma unroll
+            for (int k = 0; k < KP; ++k) w[ib][k] = wp[k];
+        }
+    }
     V v[MB][R];
 #pragma unroll
     for (int ib = 0; ib < MB; ++ib) {
@@ -108,10 +135,12 @@ __device__ __forceinline__ void stockham_pass(const int N, const int s, const in
             Dft<R>::run(v[ib]);
             const int p = b / s;
             const int q = b - p * s;
-            if (p > 0) {
-                const V* w = tw + p * (R - 1);
+            if (twiddled) {
 #pragma unroll
-                for (int k = 1; k < R; ++k) v[ib][k] = cmul(v[ib][k], w[k - 1]);
+                for (int k = 1; k <= KP; ++k) v[ib][k] = cmul(v[ib][k], w[ib][k - 1]);
+                const V* wp = tw + p * (R - 1);
+#pragma unroll
+                for (int k = KP + 1; k < R; ++k) v[ib][k] = cmul(v[ib][k], wp[k - 1]);
             }
             const int base = q + s * R * p;
 #pragma unroll

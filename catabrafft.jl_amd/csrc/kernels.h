@@ -27,21 +27,18 @@ struct RowsArgs {
 };
 
 // Four-step pass A for n = N1*N2 (lines = the N2 columns of the N1 x N2 row-major input):
-//   out[b*odist + j2*N1 + k1] = w_n^{j2 k1} * DFT_N1(in[b*idist + j1*N2 + j2])[k1]
-// w_n^e = T1[e & mask] * T2[e >> shift] (two small tables, e < n).
+//   out[b*odist + j2*N1 + k1] = DFT_N1(in[b*idist + j1*N2 + j2])[k1]
+// (the twiddle w_n^{j2 k1} between the two passes is applied by pass B as it loads).
 struct FourStepAArgs {
     int N1, N2, C, Tt, Ls, E;
     int64_t batch, idist, odist;
     int sgn_in;
-    int shift;
-    int mask;
-    const void* T1;
-    const void* T2;
 };
 
 // Four-step pass B (lines = the N1 values of k1, elements j2 at stride N1):
-//   out[b*odist + k1 + N1*k2] = DFT_N2(in[b*idist + j2*N1 + k1])[k2]
-// Reads and writes the same index set per tile, so in == out is allowed.
+//   out[b*odist + k1 + N1*k2] = DFT_N2(w_n^{j2 k1} in[b*idist + j2*N1 + k1])[k2]
+// with w_n^e = T1[e & mask] * T2[e >> shift] (T1 == nullptr: no twiddle, a plain strided-line
+// pass).  Reads and writes the same index set per tile, so in == out is allowed.
 struct FourStepBArgs {
     int N1, N2, C, Tt, Ls, E;
     int64_t batch, idist, odist;
@@ -49,6 +46,10 @@ struct FourStepBArgs {
     int sgn_out;
     double scale;
     ConvEpilogue epi;
+    int shift;
+    int mask;
+    const void* T1;
+    const void* T2;
 };
 
 // Fused small convolution FFT (Rader or Bluestein) entirely in LDS: one launch, one HBM read

@@ -4,7 +4,7 @@
 //   rows        batched contiguous line FFTs, one HBM read + one HBM write per element
 //               (radix_exec.jl's whole pass sequence in one launch)
 //   fs_a/fs_b   the two passes of the four-step n = N1*N2 (mixed_radix.jl:44-78), twiddle
-//               w_n^{j2 k1} fused into pass A's store, transpose fused into pass B's store
+//               w_n^{j2 k1} fused into pass B's load, transpose fused into pass B's store
 //   conv        Rader (prime.jl:5-82) or Bluestein fused in LDS
 //   gather / scatter
 //               the permutation / chirp ends of a large convolution whose FFTs run on the
@@ -49,6 +49,39 @@ struct GRowDst {
         p[e] = V_t<T>{v.x * sx, v.y * sy};
     }
 };
+
+// Plain global-memory line output.
+template <typename T>
+struct GPlainDst {
+    V_t<T>* p;
+    __device__ __forceinline__ void put(int e, V_t<T> v) const { p[e] = v; }
+};
+
+// Staged copy with batched memory latency: for idx = tid, tid + step, ... < total, every thread
+// first runs U loads LD(idx) (indices past the end are clamped to total - 1, so every load is
+// valid and none is predicated), then the U stores ST(idx, value) of the valid indices.  One wait
+// then covers U loads; a plain loop would wait once per element, because each element's store
+// may alias the next element's load.
+constexpr int kStageU = 8;
+
+template <int U, class LD, class ST>
+__device__ __forceinline__ void batched_for(int tid, int step, int total, const LD& ld,
+                                            const ST& st) {
+    using R = decltype(ld(0));
+    for (int base = tid; base < total; base += U * step) {
+        R v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = base + u * step;
+            v[u] = ld(i < total ? i : total - 1);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = base + u * step;
+            if (i < total) st(i, v[u]);
+        }
+    }
+}
 
 template <typename T>
 __device__ __forceinline__ LdsLine<T> lds_line(unsigned char* smem, int C, int Ls, int c) {
@@ -134,40 +167,32 @@ k_rows(const V_t<T>* __restrict__ in, V_t<T>* __restrict__ out, RowsArgs a, Pass
 
 // ------------------------------------------------------------------------------------------
 // Four-step pass A body: columns j20 .. j20+C-1 of one transform (N1 x N2 row-major).
-//   out[j2*N1 + k1] = w_n^{j2 k1} * DFT_N1(in[j1*N2 + j2])[k1]
+//   out[j2*N1 + k1] = DFT_N1(in[j1*N2 + j2])[k1]
+// The twiddle w_n^{j2 k1} is applied by pass B while it loads the middle.
 // ------------------------------------------------------------------------------------------
-template <typename T>
-struct TwiddleDst {
-    V_t<T>* p;              // this column's output row: p[k1]
-    const V_t<T>* T1;
-    const V_t<T>* T2;
-    int j2, shift, mask;
-    __device__ __forceinline__ void put(int k1, V_t<T> v) const {
-        const int e = j2 * k1;
-        const V_t<T> w = cmul(T1[e & mask], T2[e >> shift]);
-        p[k1] = cmul(v, w);
-    }
-};
-
 template <typename T, int E>
 __device__ __forceinline__ void fs_a_body(unsigned char* smem, const V_t<T>* src, V_t<T>* dst,
                                           int N1, int N2, int C, int Tt, int Ls, int j20,
-                                          int sgn, int shift, int mask, const V_t<T>* T1,
-                                          const V_t<T>* T2, const PassTable& pt,
-                                          const V_t<T>* tw) {
+                                          int sgn, const PassTable& pt, const V_t<T>* tw) {
     const int cshift = __ffs(C) - 1;
     const T sg = (T)sgn;
-    // Staged load: rows j1 of C contiguous columns (one run of C elements per row).
-    const int total = C * N1;
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-        const int c = idx & (C - 1);
-        const int e = idx >> cshift;
-        const int j2 = j20 + c;
-        V_t<T> v = V_t<T>{0, 0};
-        if (j2 < N2) v = src[(int64_t)e * N2 + j2];
-        v.y *= sg;
-        lds_line<T>(smem, C, Ls, c).put(e, v);
-    }
+    // Staged load: rows j1 of C contiguous columns (one run of C elements per row); columns past
+    // N2 (a partial last tile) load column N2 - 1 and are zeroed.
+    batched_for<kStageU>(
+        threadIdx.x, blockDim.x, C * N1,
+        [&](int idx) {
+            const int c = idx & (C - 1);
+            const int e = idx >> cshift;
+            const int j2 = j20 + c < N2 ? j20 + c : N2 - 1;
+            return src[(int64_t)e * N2 + j2];
+        },
+        [&](int idx, V_t<T> v) {
+            const int c = idx & (C - 1);
+            const int e = idx >> cshift;
+            if (j20 + c >= N2) v = V_t<T>{0, 0};
+            v.y *= sg;
+            lds_line<T>(smem, C, Ls, c).put(e, v);
+        });
     __syncthreads();
     const int tid = threadIdx.x;
     const int c = tid / Tt;
@@ -175,7 +200,7 @@ __device__ __forceinline__ void fs_a_body(unsigned char* smem, const V_t<T>* src
     const int j2 = j20 + c;
     const bool active = (c < C) && (j2 < N2);
     const LdsLine<T> L = lds_line<T>(smem, C, Ls, c < C ? c : 0);
-    TwiddleDst<T> d{dst + (int64_t)(active ? j2 : 0) * N1, T1, T2, j2, shift, mask};
+    GPlainDst<T> d{dst + (int64_t)(active ? j2 : 0) * N1};
     run_line<T, E>(pt, N1, t, Tt, active, tw, L, d, L, false, true);
 }
 
@@ -188,31 +213,55 @@ k_fs_a(const V_t<T>* __restrict__ in, V_t<T>* __restrict__ out, FourStepAArgs a,
     const int64_t bt = blockIdx.x / tiles;
     const int tile = blockIdx.x - (int)(bt * tiles);
     fs_a_body<T, E>(smem, in + bt * a.idist, out + bt * a.odist, a.N1, a.N2, a.C, a.Tt, a.Ls,
-                    tile * a.C, a.sgn_in, a.shift, a.mask, (const V_t<T>*)a.T1,
-                    (const V_t<T>*)a.T2, pt, tw);
+                    tile * a.C, a.sgn_in, pt, tw);
 }
 
 // ------------------------------------------------------------------------------------------
 // Four-step pass B body: lines k10 .. k10+C-1, staged load and staged (transposing) store.
-//   out[k1 + N1*k2] = DFT_N2(in[j2*N1 + k1])[k2];  in == out allowed (same index set).
+//   out[k1 + N1*k2] = DFT_N2(w_n^{j2 k1} in[j2*N1 + k1])[k2];  in == out allowed (same index
+// set).  w_n^e = T1[e & mask] * T2[e >> shift]; T1 == nullptr means no twiddle.
 // ------------------------------------------------------------------------------------------
+template <typename T>
+struct TwLoad {  // a staged element with the two factors of its twiddle
+    V_t<T> v, a, b;
+};
+
 template <typename T, int E>
 __device__ __forceinline__ void fs_b_body(unsigned char* smem, const V_t<T>* src, V_t<T>* dst,
                                           int N1, int N2, int C, int Tt, int Ls, int k10,
                                           int sgn_in, int sgn, double scale, const V_t<T>* K,
-                                          V_t<T>* dc, const PassTable& pt, const V_t<T>* tw) {
+                                          V_t<T>* dc, const V_t<T>* T1, const V_t<T>* T2,
+                                          int shift, int mask, const PassTable& pt,
+                                          const V_t<T>* tw) {
     const int cshift = __ffs(C) - 1;
     const int total = C * N2;
     const T sgi = (T)sgn_in;
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-        const int c = idx & (C - 1);
-        const int e = idx >> cshift;
-        const int k1 = k10 + c;
-        V_t<T> v = V_t<T>{0, 0};
-        if (k1 < N1) v = src[(int64_t)e * N1 + k1];
-        v.y *= sgi;
-        lds_line<T>(smem, C, Ls, c).put(e, v);
-    }
+    // Staged load of runs of C elements (k1) per middle row j2, with the twiddle factors loaded
+    // in the same batch.  Lines past N1 (a partial last tile) load line N1 - 1 and are zeroed.
+    batched_for<kStageU>(
+        threadIdx.x, blockDim.x, total,
+        [&](int idx) {
+            const int c = idx & (C - 1);
+            const int e = idx >> cshift;
+            const int k1 = k10 + c < N1 ? k10 + c : N1 - 1;
+            TwLoad<T> r;
+            r.v = src[(int64_t)e * N1 + k1];
+            if (T1) {
+                const int x = e * k1;
+                r.a = T1[x & mask];
+                r.b = T2[x >> shift];
+            }
+            return r;
+        },
+        [&](int idx, TwLoad<T> r) {
+            const int c = idx & (C - 1);
+            const int e = idx >> cshift;
+            V_t<T> v = r.v;
+            if (T1) v = cmul(v, cmul(r.a, r.b));
+            if (k10 + c >= N1) v = V_t<T>{0, 0};
+            v.y *= sgi;
+            lds_line<T>(smem, C, Ls, c).put(e, v);
+        });
     __syncthreads();
     const int tid = threadIdx.x;
     const int c = tid / Tt;
@@ -222,20 +271,32 @@ __device__ __forceinline__ void fs_b_body(unsigned char* smem, const V_t<T>* src
     run_line<T, E>(pt, N2, t, Tt, active, tw, L, L, L, false, false);
     // run_line ended with a barrier: the lines' outputs are in LDS.
     const T sx = (T)scale, sy = (T)(scale * sgn);
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-        const int cc = idx & (C - 1);
-        const int e = idx >> cshift;
-        const int k1 = k10 + cc;
-        if (k1 < N1) {
-            V_t<T> v = lds_line<T>(smem, C, Ls, cc)(e);
+    // Staged store in runs of C (k1) per output row k2; the epilogue spectrum is loaded in the
+    // same batch.  Indices of lines past N1 are skipped.
+    batched_for<kStageU>(
+        threadIdx.x, blockDim.x, total,
+        [&](int idx) {
+            const int cc = idx & (C - 1);
+            const int e = idx >> cshift;
+            const int k1 = k10 + cc < N1 ? k10 + cc : N1 - 1;
+            TwLoad<T> r;
+            r.v = lds_line<T>(smem, C, Ls, cc)(e);
+            if (K) r.a = K[(int64_t)k1 + (int64_t)N1 * e];
+            return r;
+        },
+        [&](int idx, TwLoad<T> r) {
+            const int cc = idx & (C - 1);
+            const int e = idx >> cshift;
+            const int k1 = k10 + cc;
+            if (k1 >= N1) return;
+            V_t<T> v = r.v;
             const int64_t o = (int64_t)k1 + (int64_t)N1 * e;
             if (K) {
                 if (dc && o == 0) *dc = v;
-                v = cconj(cmul(v, K[o]));
+                v = cconj(cmul(v, r.a));
             }
             dst[o] = V_t<T>{v.x * sx, v.y * sy};
-        }
-    }
+        });
 }
 
 template <typename T, int E>
@@ -248,7 +309,8 @@ k_fs_b(const V_t<T>* in, V_t<T>* out, FourStepBArgs a, PassTable pt,
     const int tile = blockIdx.x - (int)(bt * tiles);
     fs_b_body<T, E>(smem, in + bt * a.idist, out + bt * a.odist, a.N1, a.N2, a.C, a.Tt, a.Ls,
                     tile * a.C, a.sgn_in, a.sgn_out, a.scale, (const V_t<T>*)a.epi.K,
-                    a.epi.dc ? (V_t<T>*)a.epi.dc + 2 * bt : nullptr, pt, tw);
+                    a.epi.dc ? (V_t<T>*)a.epi.dc + 2 * bt : nullptr, (const V_t<T>*)a.T1,
+                    (const V_t<T>*)a.T2, a.shift, a.mask, pt, tw);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -315,13 +377,17 @@ __device__ __forceinline__ void conv_body(unsigned char* smem, const LINES& line
     V_t<T>* sdc = reinterpret_cast<V_t<T>*>(aim + C * LsA);
     const T sg = (T)sgn;
     // 1. staged, coalesced load of the nl input lines (flat over lines x elements)
-    for (int idx = threadIdx.x; idx < nl * n; idx += blockDim.x) {
-        const int c = idx / n;
-        const int e = idx - c * n;
-        V_t<T> v = lines.in(c)[e];
-        v.y *= sg;
-        LdsLine<T>{xre + c * LsX, xim + c * LsX}.put(e, v);
-    }
+    batched_for<kStageU>(
+        threadIdx.x, blockDim.x, nl * n,
+        [&](int idx) {
+            const int c = idx / n;
+            return lines.in(c)[idx - c * n];
+        },
+        [&](int idx, V_t<T> v) {
+            const int c = idx / n;
+            v.y *= sg;
+            LdsLine<T>{xre + c * LsX, xim + c * LsX}.put(idx - c * n, v);
+        });
     __syncthreads();
     const int tid = threadIdx.x;
     const int c = tid / Tt;
@@ -347,12 +413,16 @@ __device__ __forceinline__ void conv_body(unsigned char* smem, const LINES& line
     __syncthreads();
     // 4. staged, coalesced store
     const T sx = (T)scale, sy = (T)(scale * sgn);
-    for (int idx = threadIdx.x; idx < nl * n; idx += blockDim.x) {
-        const int c2 = idx / n;
-        const int e = idx - c2 * n;
-        const V_t<T> v = LdsLine<T>{xre + c2 * LsX, xim + c2 * LsX}(e);
-        lines.out(c2)[e] = V_t<T>{v.x * sx, v.y * sy};
-    }
+    batched_for<kStageU>(
+        threadIdx.x, blockDim.x, nl * n,
+        [&](int idx) {
+            const int c2 = idx / n;
+            return LdsLine<T>{xre + c2 * LsX, xim + c2 * LsX}(idx - c2 * n);
+        },
+        [&](int idx, V_t<T> v) {
+            const int c2 = idx / n;
+            lines.out(c2)[idx - c2 * n] = V_t<T>{v.x * sx, v.y * sy};
+        });
 }
 
 template <typename T, int E>
@@ -370,28 +440,52 @@ k_conv_small(const V_t<T>* __restrict__ in, V_t<T>* __restrict__ out, ConvSmallA
 // ------------------------------------------------------------------------------------------
 // Large convolution gather / scatter bodies over elements [e0, e1) of one line.
 // ------------------------------------------------------------------------------------------
+constexpr int kGatherU = 4;  // elements per thread whose loads share one wait
+
 template <typename Ti, typename Tc>
 __device__ __forceinline__ void gather_body(const V_t<Ti>* xs, V_t<Tc>* ws, V_t<Tc>* slots,
                                             int64_t e0, int64_t e1, int64_t estep, int64_t n,
                                             int mode, int sgn, const int* perm,
                                             const V_t<Tc>* chirp) {
     const Tc sg = (Tc)sgn;
-    for (int64_t e = e0; e < e1; e += estep) {
-        V_t<Tc> v = V_t<Tc>{0, 0};
-        if (mode == (int)ConvMode::Rader) {
-            if (e < n - 1) {
-                const V_t<Ti> u = xs[perm[e]];
-                v = V_t<Tc>{(Tc)u.x, (Tc)u.y * sg};
-            }
-            if (e == 0) {  // keep x0 for the scatter (the input may be overwritten in place)
-                const V_t<Ti> u0 = xs[0];
-                slots[1] = V_t<Tc>{(Tc)u0.x, (Tc)u0.y * sg};
-            }
-        } else if (e < n) {
-            const V_t<Ti> u = xs[e];
-            v = cmul(V_t<Tc>{(Tc)u.x, (Tc)u.y * sg}, chirp[e]);
+    const bool rader = mode == (int)ConvMode::Rader;
+    const int64_t live_end = rader ? n - 1 : n;  // a[e] = 0 from here on (zero padding)
+    const int64_t last = live_end > 0 ? live_end - 1 : 0;
+    if (rader && e0 == 0) {  // keep x0 for the scatter (the input may be overwritten in place)
+        const V_t<Ti> u0 = xs[0];
+        slots[1] = V_t<Tc>{(Tc)u0.x, (Tc)u0.y * sg};
+    }
+    // Batches of kGatherU elements: permutation indices, then the permuted loads (and chirp),
+    // then the stores; every load uses a clamped, valid index.
+    for (int64_t e = e0; e < e1; e += kGatherU * estep) {
+        int64_t src[kGatherU];
+#pragma unroll
+        for (int u = 0; u < kGatherU; ++u) {
+            const int64_t ee = e + u * estep;
+            const int64_t ec = ee < last ? ee : last;
+            src[u] = rader ? (int64_t)perm[ec] : ec;
         }
-        ws[e] = v;
+        V_t<Ti> x[kGatherU];
+        V_t<Tc> ch[kGatherU];
+#pragma unroll
+        for (int u = 0; u < kGatherU; ++u) {
+            x[u] = xs[src[u]];
+            if (!rader) {
+                const int64_t ee = e + u * estep;
+                ch[u] = chirp[ee < last ? ee : last];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kGatherU; ++u) {
+            const int64_t ee = e + u * estep;
+            if (ee >= e1) break;
+            V_t<Tc> v = V_t<Tc>{0, 0};
+            if (ee < live_end) {
+                v = V_t<Tc>{(Tc)x[u].x, (Tc)x[u].y * sg};
+                if (!rader) v = cmul(v, ch[u]);
+            }
+            ws[ee] = v;
+        }
     }
 }
 
@@ -404,22 +498,43 @@ __device__ __forceinline__ void scatter_body(const V_t<Tc>* ws, V_t<Ti>* ys,
                                              const V_t<Tc>* chirp) {
     const Tc invL = (Tc)invL_;
     const Tc sx = (Tc)scale, sy = (Tc)(scale * sgn);
-    if (mode == (int)ConvMode::Rader) {
-        const V_t<Tc> x0 = slots[1];
-        for (int64_t k = k0; k < k1 && k < n - 1; k += kstep) {
-            const V_t<Tc> v = ws[k];
-            const V_t<Tc> r = V_t<Tc>{x0.x + v.x * invL, x0.y - v.y * invL};
-            ys[iperm[k]] = V_t<Ti>{(Ti)(r.x * sx), (Ti)(r.y * sy)};
-            if (k == 0) {
-                const V_t<Tc> d = slots[0];
-                ys[0] = V_t<Ti>{(Ti)((x0.x + d.x) * sx), (Ti)((x0.y + d.y) * sy)};
-            }
+    const bool rader = mode == (int)ConvMode::Rader;
+    const int64_t kend0 = rader ? n - 1 : n;
+    const int64_t kend = k1 < kend0 ? k1 : kend0;  // this thread's range is [k0, kend)
+    if (kend <= 0) return;
+    const int64_t last = kend0 - 1;
+    V_t<Tc> x0 = V_t<Tc>{0, 0};
+    if (rader) {
+        x0 = slots[1];
+        if (k0 == 0) {
+            const V_t<Tc> d = slots[0];
+            ys[0] = V_t<Ti>{(Ti)((x0.x + d.x) * sx), (Ti)((x0.y + d.y) * sy)};
         }
-    } else {
-        for (int64_t k = k0; k < k1 && k < n; k += kstep) {
-            const V_t<Tc> v = ws[k];
-            const V_t<Tc> r = cmul(V_t<Tc>{v.x * invL, -v.y * invL}, chirp[k]);
-            ys[k] = V_t<Ti>{(Ti)(r.x * sx), (Ti)(r.y * sy)};
+    }
+    for (int64_t k = k0; k < kend; k += kGatherU * kstep) {
+        V_t<Tc> v[kGatherU], ch[kGatherU];
+        int64_t dst[kGatherU];
+#pragma unroll
+        for (int u = 0; u < kGatherU; ++u) {
+            const int64_t kk = k + u * kstep;
+            const int64_t kc = kk < last ? kk : last;
+            v[u] = ws[kc];
+            if (rader)
+                dst[u] = iperm[kc];
+            else
+                ch[u] = chirp[kc];
+        }
+#pragma unroll
+        for (int u = 0; u < kGatherU; ++u) {
+            const int64_t kk = k + u * kstep;
+            if (kk >= kend) break;
+            if (rader) {
+                const V_t<Tc> r = V_t<Tc>{x0.x + v[u].x * invL, x0.y - v[u].y * invL};
+                ys[dst[u]] = V_t<Ti>{(Ti)(r.x * sx), (Ti)(r.y * sy)};
+            } else {
+                const V_t<Tc> r = cmul(V_t<Tc>{v[u].x * invL, -v[u].y * invL}, ch[u]);
+                ys[kk] = V_t<Ti>{(Ti)(r.x * sx), (Ti)(r.y * sy)};
+            }
         }
     }
 }
@@ -626,8 +741,7 @@ k_fs_a_multi(MultiArgs ma, const MItem* __restrict__ items, const MPlan* __restr
     fs_a_body<T, kMultiE>(smem, mptr<const V_t<T>>(it.in, it.flags & kSelInMask, ma.in_base, ma.out_base),
                           mptr<V_t<T>>(it.out, (it.flags >> kSelOutShift) & 15, ma.in_base, ma.out_base),
                           P->N1, P->N2, P->CA, P->la.Tt, P->la.Ls, it.count * P->CA, sgn,
-                          P->shift, P->mask, (const V_t<T>*)P->T1, (const V_t<T>*)P->T2, P->la.pt,
-                          (const V_t<T>*)P->la.tw);
+                          P->la.pt, (const V_t<T>*)P->la.tw);
 }
 
 template <typename T>
@@ -644,7 +758,8 @@ k_fs_b_multi(MultiArgs ma, const MItem* __restrict__ items, const MPlan* __restr
                           mptr<V_t<T>>(it.out, (it.flags >> kSelOutShift) & 15, ma.in_base, ma.out_base),
                           P->N1, P->N2, P->CB, P->lb.Tt, P->lb.Ls, it.count * P->CB, 1, sgn, scale,
                           epi ? (const V_t<T>*)P->K : nullptr,
-                          epi ? reinterpret_cast<V_t<T>*>(it.dc) : nullptr, P->lb.pt,
+                          epi ? reinterpret_cast<V_t<T>*>(it.dc) : nullptr,
+                          (const V_t<T>*)P->T1, (const V_t<T>*)P->T2, P->shift, P->mask, P->lb.pt,
                           (const V_t<T>*)P->lb.tw);
 }
 

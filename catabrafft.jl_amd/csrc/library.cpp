@@ -575,10 +575,6 @@ static int run_fourstep(const Plan& p, const void* in, void* mid, void* out, int
         a.idist = idist;
         a.odist = mdist;
         a.sgn_in = sgn;
-        a.shift = fs.shift;
-        a.mask = fs.mask;
-        a.T1 = fs.T1.p;
-        a.T2 = fs.T2.p;
         CFFT_HIP(launch_fourstep_a(p.prec, inb, midb, a, fs.a.pt, fs.a.dtw.p, st));
         FourStepBArgs b{};
         b.N1 = fs.N1;
@@ -595,6 +591,10 @@ static int run_fourstep(const Plan& p, const void* in, void* mid, void* out, int
         b.scale = scale;
         b.epi = epi;
         if (epi.dc) b.epi.dc = (char*)epi.dc + (size_t)b0 * 2 * es;
+        b.shift = fs.shift;
+        b.mask = fs.mask;
+        b.T1 = fs.T1.p;
+        b.T2 = fs.T2.p;
         CFFT_HIP(launch_fourstep_b(p.prec, midb, outb, b, fs.b.pt, fs.b.dtw.p, st));
     }
     return CFFT_OK;

@@ -32,8 +32,10 @@ def compile_one(src):
     newest = max([os.path.getmtime(src)] + [os.path.getmtime(d) for d in deps()])
     if os.path.exists(obj) and os.path.getmtime(obj) >= newest:
         return obj
-    cmd = ["g++"] + FLAGS + ["-x", "c++", "-c", src, "-o", obj]
+    tmp = f"{obj}.{os.getpid()}.tmp"
+    cmd = ["g++"] + FLAGS + ["-x", "c++", "-c", src, "-o", tmp]
     subprocess.run(cmd, check=True)
+    os.replace(tmp, obj)
     return obj
 
 
@@ -41,7 +43,12 @@ def build():
     os.makedirs(OUT, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=8) as ex:
         objs = list(ex.map(compile_one, sources()))
-    subprocess.run(["g++", "-shared", "-o", LIB] + objs + ["-lpthread"], check=True)
+    if os.path.exists(LIB) and os.path.getmtime(LIB) >= max(os.path.getmtime(o) for o in objs):
+        return LIB
+    # link under a private name and rename: concurrent test workers never see a partial file
+    tmp = f"{LIB}.{os.getpid()}.tmp"
+    subprocess.run(["g++", "-shared", "-o", tmp] + objs + ["-lpthread"], check=True)
+    os.replace(tmp, LIB)
     return LIB
 
 

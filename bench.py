@@ -75,6 +75,20 @@ def cpu_baseline(budget_s: float = 12.0):
     }
 
 
+def measured_traffic(plan_desc: str):
+    """HBM bytes per step of a case as measured by rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by a
+    plain copy) for this exact plan, from profiles/batched_traffic.json; None when the plan was
+    not profiled."""
+    path = os.path.join(ROOT, "profiles", "batched_traffic.json")
+    try:
+        with open(path) as f:
+            table = json.load(f)["cases"]
+    except (OSError, ValueError, KeyError):
+        return None
+    entry = table.get(plan_desc)
+    return entry["bytes_per_step"] if entry else None
+
+
 def bench_batched(args):
     import torch
 
@@ -151,6 +165,7 @@ def bench_batched(args):
     dom = int(np.argmax(per_case))
     n_d, b_d, dt_d = cases[dom]
     achieved = algo_bytes(n_d, b_d, dt_d) / (per_case[dom] * 1e-3) / 1e9
+    traffic = measured_traffic(plans[dom].describe())
     out = {
         "metric": "GFLOP/s (5N·log2N) & % HBM peak, batched FP32 vs FP64; serving p50/p99 latency",
         "value": round(value, 1),
@@ -169,8 +184,12 @@ def bench_batched(args):
                    "global_batch": sum(b for _, b, _ in cases), "parallelism": "single GPU"},
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "traffic_vs_algorithmic": (round(traffic / algo_bytes(n_d, b_d, dt_d), 3)
+                                       if traffic else None),
             "kernel": f"n={n_d} x {b_d} {np.dtype(dt_d).name}: {plans[dom].describe()}",
+            "per": "one step of the case: every launch of its plan (achieved, traffic in bytes)",
         },
         "breakdown": breakdown,
     }

@@ -580,15 +580,28 @@ template <typename T, class ZSRC>
 __device__ __forceinline__ void r2c_post(const ZSRC& Z, V_t<T>* X, const V_t<T>* w, int m, int t,
                                          int Tt, bool active, T scale) {
     if (!active) return;
-    for (int k = t; k <= m / 2; k += Tt) {
-        const int j = m - k;
-        const V_t<T> zk = Z(k), zj = Z(j == m ? 0 : j);
-        V_t<T> xk, xj;
-        r2c_pair<T>(zk, zj, w[k], xk);
-        const V_t<T> wj = V_t<T>{-w[k].x, w[k].y};  // w^{m-k} = -conj(w^k)
-        r2c_pair<T>(zj, zk, wj, xj);
-        X[k] = V_t<T>{xk.x * scale, xk.y * scale};
-        X[j] = V_t<T>{xj.x * scale, xj.y * scale};
+    constexpr int U = 4;  // pairs per batch: their table loads share one wait
+    const int h = m / 2;
+    for (int k0 = t; k0 <= h; k0 += U * Tt) {
+        V_t<T> wk[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = k0 + u * Tt;
+            wk[u] = w[k < h ? k : h];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = k0 + u * Tt;
+            if (k > h) break;
+            const int j = m - k;
+            const V_t<T> zk = Z(k), zj = Z(j == m ? 0 : j);
+            V_t<T> xk, xj;
+            r2c_pair<T>(zk, zj, wk[u], xk);
+            const V_t<T> wj = V_t<T>{-wk[u].x, wk[u].y};  // w^{m-k} = -conj(w^k)
+            r2c_pair<T>(zj, zk, wj, xj);
+            X[k] = V_t<T>{xk.x * scale, xk.y * scale};
+            X[j] = V_t<T>{xj.x * scale, xj.y * scale};
+        }
     }
 }
 

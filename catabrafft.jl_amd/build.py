@@ -25,8 +25,12 @@ LIB = os.path.join(LIBDIR, "libcatabrafft.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # -packed-fp32-ops keeps the compiler from emitting packed FP32 VALU instructions.
+# -ffp-contract=off: no multiply-add fusion, so every kernel that computes a transform performs
+# the same IEEE operations in the same order and gives the same bits (a transform alone or in a
+# work list, pipelined or not), and so does the CPU emulation build of the same sources.
 CFLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-fvisibility=hidden",
+    "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", CSRC, "-I", os.path.join(ROOT, "include"),
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",

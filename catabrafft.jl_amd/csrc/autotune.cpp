@@ -35,14 +35,16 @@ static std::vector<PlanOptions> candidates(Ctx* ctx, const Plan& p, int flags) {
         return out;
     }
     if (is_smooth(p.n)) {
-        // direct shapes
+        // direct shapes, with and without the pipelined rows kernel
         for (int E : Es)
-            for (int tt : tts) {
-                PlanOptions o = base;
-                o.E = E;
-                o.target_threads = tt;
-                out.push_back(o);
-            }
+            for (int tt : tts)
+                for (int pipe : {0, 1}) {
+                    PlanOptions o = base;
+                    o.E = E;
+                    o.target_threads = tt;
+                    o.pipe = pipe;
+                    out.push_back(o);
+                }
         // four-step splits near sqrt(n) with a few tile widths
         std::vector<int64_t> splits;
         for (int64_t a = 2; a < p.n; ++a) {
@@ -102,7 +104,8 @@ static void copy_shape(const Plan& from, Plan& to) {
 
 // ------------------------------------------------------------------------------- wisdom
 // Tuned choices keyed by (n, batch, idist, odist, dtype, flags); kept per context and written /
-// read as text lines "n batch idist odist dtype flags : E target_threads n1 ca cb chunk".
+// read as text lines "n batch idist odist dtype flags : E target_threads n1 ca cb chunk pipe"
+// (files without the last field read pipe = default).
 static std::string wisdom_key(const Plan& p, int flags) {
     std::ostringstream k;
     k << p.n << " " << p.batch << " " << p.idist << " " << p.odist << " " << p.dtype << " " << flags;
@@ -116,8 +119,8 @@ int wisdom_export(Ctx* ctx, const char* path) {
     std::fprintf(f, "# catabrafft wisdom v1 (gfx950)\n");
     for (const auto& kv : ctx->wisdom) {
         const PlanOptions& o = kv.second;
-        std::fprintf(f, "%s : %d %d %lld %d %d %lld\n", kv.first.c_str(), o.E, o.target_threads,
-                     (long long)o.n1, o.ca, o.cb, (long long)o.chunk);
+        std::fprintf(f, "%s : %d %d %lld %d %d %lld %d\n", kv.first.c_str(), o.E,
+                     o.target_threads, (long long)o.n1, o.ca, o.cb, (long long)o.chunk, o.pipe);
     }
     std::fclose(f);
     return CFFT_OK;
@@ -138,8 +141,11 @@ int wisdom_import(Ctx* ctx, const char* path) {
         while (!key.empty() && key.back() == ' ') key.pop_back();
         PlanOptions o;
         long long n1 = 0, chunk = -1;
-        if (std::sscanf(colon + 1, "%d %d %lld %d %d %lld", &o.E, &o.target_threads, &n1, &o.ca,
-                        &o.cb, &chunk) != 6) {
+        int pipe = -1;
+        const int got = std::sscanf(colon + 1, "%d %d %lld %d %d %lld %d", &o.E,
+                                    &o.target_threads, &n1, &o.ca, &o.cb, &chunk, &pipe);
+        o.pipe = got == 7 ? pipe : -1;
+        if (got != 6 && got != 7) {
             std::fclose(f);
             return fail(CFFT_ERR_INVALID, "malformed wisdom line");
         }

@@ -83,12 +83,43 @@ __host__ __device__ constexpr int tw_prefetch_vgprs(int data) {
 // Per-thread butterfly capacity: MB butterflies of radix R with E complex registers.
 __host__ __device__ constexpr int bfly_per_thread(int E, int R) { return E / R > 0 ? E / R : 1; }
 
+// Twiddle tables.  Pass i of a line reads w_g^{p k} (p < n1, 1 <= k < R) from its block at
+// twoff[i] through an accessor:
+//   TwGlobal: the plan's table in global memory, row-major per pass, [p (R-1) + k - 1];
+//   TwLds:    a copy in LDS, k-major per pass, [(k-1) n1 + p]: consecutive butterflies read
+//             consecutive words, so a wave's reads are free of bank conflicts.
+// Global twiddles are loaded ahead of the butterflies (kPrefetch); LDS ones where they are used.
+template <typename V>
+struct TwRowAcc {
+    const V* p;
+    static constexpr bool kPrefetch = true;
+    template <int R>
+    __device__ __forceinline__ V get(int pp, int k) const { return p[pp * (R - 1) + k - 1]; }
+};
+template <typename V>
+struct TwColAcc {
+    const V* p;
+    int n1;
+    static constexpr bool kPrefetch = false;
+    template <int R>
+    __device__ __forceinline__ V get(int pp, int k) const { return p[(k - 1) * n1 + pp]; }
+};
+template <typename V>
+struct TwGlobal {
+    const V* tw;
+    __device__ __forceinline__ TwRowAcc<V> pass(int off, int n1) const { return {tw + off}; }
+};
+template <typename V>
+struct TwLds {
+    const V* tw;
+    __device__ __forceinline__ TwColAcc<V> pass(int off, int n1) const { return {tw + off, n1}; }
+};
+
 // One Stockham pass.  SRC(e) returns element e of the pass input, DST.put(e, v) stores element e
 // of its output.  mid_barrier separates the reads from the writes when both go to the same LDS.
-template <typename T, int R, int MB, class SRC, class DST>
+template <typename T, int R, int MB, class SRC, class DST, class TWA>
 __device__ __forceinline__ void stockham_pass(const int N, const int s, const int t_in, const int Tt,
-                                              const bool active,
-                                              const typename Vec2<T>::type* __restrict__ tw,
+                                              const bool active, const TWA& tw,
                                               const SRC& src, const DST& dst,
                                               const bool mid_barrier) {
     using V = typename Vec2<T>::type;
@@ -104,7 +135,7 @@ __device__ __forceinline__ void stockham_pass(const int N, const int s, const in
     // butterfly); the rest are loaded after the butterflies.  The table has rows for p = 0 (all
     // ones), so every butterfly multiplies.  The last pass (s = N / R) has no twiddles.
     constexpr int VW = (int)(sizeof(V) / 4);
-    constexpr int KP0 = tw_prefetch_vgprs(MB * R * VW) / (MB * VW);
+    constexpr int KP0 = TWA::kPrefetch ? tw_prefetch_vgprs(MB * R * VW) / (MB * VW) : 0;
     constexpr int KP = (R - 1) < KP0 ? (R - 1) : KP0;
     const bool twiddled = s * R < N;
     V w[MB][KP > 0 ? KP : 1];
@@ -113,9 +144,8 @@ __device__ __forceinline__ void stockham_pass(const int N, const int s, const in
         for (int ib = 0; ib < MB; ++ib) {
             const int bt = t + ib * Tt;
             const int b = bt < nb ? bt : nb - 1;
-            const V* wp = tw + (b / s) * (R - 1);
 #pragma unroll
-            for (int k = 0; k < KP; ++k) w[ib][k] = wp[k];
+            for (int k = 0; k < KP; ++k) w[ib][k] = tw.template get<R>(b / s, k + 1);
         }
     }
     V v[MB][R];
@@ -138,9 +168,8 @@ __device__ __forceinline__ void stockham_pass(const int N, const int s, const in
             if (twiddled) {
 #pragma unroll
                 for (int k = 1; k <= KP; ++k) v[ib][k] = cmul(v[ib][k], w[ib][k - 1]);
-                const V* wp = tw + p * (R - 1);
 #pragma unroll
-                for (int k = KP + 1; k < R; ++k) v[ib][k] = cmul(v[ib][k], wp[k - 1]);
+                for (int k = KP + 1; k < R; ++k) v[ib][k] = cmul(v[ib][k], tw.template get<R>(p, k));
             }
             const int base = q + s * R * p;
 #pragma unroll
@@ -166,11 +195,50 @@ __device__ __forceinline__ void stockham_pass(const int N, const int s, const in
         default: break;                                              \
     }
 
-// Runs every pass of one line.  GSRC / GDST are the input and output functors of the line (HBM,
-// or a gather / scatter over another LDS buffer); when from_global (to_global) is false the
-// line's input is already in LDS (its output is left in LDS).  dst_aliases_lds asks for a
-// barrier between the reads and writes of the last pass because GDST writes the line's own LDS.
-// The caller provides the barrier before the first pass when it staged the input into LDS.
+// Runs passes ip0.. of one line (s0 = product of the radices before ip0).  GSRC / GDST are the
+// input and output functors of the line (HBM, or a gather / scatter over another LDS buffer);
+// when from_global (to_global) is false the line's input is already in LDS (its output is left
+// in LDS).  dst_aliases_lds asks for a barrier between the reads and writes of the last pass
+// because GDST writes the line's own LDS.  The caller provides the barrier before the first pass
+// when it staged the input into LDS.  TWT is a twiddle table (TwGlobal or TwLds).
+template <typename T, int E, class TWT, class GSRC, class GDST>
+__device__ __forceinline__ void run_line_t(const PassTable& pt, const int N, const int t,
+                                           const int Tt, const bool active, const TWT& twt,
+                                           const GSRC& gsrc, const GDST& gdst,
+                                           const LdsLine<T>& lds, const bool from_global,
+                                           const bool to_global, const bool dst_aliases_lds,
+                                           const int ip0, const int s0) {
+    int s = s0;
+    for (int ip = ip0; ip < pt.npass; ++ip) {
+        const int R = pt.radix[ip];
+        const bool first = ip == ip0, last = ip == pt.npass - 1;
+        const auto twa = twt.pass(pt.twoff[ip], N / (s * R));
+        const bool sg = first && from_global;
+        const bool dg = last && to_global;
+        if (sg && dg) {
+#define CFFT_CALL_GG(RR) stockham_pass<T, RR, bfly_per_thread(E, RR)>(N, s, t, Tt, active, twa, gsrc, gdst, dst_aliases_lds)
+            CFFT_RADIX_SWITCH(R, CFFT_CALL_GG)
+#undef CFFT_CALL_GG
+        } else if (sg) {
+#define CFFT_CALL_GL(RR) stockham_pass<T, RR, bfly_per_thread(E, RR)>(N, s, t, Tt, active, twa, gsrc, lds, false)
+            CFFT_RADIX_SWITCH(R, CFFT_CALL_GL)
+#undef CFFT_CALL_GL
+        } else if (dg) {
+#define CFFT_CALL_LG(RR) stockham_pass<T, RR, bfly_per_thread(E, RR)>(N, s, t, Tt, active, twa, lds, gdst, dst_aliases_lds)
+            CFFT_RADIX_SWITCH(R, CFFT_CALL_LG)
+#undef CFFT_CALL_LG
+        } else {
+#define CFFT_CALL_LL(RR) stockham_pass<T, RR, bfly_per_thread(E, RR)>(N, s, t, Tt, active, twa, lds, lds, true)
+            CFFT_RADIX_SWITCH(R, CFFT_CALL_LL)
+#undef CFFT_CALL_LL
+        }
+        // The pass output sits in LDS for the next pass (or for the caller's staged store).
+        if (!dg) __syncthreads();
+        s *= R;
+    }
+}
+
+// Every pass of one line with the plan's twiddle table in global memory.
 template <typename T, int E, class GSRC, class GDST>
 __device__ __forceinline__ void run_line(const PassTable& pt, const int N, const int t,
                                          const int Tt, const bool active,
@@ -179,34 +247,8 @@ __device__ __forceinline__ void run_line(const PassTable& pt, const int N, const
                                          const LdsLine<T>& lds, const bool from_global,
                                          const bool to_global,
                                          const bool dst_aliases_lds = false) {
-    int s = 1;
-    for (int ip = 0; ip < pt.npass; ++ip) {
-        const int R = pt.radix[ip];
-        const bool first = ip == 0, last = ip == pt.npass - 1;
-        const typename Vec2<T>::type* twp = tw + pt.twoff[ip];
-        const bool sg = first && from_global;
-        const bool dg = last && to_global;
-        if (sg && dg) {
-#define CFFT_CALL_GG(RR) stockham_pass<T, RR, bfly_per_thread(E, RR)>(N, s, t, Tt, active, twp, gsrc, gdst, dst_aliases_lds)
-            CFFT_RADIX_SWITCH(R, CFFT_CALL_GG)
-#undef CFFT_CALL_GG
-        } else if (sg) {
-#define CFFT_CALL_GL(RR) stockham_pass<T, RR, bfly_per_thread(E, RR)>(N, s, t, Tt, active, twp, gsrc, lds, false)
-            CFFT_RADIX_SWITCH(R, CFFT_CALL_GL)
-#undef CFFT_CALL_GL
-        } else if (dg) {
-#define CFFT_CALL_LG(RR) stockham_pass<T, RR, bfly_per_thread(E, RR)>(N, s, t, Tt, active, twp, lds, gdst, dst_aliases_lds)
-            CFFT_RADIX_SWITCH(R, CFFT_CALL_LG)
-#undef CFFT_CALL_LG
-        } else {
-#define CFFT_CALL_LL(RR) stockham_pass<T, RR, bfly_per_thread(E, RR)>(N, s, t, Tt, active, twp, lds, lds, true)
-            CFFT_RADIX_SWITCH(R, CFFT_CALL_LL)
-#undef CFFT_CALL_LL
-        }
-        // The pass output sits in LDS for the next pass (or for the caller's staged store).
-        if (!dg) __syncthreads();
-        s *= R;
-    }
+    run_line_t<T, E>(pt, N, t, Tt, active, TwGlobal<typename Vec2<T>::type>{tw}, gsrc, gdst, lds,
+                     from_global, to_global, dst_aliases_lds, 0, 1);
 }
 
 }  // namespace cfft

@@ -24,6 +24,7 @@ struct RowsArgs {
     int sgn_in, sgn_out;   // -1 conjugates on load / store (backward = conj(fft(conj x)))
     double scale;          // applied on store
     ConvEpilogue epi;      // epi.K == nullptr: none
+    int grid;              // pipelined kernel: workgroups launched (each loops over line groups)
 };
 
 // Four-step pass A for n = N1*N2 (lines = the N2 columns of the N1 x N2 row-major input):
@@ -110,6 +111,15 @@ void init_kernels();
 
 hipError_t launch_rows(Prec p, const void* in, void* out, const RowsArgs& a, const PassTable& pt,
                        const void* tw, hipStream_t st);
+// Persistent pipelined rows kernel (needs npass >= 2, no epilogue); a.grid workgroups.  twn =
+// entries of the twiddle table, which the kernel copies into LDS after the work lines.
+hipError_t launch_rows_pipe(Prec p, const void* in, void* out, const RowsArgs& a,
+                            const PassTable& pt, const void* tw, size_t twn, hipStream_t st);
+// LDS bytes of a pipelined rows launch and its resident workgroups per CU (0: cannot launch).
+// Its workgroups have at most kPipeMaxThreads threads.
+constexpr int kPipeMaxThreads = 512;
+size_t rows_pipe_lds(Prec p, int C, int Ls, size_t twn);
+int rows_pipe_blocks_per_cu(Prec p, int E, int R0, int block, size_t lds);
 hipError_t launch_fourstep_a(Prec p, const void* in, void* out, const FourStepAArgs& a,
                              const PassTable& pt, const void* tw, hipStream_t st);
 hipError_t launch_fourstep_b(Prec p, const void* in, void* out, const FourStepBArgs& a,

@@ -8,6 +8,23 @@ namespace cfft {
 void init_kernels() {
     init_kernels_f32();
     init_kernels_f64();
+    init_kernels_pipe_f32();
+    init_kernels_pipe_f64();
+}
+
+hipError_t launch_rows_pipe(Prec p, const void* in, void* out, const RowsArgs& a,
+                            const PassTable& pt, const void* tw, size_t twn, hipStream_t st) {
+    return p == Prec::F32 ? rows_pipe_f32(in, out, a, pt, tw, twn, st)
+                          : rows_pipe_f64(in, out, a, pt, tw, twn, st);
+}
+size_t rows_pipe_lds(Prec p, int C, int Ls, size_t twn) {
+    const size_t el = p == Prec::F32 ? 4 : 8;
+    const size_t off = (2 * el * (size_t)C * Ls + 15) & ~(size_t)15;
+    return off + twn * 2 * el;
+}
+int rows_pipe_blocks_per_cu(Prec p, int E, int R0, int block, size_t lds) {
+    return p == Prec::F32 ? rows_pipe_occ_f32(E, R0, block, lds)
+                          : rows_pipe_occ_f64(E, R0, block, lds);
 }
 
 hipError_t launch_rows(Prec p, const void* in, void* out, const RowsArgs& a, const PassTable& pt,

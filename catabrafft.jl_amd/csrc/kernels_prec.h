@@ -28,6 +28,15 @@ hipError_t real_aux_f64(const void* in, void* out, const RealAuxArgs& a, hipStre
 void init_kernels_f32();
 void init_kernels_f64();
 
+hipError_t rows_pipe_f32(const void* in, void* out, const RowsArgs& a, const PassTable& pt,
+                         const void* tw, size_t twn, hipStream_t st);
+hipError_t rows_pipe_f64(const void* in, void* out, const RowsArgs& a, const PassTable& pt,
+                         const void* tw, size_t twn, hipStream_t st);
+int rows_pipe_occ_f32(int E, int R0, int block, size_t lds);
+int rows_pipe_occ_f64(int E, int R0, int block, size_t lds);
+void init_kernels_pipe_f32();
+void init_kernels_pipe_f64();
+
 hipError_t gather_f32_f32(const void* x, void* w, const ConvGatherArgs& a, hipStream_t st);
 hipError_t gather_f32_f64(const void* x, void* w, const ConvGatherArgs& a, hipStream_t st);
 hipError_t gather_f64_f64(const void* x, void* w, const ConvGatherArgs& a, hipStream_t st);

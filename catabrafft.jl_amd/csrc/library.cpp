@@ -247,13 +247,28 @@ static int rows_lines_per_block(const LinePlan& lp, Prec pr, int64_t nlines, siz
     return c;
 }
 
-static int plan_direct(Ctx* ctx, Plan& p, int E, int target_threads) {
+// Workgroups of the pipelined rows kernel that are resident at once on the device (0 = the plan
+// cannot use it: one pass only, or the lines plus the LDS twiddle copy exceed the LDS).
+static int pipe_resident(Ctx* ctx, const Plan& p) {
+    if (p.line.pt.npass < 2 || p.C * p.line.Tt > kPipeMaxThreads) return 0;
+    const size_t twn = p.line.dtw.bytes / (size_t)esize(p.prec);
+    const size_t lds = rows_pipe_lds(p.prec, p.C, p.line.Ls, twn);
+    if (lds > ctx->lds_max) return 0;
+    const int bpc = rows_pipe_blocks_per_cu(p.prec, p.line.E, p.line.pt.radix[0],
+                                            p.C * p.line.Tt, lds);
+    return bpc > 0 ? bpc * ctx->num_cu : 0;
+}
+
+static int plan_direct(Ctx* ctx, Plan& p, int E, int target_threads, int pipe = -1) {
     int rc = make_line(p.line, p.n, p.prec, E, 1024);
     if (rc) return rc;
     const size_t per = 2 * (size_t)esize(p.prec) / 2 * p.line.Ls;
     if (per > ctx->lds_max) return fail(CFFT_ERR_UNSUPPORTED, "line does not fit LDS");
     p.C = rows_lines_per_block(p.line, p.prec, p.batch, std::max<size_t>(per, ctx->lds_rows_budget),
                                target_threads);
+    // The pipelined kernel trades occupancy for a prefetch: a tuning candidate (MEASURE /
+    // ENCHANT, wisdom), not the default.
+    p.pipe_res = pipe > 0 ? pipe_resident(ctx, p) : 0;
     p.kind = Kind::Direct;
     return CFFT_OK;
 }
@@ -498,7 +513,8 @@ static int plan_fft_core(Ctx* ctx, Plan& p, const PlanOptions& o) {
     }
     const int E = o.E ? o.E : 16;
     if (o.force_conv == 0 && is_smooth(p.n)) {
-        if (o.n1 == 0 && line_fits(ctx, p.n, p.prec, E)) return plan_direct(ctx, p, E, o.target_threads);
+        if (o.n1 == 0 && line_fits(ctx, p.n, p.prec, E))
+            return plan_direct(ctx, p, E, o.target_threads, o.pipe);
         int rc = plan_fourstep_factors(ctx, p.fs, p.n, p.prec, E, o.n1, o.ca, o.cb,
                                        o.chunk >= 0 ? o.chunk : default_chunk(p));
         if (rc) return rc;
@@ -523,7 +539,10 @@ static std::string describe(const Plan& p) {
        << " ";
     switch (p.kind) {
         case Kind::Trivial: os << "trivial"; break;
-        case Kind::Direct: os << "direct " << rad(p.line) << " C=" << p.C; break;
+        case Kind::Direct:
+            os << "direct " << rad(p.line) << " C=" << p.C;
+            if (p.pipe_res) os << " pipe";
+            break;
         case Kind::FourStep:
             os << "fourstep N1=" << p.fs.N1 << " " << rad(p.fs.a) << " CA=" << p.fs.CA
                << " N2=" << p.fs.N2 << " " << rad(p.fs.b) << " CB=" << p.fs.CB;
@@ -625,6 +644,14 @@ static int run_fft(const Plan& p, const void* in, void* out, int sgn, double sca
             a.sgn_out = sgn;
             a.scale = scale;
             a.epi = epi;
+            if (p.pipe_res > 0 && !epi.K) {
+                const int64_t groups = (p.batch + p.C - 1) / p.C;
+                a.grid = (int)std::min<int64_t>(groups, p.pipe_res);
+                if (a.grid == 0) return CFFT_OK;
+                CFFT_HIP(launch_rows_pipe(p.prec, in, out, a, p.line.pt, p.line.dtw.p,
+                                          p.line.dtw.bytes / (size_t)esize(p.prec), st));
+                return CFFT_OK;
+            }
             CFFT_HIP(launch_rows(p.prec, in, out, a, p.line.pt, p.line.dtw.p, st));
             return CFFT_OK;
         }

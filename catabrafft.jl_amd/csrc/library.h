@@ -56,6 +56,7 @@ struct Plan {
     // direct and fused convolution: the line FFT and lines per workgroup
     LinePlan line;
     int C = 1;
+    int pipe_res = 0;  // direct: resident workgroups of the pipelined rows kernel (0 = not used)
     // four-step
     FourStepPlan fs;
     mutable std::mutex scratch_mu;
@@ -88,6 +89,7 @@ struct PlanOptions {
     int64_t n1 = 0;            // four-step split (0 = balanced / direct when it fits)
     int ca = 0, cb = 0;        // four-step column tiles (0 = default)
     int64_t chunk = -1;        // four-step transforms per round (-1 = default, 0 = whole batch)
+    int pipe = -1;             // direct: pipelined rows kernel (1 = on; -1 = default = off)
     int force_conv = 0;
     bool explicit_opts = false;
     bool alloc_work = true;    // large convolutions: allocate work buffers for `batch` lines

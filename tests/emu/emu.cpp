@@ -118,10 +118,20 @@ static double now_ms() {
 hipError_t hipGetDeviceCount(int* n) { *n = 1; return hipSuccess; }
 hipError_t hipSetDevice(int) { return hipSuccess; }
 hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t a, int) {
-    *v = a == hipDeviceAttributeMaxSharedMemoryPerBlock ? 160 * 1024 : 256;
+    // 2 CUs: persistent kernels then cover the tests' small batches with a few workgroups that
+    // each loop over several line groups
+    *v = a == hipDeviceAttributeMaxSharedMemoryPerBlock ? 160 * 1024 : 2;
     return hipSuccess;
 }
 hipError_t hipFuncSetAttribute(const void*, hipFuncAttribute, int) { return hipSuccess; }
+// Resident workgroups per CU as the LDS allows (160 KiB per CU), at most 2: small enough that
+// the emulated grids of the tests loop several times in the persistent kernels.
+hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int* n, const void*, int, size_t lds) {
+    const size_t per_cu = 160 * 1024;
+    int v = lds ? (int)(per_cu / lds) : 2;
+    *n = v < 2 ? v : 2;
+    return hipSuccess;
+}
 hipError_t hipGetLastError() { return hipSuccess; }
 const char* hipGetErrorString(hipError_t) { return "emulated error"; }
 hipError_t hipDeviceSynchronize() { return hipSuccess; }

@@ -206,6 +206,42 @@ def test_emulated_wisdom_round_trip(E, tmp_path):
     lib.cfft_finalize(ctx2)
 
 
+@pytest.mark.parametrize("n,batch,dt", [(2520, 11, np.complex64), (2520, 9, np.complex128),
+                                        (64, 150, np.complex64), (12, 500, np.complex128),
+                                        (11 * 13 * 8, 7, np.complex64)])
+def test_emulated_pipelined_rows_match_plain(E, tmp_path, n, batch, dt):
+    """The persistent pipelined rows kernel (its few workgroups each loop over several line
+    groups here) against the plain rows kernel, chosen through wisdom: identical bits, and the
+    oracle, forward and backward, out of place and in place."""
+    lib, ctx = E
+    dcode = 0 if dt == np.complex64 else 1
+    outs = {}
+    for pipe in (0, 1):
+        path = tmp_path / f"w{pipe}.txt"
+        path.write_text(f"{n} {batch} {n} {n} {dcode} 1 : 16 256 0 0 0 -1 {pipe}\n")
+        c2 = ctypes.c_void_p()
+        assert lib.cfft_init(0, ctypes.byref(c2)) == 0
+        assert lib.cfft_wisdom_import(c2, str(path).encode()) == 0
+        p = _plan(lib, c2, n, batch, dt, flags=1)
+        assert ("pipe" in _desc(lib, p)) == bool(pipe), _desc(lib, p)
+        x = seeded_complex(n, batch=batch, seed=5, dtype=dt)
+        res = []
+        for direction, norm in ((-1, 0), (1, 1)):
+            y = np.zeros_like(x)
+            assert lib.cfft_exec(p, x.ctypes.data, y.ctypes.data, direction, norm, None) == 0
+            ref = O.fft(x.astype(np.complex128), "d", direction, bool(norm))
+            assert rel_l2(y, ref) < _tol(n, dt), (pipe, direction)
+            res.append(y)
+        z = x.copy()
+        assert lib.cfft_exec(p, z.ctypes.data, z.ctypes.data, -1, 0, None) == 0
+        assert np.array_equal(z, res[0])
+        outs[pipe] = res
+        lib.cfft_plan_destroy(p)
+        lib.cfft_finalize(c2)
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("n,batch,rt", [(16, 3, np.float32), (2520, 2, np.float64), (1000, 2, np.float32),
                                          (17, 3, np.float32), (1 << 16, 1, np.float32),
                                          (2 * 4099, 1, np.float32), (99, 2, np.float64), (2, 2, np.float32),

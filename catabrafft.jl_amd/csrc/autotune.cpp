@@ -69,15 +69,17 @@ static std::vector<PlanOptions> candidates(Ctx* ctx, const Plan& p, int flags) {
             for (int E : Es)
                 for (int ca : tiles)
                     for (int cb : tiles)
-                        for (int64_t ch : chunks) {
-                            PlanOptions o = base;
-                            o.E = E;
-                            o.n1 = a;
-                            o.ca = ca;
-                            o.cb = cb;
-                            o.chunk = ch;
-                            out.push_back(o);
-                        }
+                        for (int64_t ch : chunks)
+                            for (int pipe : {0, 1}) {
+                                PlanOptions o = base;
+                                o.E = E;
+                                o.n1 = a;
+                                o.ca = ca;
+                                o.cb = cb;
+                                o.chunk = ch;
+                                o.pipe = pipe;
+                                out.push_back(o);
+                            }
     } else {
         for (int E : Es)
             for (int tt : tts) {

@@ -34,12 +34,14 @@ struct FourStepAArgs {
     int N1, N2, C, Tt, Ls, E;
     int64_t batch, idist, odist;
     int sgn_in;
+    int grid;              // pipelined kernel: workgroups launched (each loops over tiles)
 };
 
 // Four-step pass B (lines = the N1 values of k1, elements j2 at stride N1):
 //   out[b*odist + k1 + N1*k2] = DFT_N2(w_n^{j2 k1} in[b*idist + j2*N1 + k1])[k2]
-// with w_n^e = T1[e & mask] * T2[e >> shift] (T1 == nullptr: no twiddle, a plain strided-line
-// pass).  Reads and writes the same index set per tile, so in == out is allowed.
+// with w_n^e = (A[e & m] * B[(e >> tbits) & m]) * C[e >> 2 tbits], the tables back to back in
+// tw3 (tw3 == nullptr: no twiddle, a plain strided-line pass), copied into LDS after the lines.
+// Reads and writes the same index set per tile, so in == out is allowed.
 struct FourStepBArgs {
     int N1, N2, C, Tt, Ls, E;
     int64_t batch, idist, odist;
@@ -47,11 +49,17 @@ struct FourStepBArgs {
     int sgn_out;
     double scale;
     ConvEpilogue epi;
-    int shift;
-    int mask;
-    const void* T1;
-    const void* T2;
+    int tbits, tnc;
+    const void* tw3;
+    int grid;              // pipelined kernel: workgroups launched (each loops over tiles)
 };
+
+// Entries of a three-level twiddle table, and the LDS bytes of a pass-B workgroup: C lines of
+// stride Ls, then the table (16-byte aligned).
+__host__ __device__ constexpr int tw3_entries(int tbits, int tnc) { return (2 << tbits) + tnc; }
+__host__ __device__ constexpr size_t fsb_tw_offset(size_t elem, int C, int Ls) {
+    return (2 * elem * (size_t)C * Ls + 15) & ~(size_t)15;
+}
 
 // Fused small convolution FFT (Rader or Bluestein) entirely in LDS: one launch, one HBM read
 // and one HBM write per transform.
@@ -122,6 +130,17 @@ size_t rows_pipe_lds(Prec p, int C, int Ls, size_t twn);
 int rows_pipe_blocks_per_cu(Prec p, int E, int R0, int block, size_t lds);
 hipError_t launch_fourstep_a(Prec p, const void* in, void* out, const FourStepAArgs& a,
                              const PassTable& pt, const void* tw, hipStream_t st);
+// Persistent pipelined four-step passes (a.grid workgroups of at most kPipeMaxThreads, no
+// epilogue); twn = entries of the line's twiddle table, copied into LDS.  fs_pipe_lds gives the
+// LDS bytes (pass 1 = B: lines, pass twiddles, then the three-level table of t3n entries).
+hipError_t launch_fourstep_a_pipe(Prec p, const void* in, void* out, const FourStepAArgs& a,
+                                  const PassTable& pt, const void* tw, size_t twn,
+                                  hipStream_t st);
+hipError_t launch_fourstep_b_pipe(Prec p, const void* in, void* out, const FourStepBArgs& a,
+                                  const PassTable& pt, const void* tw, size_t twn,
+                                  hipStream_t st);
+size_t fs_pipe_lds(Prec p, int C, int Ls, size_t twn, size_t t3n);
+int fs_pipe_blocks_per_cu(Prec p, int E, int pass, int block, size_t lds);
 hipError_t launch_fourstep_b(Prec p, const void* in, void* out, const FourStepBArgs& a,
                              const PassTable& pt, const void* tw, hipStream_t st);
 hipError_t launch_conv_small(Prec p, const void* in, void* out, const ConvSmallArgs& a,

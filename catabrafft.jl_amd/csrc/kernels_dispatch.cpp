@@ -22,6 +22,28 @@ size_t rows_pipe_lds(Prec p, int C, int Ls, size_t twn) {
     const size_t off = (2 * el * (size_t)C * Ls + 15) & ~(size_t)15;
     return off + twn * 2 * el;
 }
+hipError_t launch_fourstep_a_pipe(Prec p, const void* in, void* out, const FourStepAArgs& a,
+                                  const PassTable& pt, const void* tw, size_t twn,
+                                  hipStream_t st) {
+    return p == Prec::F32 ? fsa_pipe_f32(in, out, a, pt, tw, twn, st)
+                          : fsa_pipe_f64(in, out, a, pt, tw, twn, st);
+}
+hipError_t launch_fourstep_b_pipe(Prec p, const void* in, void* out, const FourStepBArgs& a,
+                                  const PassTable& pt, const void* tw, size_t twn,
+                                  hipStream_t st) {
+    return p == Prec::F32 ? fsb_pipe_f32(in, out, a, pt, tw, twn, st)
+                          : fsb_pipe_f64(in, out, a, pt, tw, twn, st);
+}
+size_t fs_pipe_lds(Prec p, int C, int Ls, size_t twn, size_t t3n) {
+    const size_t v = p == Prec::F32 ? 8 : 16;
+    const size_t tw_off = (v * (size_t)C * Ls + 15) & ~(size_t)15;
+    const size_t t3_off = (tw_off + twn * v + 15) & ~(size_t)15;
+    return t3_off + t3n * v;
+}
+int fs_pipe_blocks_per_cu(Prec p, int E, int pass, int block, size_t lds) {
+    return p == Prec::F32 ? fs_pipe_occ_f32(E, pass, block, lds)
+                          : fs_pipe_occ_f64(E, pass, block, lds);
+}
 int rows_pipe_blocks_per_cu(Prec p, int E, int R0, int block, size_t lds) {
     return p == Prec::F32 ? rows_pipe_occ_f32(E, R0, block, lds)
                           : rows_pipe_occ_f64(E, R0, block, lds);

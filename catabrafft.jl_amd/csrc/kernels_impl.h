@@ -219,45 +219,55 @@ k_fs_a(const V_t<T>* __restrict__ in, V_t<T>* __restrict__ out, FourStepAArgs a,
 // ------------------------------------------------------------------------------------------
 // Four-step pass B body: lines k10 .. k10+C-1, staged load and staged (transposing) store.
 //   out[k1 + N1*k2] = DFT_N2(w_n^{j2 k1} in[j2*N1 + k1])[k2];  in == out allowed (same index
-// set).  w_n^e = T1[e & mask] * T2[e >> shift]; T1 == nullptr means no twiddle.
+// set).  w_n^e = (A[e & m] * B[(e >> tbits) & m]) * C[e >> 2 tbits] from the three-level table
+// tw3 (nullptr: no twiddle), which is copied into LDS after the lines.
 // ------------------------------------------------------------------------------------------
 template <typename T>
-struct TwLoad {  // a staged element with the two factors of its twiddle
-    V_t<T> v, a, b;
+struct TwLoad {  // a staged element and a value that goes with it
+    V_t<T> v, a;
 };
+
+// The four-step twiddle w_n^x from a three-level table in LDS.
+template <typename T>
+__device__ __forceinline__ V_t<T> tw3_get(const V_t<T>* t3, int tbits, int x) {
+    const int m = (1 << tbits) - 1;
+    return cmul(cmul(t3[x & m], t3[(1 << tbits) + ((x >> tbits) & m)]),
+                t3[(2 << tbits) + (x >> (2 * tbits))]);
+}
+
+// Copies n entries of a global table into LDS.
+template <typename T>
+__device__ __forceinline__ void stage_table(const V_t<T>* g, V_t<T>* l, int n) {
+    batched_for<kStageU>(threadIdx.x, blockDim.x, n, [&](int i) { return g[i]; },
+                         [&](int i, V_t<T> v) { l[i] = v; });
+}
 
 template <typename T, int E>
 __device__ __forceinline__ void fs_b_body(unsigned char* smem, const V_t<T>* src, V_t<T>* dst,
                                           int N1, int N2, int C, int Tt, int Ls, int k10,
                                           int sgn_in, int sgn, double scale, const V_t<T>* K,
-                                          V_t<T>* dc, const V_t<T>* T1, const V_t<T>* T2,
-                                          int shift, int mask, const PassTable& pt,
-                                          const V_t<T>* tw) {
+                                          V_t<T>* dc, const V_t<T>* tw3, int tbits, int tnc,
+                                          const PassTable& pt, const V_t<T>* tw) {
     const int cshift = __ffs(C) - 1;
     const int total = C * N2;
     const T sgi = (T)sgn_in;
-    // Staged load of runs of C elements (k1) per middle row j2, with the twiddle factors loaded
-    // in the same batch.  Lines past N1 (a partial last tile) load line N1 - 1 and are zeroed.
+    V_t<T>* t3 = reinterpret_cast<V_t<T>*>(smem + fsb_tw_offset(sizeof(T), C, Ls));
+    if (tw3) stage_table<T>(tw3, t3, tw3_entries(tbits, tnc));
+    __syncthreads();
+    // Staged load of runs of C elements (k1) per middle row j2, times the four-step twiddle.
+    // Lines past N1 (a partial last tile) load line N1 - 1 and are zeroed.
     batched_for<kStageU>(
         threadIdx.x, blockDim.x, total,
         [&](int idx) {
             const int c = idx & (C - 1);
             const int e = idx >> cshift;
             const int k1 = k10 + c < N1 ? k10 + c : N1 - 1;
-            TwLoad<T> r;
-            r.v = src[(int64_t)e * N1 + k1];
-            if (T1) {
-                const int x = e * k1;
-                r.a = T1[x & mask];
-                r.b = T2[x >> shift];
-            }
-            return r;
+            return src[(int64_t)e * N1 + k1];
         },
-        [&](int idx, TwLoad<T> r) {
+        [&](int idx, V_t<T> v) {
             const int c = idx & (C - 1);
             const int e = idx >> cshift;
-            V_t<T> v = r.v;
-            if (T1) v = cmul(v, cmul(r.a, r.b));
+            if (tw3) v = cmul(v, tw3_get<T>(t3, tbits, e * (k10 + c)));
             if (k10 + c >= N1) v = V_t<T>{0, 0};
             v.y *= sgi;
             lds_line<T>(smem, C, Ls, c).put(e, v);
@@ -309,8 +319,8 @@ k_fs_b(const V_t<T>* in, V_t<T>* out, FourStepBArgs a, PassTable pt,
     const int tile = blockIdx.x - (int)(bt * tiles);
     fs_b_body<T, E>(smem, in + bt * a.idist, out + bt * a.odist, a.N1, a.N2, a.C, a.Tt, a.Ls,
                     tile * a.C, a.sgn_in, a.sgn_out, a.scale, (const V_t<T>*)a.epi.K,
-                    a.epi.dc ? (V_t<T>*)a.epi.dc + 2 * bt : nullptr, (const V_t<T>*)a.T1,
-                    (const V_t<T>*)a.T2, a.shift, a.mask, pt, tw);
+                    a.epi.dc ? (V_t<T>*)a.epi.dc + 2 * bt : nullptr, (const V_t<T>*)a.tw3,
+                    a.tbits, a.tnc, pt, tw);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -772,7 +782,7 @@ k_fs_b_multi(MultiArgs ma, const MItem* __restrict__ items, const MPlan* __restr
                           P->N1, P->N2, P->CB, P->lb.Tt, P->lb.Ls, it.count * P->CB, 1, sgn, scale,
                           epi ? (const V_t<T>*)P->K : nullptr,
                           epi ? reinterpret_cast<V_t<T>*>(it.dc) : nullptr,
-                          (const V_t<T>*)P->T1, (const V_t<T>*)P->T2, P->shift, P->mask, P->lb.pt,
+                          (const V_t<T>*)P->tw3, P->tbits, P->tnc, P->lb.pt,
                           (const V_t<T>*)P->lb.tw);
 }
 
@@ -862,7 +872,8 @@ hipError_t fsa_t(const void* in, void* out, const FourStepAArgs& a, const PassTa
 template <typename T, int E>
 hipError_t fsb_t(const void* in, void* out, const FourStepBArgs& a, const PassTable& pt,
                  const void* tw, hipStream_t st) {
-    const size_t lds = 2 * sizeof(T) * (size_t)a.C * a.Ls;
+    const size_t lds = fsb_tw_offset(sizeof(T), a.C, a.Ls) +
+                       (a.tw3 ? (size_t)tw3_entries(a.tbits, a.tnc) * sizeof(V_t<T>) : 0);
     const int64_t blocks = (int64_t)((a.N1 + a.C - 1) / a.C) * a.batch;
     hipLaunchKernelGGL((k_fs_b<T, E>), dim3((unsigned)blocks), dim3(a.C * a.Tt), lds, st,
                        (const V_t<T>*)in, (V_t<T>*)out, a, pt, (const V_t<T>*)tw);

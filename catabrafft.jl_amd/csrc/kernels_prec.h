@@ -34,6 +34,16 @@ hipError_t rows_pipe_f64(const void* in, void* out, const RowsArgs& a, const Pas
                          const void* tw, size_t twn, hipStream_t st);
 int rows_pipe_occ_f32(int E, int R0, int block, size_t lds);
 int rows_pipe_occ_f64(int E, int R0, int block, size_t lds);
+hipError_t fsa_pipe_f32(const void* in, void* out, const FourStepAArgs& a, const PassTable& pt,
+                        const void* tw, size_t twn, hipStream_t st);
+hipError_t fsa_pipe_f64(const void* in, void* out, const FourStepAArgs& a, const PassTable& pt,
+                        const void* tw, size_t twn, hipStream_t st);
+hipError_t fsb_pipe_f32(const void* in, void* out, const FourStepBArgs& a, const PassTable& pt,
+                        const void* tw, size_t twn, hipStream_t st);
+hipError_t fsb_pipe_f64(const void* in, void* out, const FourStepBArgs& a, const PassTable& pt,
+                        const void* tw, size_t twn, hipStream_t st);
+int fs_pipe_occ_f32(int E, int pass, int block, size_t lds);
+int fs_pipe_occ_f64(int E, int pass, int block, size_t lds);
 void init_kernels_pipe_f32();
 void init_kernels_pipe_f64();
 

@@ -266,8 +266,6 @@ static int plan_direct(Ctx* ctx, Plan& p, int E, int target_threads, int pipe = 
     if (per > ctx->lds_max) return fail(CFFT_ERR_UNSUPPORTED, "line does not fit LDS");
     p.C = rows_lines_per_block(p.line, p.prec, p.batch, std::max<size_t>(per, ctx->lds_rows_budget),
                                target_threads);
-    // The pipelined kernel trades occupancy for a prefetch: a tuning candidate (MEASURE /
-    // ENCHANT, wisdom), not the default.
     p.pipe_res = pipe > 0 ? pipe_resident(ctx, p) : 0;
     p.kind = Kind::Direct;
     return CFFT_OK;
@@ -285,9 +283,12 @@ static bool line_fits(Ctx* ctx, int64_t N, Prec pr, int E) {
     return per <= ctx->lds_max;
 }
 
-static int pick_tile(Ctx* ctx, const LinePlan& lp, Prec pr, int want) {
+// Lines per four-step workgroup: a power of two, at most `want`, within 1024 threads and the LDS
+// left after `reserve` bytes (pass B's twiddle table).
+static int pick_tile(Ctx* ctx, const LinePlan& lp, Prec pr, int want, size_t reserve = 0) {
     const size_t per = 2 * (size_t)(pr == Prec::F32 ? 4 : 8) * lp.Ls;
-    int c = pow2_floor(std::max<int64_t>(1, (int64_t)(ctx->lds_max / per)));
+    const size_t room = ctx->lds_max > reserve + 16 ? ctx->lds_max - reserve - 16 : 0;
+    int c = pow2_floor(std::max<int64_t>(1, (int64_t)(room / per)));
     c = std::min(c, pow2_floor(std::max(1, 1024 / lp.Tt)));
     c = std::min(c, want);
     return std::max(c, 1);
@@ -320,26 +321,32 @@ static int plan_fourstep_factors(Ctx* ctx, FourStepPlan& fs, int64_t n, Prec pr,
     if (rc) return rc;
     rc = make_line(fs.b, fs.N2, pr, E, 1024);
     if (rc) return rc;
+    // w_n^e for e < n as (A[e & m] * B[(e >> b) & m]) * C[e >> 2b], b = ceil(log2(n) / 3):
+    // three tables of about n^(1/3) entries, small enough for pass B's LDS
+    int tb = 0;
+    while (((int64_t)1 << (3 * tb)) < n) ++tb;
+    fs.tbits = tb;
+    fs.tnc = (int)(((n - 1) >> (2 * tb)) + 1);
+    const size_t es = (size_t)esize(pr);
+    const size_t table = (size_t)tw3_entries(fs.tbits, fs.tnc) * es;
     const int want = pr == Prec::F32 ? 16 : 8;
     fs.CA = ca > 0 ? ca : pick_tile(ctx, fs.a, pr, want);
-    fs.CB = cb > 0 ? cb : pick_tile(ctx, fs.b, pr, want);
+    fs.CB = cb > 0 ? cb : pick_tile(ctx, fs.b, pr, want, table);
     // a requested tile must be a power of two that fits 1024 threads and the LDS
-    for (auto [C, lp] : {std::make_pair(fs.CA, &fs.a), std::make_pair(fs.CB, &fs.b)}) {
-        const size_t lds = 2 * (size_t)(pr == Prec::F32 ? 4 : 8) * lp->Ls * (size_t)C;
-        if (C < 1 || (C & (C - 1)) || C * lp->Tt > 1024 || lds > ctx->lds_max)
+    for (int pass = 0; pass < 2; ++pass) {
+        const int C = pass ? fs.CB : fs.CA;
+        const LinePlan& lp = pass ? fs.b : fs.a;
+        const size_t lds = pass ? fsb_tw_offset(es / 2, C, lp.Ls) + table
+                                : 2 * (es / 2) * lp.Ls * (size_t)C;
+        if (C < 1 || (C & (C - 1)) || C * lp.Tt > 1024 || lds > ctx->lds_max)
             return fail(CFFT_ERR_UNSUPPORTED, "four-step tile does not fit a workgroup");
     }
-    // w_n^e for e < n as T1[e & mask] * T2[e >> shift]
-    int shift = 0;
-    while (((int64_t)1 << (2 * shift)) < n) ++shift;
-    fs.shift = shift;
-    fs.mask = (1 << shift) - 1;
-    std::vector<cld> t1((size_t)1 << shift), t2((size_t)((n >> shift) + 1));
-    for (int64_t i = 0; i < (int64_t)t1.size(); ++i) t1[i] = omega(i, n);
-    for (int64_t i = 0; i < (int64_t)t2.size(); ++i) t2[i] = omega(i << shift, n);
-    rc = upload(fs.T1, t1, pr);
-    if (rc) return rc;
-    return upload(fs.T2, t2, pr);
+    std::vector<cld> t3;
+    t3.reserve(tw3_entries(fs.tbits, fs.tnc));
+    for (int64_t i = 0; i < ((int64_t)1 << tb); ++i) t3.push_back(omega(i, n));
+    for (int64_t i = 0; i < ((int64_t)1 << tb); ++i) t3.push_back(omega(i << tb, n));
+    for (int64_t i = 0; i < fs.tnc; ++i) t3.push_back(omega(i << (2 * tb), n));
+    return upload(fs.tw3, t3, pr);
 }
 
 static int plan_fft_core(Ctx* ctx, Plan& p, const PlanOptions& o);
@@ -499,11 +506,30 @@ static int plan_conv(Ctx* ctx, Plan& p, const PlanOptions& o) {
 
 // Default four-step chunk: transforms per chunk whose middle stays within ~96 MB of the 256 MB
 // Infinity Cache (the whole batch when it is small already).
+// Transforms per four-step round by default: the whole batch (measured fastest on MI355X for
+// 2^20 x 64 in F32 and F64 once pass B's twiddles come from LDS); smaller rounds, whose middle
+// stays in the Infinity Cache, are tuning candidates.  CFFT_FOURSTEP_CHUNK overrides it.
 static int64_t default_chunk(const Plan& p) {
     if (const char* e = std::getenv("CFFT_FOURSTEP_CHUNK")) return std::max<int64_t>(0, std::atoll(e));
-    const int64_t bytes = p.n * esize(p.prec);
-    const int64_t c = std::max<int64_t>(1, ((int64_t)96 << 20) / std::max<int64_t>(bytes, 1));
-    return c >= p.batch ? 0 : c;
+    (void)p;
+    return 0;
+}
+
+// Resident workgroups of the pipelined four-step passes (0 = a pass cannot use it).
+static void fs_pipe_setup(Ctx* ctx, Plan& p) {
+    FourStepPlan& fs = p.fs;
+    fs.pipe_a = fs.pipe_b = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+        const LinePlan& lp = pass ? fs.b : fs.a;
+        const int C = pass ? fs.CB : fs.CA;
+        if (C * lp.Tt > kPipeMaxThreads || (int64_t)C * lp.N > 16 * (int64_t)C * lp.Tt) continue;
+        const size_t twn = lp.dtw.bytes / (size_t)esize(p.prec);
+        const size_t t3n = pass ? (size_t)tw3_entries(fs.tbits, fs.tnc) : 0;
+        const size_t lds = fs_pipe_lds(p.prec, C, lp.Ls, twn, t3n);
+        if (lds > ctx->lds_max) continue;
+        const int bpc = fs_pipe_blocks_per_cu(p.prec, lp.E, pass, C * lp.Tt, lds);
+        (pass ? fs.pipe_b : fs.pipe_a) = bpc > 0 ? bpc * ctx->num_cu : 0;
+    }
 }
 
 static int plan_fft_core(Ctx* ctx, Plan& p, const PlanOptions& o) {
@@ -513,12 +539,20 @@ static int plan_fft_core(Ctx* ctx, Plan& p, const PlanOptions& o) {
     }
     const int E = o.E ? o.E : 16;
     if (o.force_conv == 0 && is_smooth(p.n)) {
-        if (o.n1 == 0 && line_fits(ctx, p.n, p.prec, E))
-            return plan_direct(ctx, p, E, o.target_threads, o.pipe);
+        // Direct lines default to E = 8 (more threads per line; measured faster than 16 on
+        // MI355X for F32 and F64) when the line fits a workgroup that way, and F32 to the
+        // pipelined rows kernel.
+        if (o.n1 == 0) {
+            const int Ed = o.E ? o.E : (line_fits(ctx, p.n, p.prec, 8) ? 8 : 16);
+            const int pipe = o.pipe >= 0 ? o.pipe : (!o.explicit_opts && p.prec == Prec::F32);
+            if (line_fits(ctx, p.n, p.prec, Ed))
+                return plan_direct(ctx, p, Ed, o.target_threads, pipe);
+        }
         int rc = plan_fourstep_factors(ctx, p.fs, p.n, p.prec, E, o.n1, o.ca, o.cb,
                                        o.chunk >= 0 ? o.chunk : default_chunk(p));
         if (rc) return rc;
         p.kind = Kind::FourStep;
+        if (o.pipe > 0) fs_pipe_setup(ctx, p);
         return CFFT_OK;
     }
     return plan_conv(ctx, p, o);
@@ -547,6 +581,8 @@ static std::string describe(const Plan& p) {
             os << "fourstep N1=" << p.fs.N1 << " " << rad(p.fs.a) << " CA=" << p.fs.CA
                << " N2=" << p.fs.N2 << " " << rad(p.fs.b) << " CB=" << p.fs.CB;
             if (p.fs.chunk) os << " chunk=" << p.fs.chunk;
+            if (p.fs.pipe_a || p.fs.pipe_b)
+                os << " pipe=" << (p.fs.pipe_a ? "A" : "") << (p.fs.pipe_b ? "B" : "");
             break;
         case Kind::ConvSmall:
             os << (p.mode == ConvMode::Rader ? "rader" : "bluestein") << "-fused L=" << p.L << " "
@@ -594,7 +630,14 @@ static int run_fourstep(const Plan& p, const void* in, void* mid, void* out, int
         a.idist = idist;
         a.odist = mdist;
         a.sgn_in = sgn;
-        CFFT_HIP(launch_fourstep_a(p.prec, inb, midb, a, fs.a.pt, fs.a.dtw.p, st));
+        const int64_t es_ = esize(p.prec);
+        if (fs.pipe_a > 0) {
+            a.grid = (int)std::min<int64_t>(fs.pipe_a, (int64_t)((fs.N2 + fs.CA - 1) / fs.CA) * nb);
+            CFFT_HIP(launch_fourstep_a_pipe(p.prec, inb, midb, a, fs.a.pt, fs.a.dtw.p,
+                                            fs.a.dtw.bytes / (size_t)es_, st));
+        } else {
+            CFFT_HIP(launch_fourstep_a(p.prec, inb, midb, a, fs.a.pt, fs.a.dtw.p, st));
+        }
         FourStepBArgs b{};
         b.N1 = fs.N1;
         b.N2 = fs.N2;
@@ -610,11 +653,16 @@ static int run_fourstep(const Plan& p, const void* in, void* mid, void* out, int
         b.scale = scale;
         b.epi = epi;
         if (epi.dc) b.epi.dc = (char*)epi.dc + (size_t)b0 * 2 * es;
-        b.shift = fs.shift;
-        b.mask = fs.mask;
-        b.T1 = fs.T1.p;
-        b.T2 = fs.T2.p;
-        CFFT_HIP(launch_fourstep_b(p.prec, midb, outb, b, fs.b.pt, fs.b.dtw.p, st));
+        b.tbits = fs.tbits;
+        b.tnc = fs.tnc;
+        b.tw3 = fs.tw3.p;
+        if (fs.pipe_b > 0 && !epi.K) {
+            b.grid = (int)std::min<int64_t>(fs.pipe_b, (int64_t)((fs.N1 + fs.CB - 1) / fs.CB) * nb);
+            CFFT_HIP(launch_fourstep_b_pipe(p.prec, midb, outb, b, fs.b.pt, fs.b.dtw.p,
+                                            fs.b.dtw.bytes / (size_t)es_, st));
+        } else {
+            CFFT_HIP(launch_fourstep_b(p.prec, midb, outb, b, fs.b.pt, fs.b.dtw.p, st));
+        }
     }
     return CFFT_OK;
 }
@@ -991,8 +1039,8 @@ static int real_exec(const Plan& p, const void* in, void* out, int direction, in
 }
 
 size_t plan_device_bytes(const Plan& p) {
-    size_t b = p.line.dtw.bytes + p.fs.a.dtw.bytes + p.fs.b.dtw.bytes + p.fs.T1.bytes +
-               p.fs.T2.bytes + p.scratch.bytes + p.perm.bytes + p.iperm.bytes + p.chirp.bytes +
+    size_t b = p.line.dtw.bytes + p.fs.a.dtw.bytes + p.fs.b.dtw.bytes + p.fs.tw3.bytes +
+               p.scratch.bytes + p.perm.bytes + p.iperm.bytes + p.chirp.bytes +
                p.K.bytes + p.dc.bytes + p.w1.bytes + p.w2.bytes;
     if (p.sub) b += plan_device_bytes(*p.sub);
     b += p.rw.bytes + p.rs1.bytes + p.rs2.bytes;

@@ -38,8 +38,11 @@ struct LinePlan {
 struct FourStepPlan {
     int N1 = 0, N2 = 0, CA = 1, CB = 1;
     LinePlan a, b;
-    DevBuf T1, T2;
-    int shift = 0, mask = 0;
+    // w_n^e = (A[e & m] * B[(e >> tbits) & m]) * C[e >> 2 tbits], m = 2^tbits - 1: the three
+    // tables back to back in tw3 (2^tbits, 2^tbits and tnc entries), staged into LDS by pass B
+    DevBuf tw3;
+    int tbits = 0, tnc = 0;
+    int pipe_a = 0, pipe_b = 0;  // resident workgroups of the pipelined passes (0 = not used)
     int64_t chunk = 0;  // transforms per pass-A/pass-B round (0 = the whole batch)
 };
 
@@ -89,7 +92,7 @@ struct PlanOptions {
     int64_t n1 = 0;            // four-step split (0 = balanced / direct when it fits)
     int ca = 0, cb = 0;        // four-step column tiles (0 = default)
     int64_t chunk = -1;        // four-step transforms per round (-1 = default, 0 = whole batch)
-    int pipe = -1;             // direct: pipelined rows kernel (1 = on; -1 = default = off)
+    int pipe = -1;             // direct: pipelined rows kernel (0 = off, 1 = on, -1 = default)
     int force_conv = 0;
     bool explicit_opts = false;
     bool alloc_work = true;    // large convolutions: allocate work buffers for `batch` lines

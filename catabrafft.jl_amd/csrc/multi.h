@@ -24,10 +24,9 @@ struct MPlan {
     int n, C, pad0, pad1;       // transform length, lines per workgroup (rows / fused conv)
     MLine line;                 // rows: the transform; fused conv and conv sub-FFTs: length L
     // four-step pass A (la, N1, column tile CA) and pass B (lb, N2, row tile CB)
-    int N1, N2, CA, CB, shift, mask;
+    int N1, N2, CA, CB, tbits, tnc;
     MLine la, lb;
-    const void* T1;
-    const void* T2;
+    const void* tw3;            // pass B's three-level twiddle table (see FourStepBArgs)
     // convolution (Rader / Bluestein)
     int mode, L, LsX, LsA;
     const int* perm;

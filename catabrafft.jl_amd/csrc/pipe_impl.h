@@ -1,4 +1,4 @@
-// pipe_impl.h -- persistent, software-pipelined rows kernel (gfx950).
+// pipe_impl.h -- persistent, software-pipelined rows and four-step kernels (gfx950).
 //
 // Each workgroup loops over groups of C lines.  The first pass's inputs of the NEXT group are
 // loaded into registers while the current group runs its LDS passes and stores, so HBM reads are
@@ -119,6 +119,161 @@ k_rows_pipe(const V_t<T>* __restrict__ in, V_t<T>* __restrict__ out, RowsArgs a,
     }
 }
 
+// ------------------------------------------------------------------------------------------
+// Pipelined four-step passes.  A workgroup loops over tiles (pass A: C columns of one transform;
+// pass B: C rows of the middle); the next tile's staged elements are loaded into registers while
+// the current tile runs its passes and stores.  Same staged layouts, passes and twiddle values
+// as k_fs_a / k_fs_b, hence the same results.
+// ------------------------------------------------------------------------------------------
+constexpr int kPipeStage = 16;  // staged elements per thread (C * N / threads <= 16)
+
+// LDS offset of pass B's three-level twiddle table: after the lines and the pass twiddles.
+template <typename T>
+__host__ __device__ constexpr size_t pipe_t3_offset(int C, int Ls, int twn) {
+    return (pipe_tw_offset<T>(C, Ls) + (size_t)twn * sizeof(V_t<T>) + 15) & ~(size_t)15;
+}
+
+template <typename T, int E>
+__global__ void __launch_bounds__(kPipeMaxThreads)
+k_fs_a_pipe(const V_t<T>* __restrict__ in, V_t<T>* __restrict__ out, FourStepAArgs a,
+            PassTable pt, const V_t<T>* __restrict__ tw) {
+    CFFT_DYN_LDS(smem);
+    using V = V_t<T>;
+    const int C = a.C, N1 = a.N1, N2 = a.N2, Tt = a.Tt;
+    const int cshift = __ffs(C) - 1;
+    const int total = C * N1;
+    const int tiles = (N2 + C - 1) / C;
+    const int64_t ntiles = (int64_t)tiles * a.batch;
+    V* twl = reinterpret_cast<V*>(smem + pipe_tw_offset<T>(C, a.Ls));
+    V x[kPipeStage];
+    // this thread's staged elements of tile q: idx = tid + u * blockDim (row e, column c)
+    auto load_tile = [&](int64_t q) {
+        const int64_t bt = q / tiles;
+        const int j20 = (int)(q - bt * tiles) * C;
+        const V* src = in + bt * a.idist;
+#pragma unroll
+        for (int u = 0; u < kPipeStage; ++u) {
+            const int idx = threadIdx.x + u * blockDim.x;
+            if (idx < total) {
+                const int c = idx & (C - 1);
+                const int e = idx >> cshift;
+                const int j2 = j20 + c < N2 ? j20 + c : N2 - 1;
+                x[u] = src[(int64_t)e * N2 + j2];
+            }
+        }
+    };
+    int64_t q = blockIdx.x;
+    if (q < ntiles) load_tile(q);
+    stage_twiddles<T>(pt, N1, tw, twl);
+    const int tid = threadIdx.x;
+    const int c = tid / Tt;
+    const int t = tid - c * Tt;
+    const LdsLine<T> L = lds_line<T>(smem, C, a.Ls, c);
+    const TwLds<V> twt{twl};
+    const T sg = (T)a.sgn_in;
+    for (; q < ntiles; q += gridDim.x) {
+        const int64_t bt = q / tiles;
+        const int j20 = (int)(q - bt * tiles) * C;
+#pragma unroll
+        for (int u = 0; u < kPipeStage; ++u) {
+            const int idx = threadIdx.x + u * blockDim.x;
+            if (idx < total) {
+                const int cc = idx & (C - 1);
+                V v = x[u];
+                if (j20 + cc >= N2) v = V{0, 0};
+                v.y *= sg;
+                lds_line<T>(smem, C, a.Ls, cc).put(idx >> cshift, v);
+            }
+        }
+        __syncthreads();
+        if (q + gridDim.x < ntiles) load_tile(q + gridDim.x);
+        const int j2 = j20 + c;
+        const bool active = j2 < N2;
+        GPlainDst<T> d{out + bt * a.odist + (int64_t)(active ? j2 : 0) * N1};
+        run_line_t<T, E>(pt, N1, t, Tt, active, twt, L, d, L, false, true, false, 0, 1);
+        __syncthreads();
+    }
+}
+
+template <typename T, int E>
+__global__ void __launch_bounds__(kPipeMaxThreads)
+k_fs_b_pipe(const V_t<T>* in, V_t<T>* out, FourStepBArgs a, PassTable pt,
+            const V_t<T>* __restrict__ tw, int twn) {
+    CFFT_DYN_LDS(smem);
+    using V = V_t<T>;
+    const int C = a.C, N1 = a.N1, N2 = a.N2, Tt = a.Tt;
+    const int cshift = __ffs(C) - 1;
+    const int total = C * N2;
+    const int tiles = (N1 + C - 1) / C;
+    const int64_t ntiles = (int64_t)tiles * a.batch;
+    V* twl = reinterpret_cast<V*>(smem + pipe_tw_offset<T>(C, a.Ls));
+    V* t3 = reinterpret_cast<V*>(smem + pipe_t3_offset<T>(C, a.Ls, twn));
+    const V* tw3 = (const V*)a.tw3;
+    V x[kPipeStage];
+    // this thread's staged elements of tile q: runs of C lines (k1) per middle row j2 = e
+    auto load_tile = [&](int64_t q) {
+        const int64_t bt = q / tiles;
+        const int k10 = (int)(q - bt * tiles) * C;
+        const V* src = in + bt * a.idist;
+#pragma unroll
+        for (int u = 0; u < kPipeStage; ++u) {
+            const int idx = threadIdx.x + u * blockDim.x;
+            if (idx < total) {
+                const int cc = idx & (C - 1);
+                const int e = idx >> cshift;
+                const int k1 = k10 + cc < N1 ? k10 + cc : N1 - 1;
+                x[u] = src[(int64_t)e * N1 + k1];
+            }
+        }
+    };
+    int64_t q = blockIdx.x;
+    if (q < ntiles) load_tile(q);
+    stage_twiddles<T>(pt, N2, tw, twl);
+    if (tw3) stage_table<T>(tw3, t3, tw3_entries(a.tbits, a.tnc));
+    const int tid = threadIdx.x;
+    const int c = tid / Tt;
+    const int t = tid - c * Tt;
+    const LdsLine<T> L = lds_line<T>(smem, C, a.Ls, c);
+    const TwLds<V> twt{twl};
+    const T sgi = (T)a.sgn_in;
+    const T sx = (T)a.scale, sy = (T)(a.scale * a.sgn_out);
+    __syncthreads();  // the tables are in LDS
+    for (; q < ntiles; q += gridDim.x) {
+        const int64_t bt = q / tiles;
+        const int k10 = (int)(q - bt * tiles) * C;
+#pragma unroll
+        for (int u = 0; u < kPipeStage; ++u) {
+            const int idx = threadIdx.x + u * blockDim.x;
+            if (idx < total) {
+                const int cc = idx & (C - 1);
+                const int e = idx >> cshift;
+                V v = x[u];
+                if (tw3) v = cmul(v, tw3_get<T>(t3, a.tbits, e * (k10 + cc)));
+                if (k10 + cc >= N1) v = V{0, 0};
+                v.y *= sgi;
+                lds_line<T>(smem, C, a.Ls, cc).put(e, v);
+            }
+        }
+        __syncthreads();
+        if (q + gridDim.x < ntiles) load_tile(q + gridDim.x);
+        const bool active = k10 + c < N1;
+        run_line_t<T, E>(pt, N2, t, Tt, active, twt, L, L, L, false, false, false, 0, 1);
+        // run_line_t ended with a barrier: the lines' outputs are in LDS.  Staged store in runs
+        // of C (k1) per output row k2.
+        V* dst = out + bt * a.odist;
+        for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+            const int cc = idx & (C - 1);
+            const int e = idx >> cshift;
+            const int k1 = k10 + cc;
+            if (k1 < N1) {
+                const V v = lds_line<T>(smem, C, a.Ls, cc)(e);
+                dst[(int64_t)k1 + (int64_t)N1 * e] = V{v.x * sx, v.y * sy};
+            }
+        }
+        __syncthreads();
+    }
+}
+
 namespace detail {
 
 template <typename T, int E, int R0>
@@ -178,7 +333,37 @@ int rows_pipe_occ_t(int R0, int block, size_t lds) {
 }
 
 template <typename T, int E>
+hipError_t fsa_pipe_t(const void* in, void* out, const FourStepAArgs& a, const PassTable& pt,
+                      const void* tw, size_t twn, hipStream_t st) {
+    const size_t lds = pipe_tw_offset<T>(a.C, a.Ls) + twn * sizeof(V_t<T>);
+    hipLaunchKernelGGL((k_fs_a_pipe<T, E>), dim3((unsigned)a.grid), dim3(a.C * a.Tt), lds, st,
+                       (const V_t<T>*)in, (V_t<T>*)out, a, pt, (const V_t<T>*)tw);
+    return hipGetLastError();
+}
+
+template <typename T, int E>
+hipError_t fsb_pipe_t(const void* in, void* out, const FourStepBArgs& a, const PassTable& pt,
+                      const void* tw, size_t twn, hipStream_t st) {
+    const size_t lds = pipe_t3_offset<T>(a.C, a.Ls, (int)twn) +
+                       (a.tw3 ? (size_t)tw3_entries(a.tbits, a.tnc) * sizeof(V_t<T>) : 0);
+    hipLaunchKernelGGL((k_fs_b_pipe<T, E>), dim3((unsigned)a.grid), dim3(a.C * a.Tt), lds, st,
+                       (const V_t<T>*)in, (V_t<T>*)out, a, pt, (const V_t<T>*)tw, (int)twn);
+    return hipGetLastError();
+}
+
+template <typename T, int E>
+int fs_pipe_occ_t(int pass, int block, size_t lds) {
+    int nb = 0;
+    const void* k = pass ? reinterpret_cast<const void*>(k_fs_b_pipe<T, E>)
+                         : reinterpret_cast<const void*>(k_fs_a_pipe<T, E>);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, block, lds) != hipSuccess) return 0;
+    return nb;
+}
+
+template <typename T, int E>
 void rows_pipe_init_t() {
+    allow_big_lds(k_fs_a_pipe<T, E>);
+    allow_big_lds(k_fs_b_pipe<T, E>);
     rows_pipe_init_r<T, E, 16>();
     rows_pipe_init_r<T, E, 13>();
     rows_pipe_init_r<T, E, 11>();

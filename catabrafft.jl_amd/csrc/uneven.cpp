@@ -99,12 +99,11 @@ PlanEntry WorkList::add_plan(const Plan& p) {
             m.N2 = p.fs.N2;
             m.CA = p.fs.CA;
             m.CB = p.fs.CB;
-            m.shift = p.fs.shift;
-            m.mask = p.fs.mask;
+            m.tbits = p.fs.tbits;
+            m.tnc = p.fs.tnc;
             m.la = make_mline(p.fs.a);
             m.lb = make_mline(p.fs.b);
-            m.T1 = p.fs.T1.p;
-            m.T2 = p.fs.T2.p;
+            m.tw3 = p.fs.tw3.p;
             break;
         case Kind::ConvSmall:
         case Kind::ConvLarge:
@@ -134,12 +133,11 @@ PlanEntry WorkList::add_plan(const Plan& p) {
             ms.N2 = s.fs.N2;
             ms.CA = s.fs.CA;
             ms.CB = s.fs.CB;
-            ms.shift = s.fs.shift;
-            ms.mask = s.fs.mask;
+            ms.tbits = s.fs.tbits;
+            ms.tnc = s.fs.tnc;
             ms.la = make_mline(s.fs.a);
             ms.lb = make_mline(s.fs.b);
-            ms.T1 = s.fs.T1.p;
-            ms.T2 = s.fs.T2.p;
+            ms.tw3 = s.fs.tw3.p;
         }
         ms.K = p.K.p;
         e.msub = add_mplan(ms);
@@ -226,7 +224,8 @@ void WorkList::add_transforms(const PlanEntry& e, const std::vector<XformRef>& x
             const FourStepPlan& f = p.fs;
             const int ta = (f.N2 + f.CA - 1) / f.CA, tb = (f.N1 + f.CB - 1) / f.CB;
             const size_t lda = 2 * (esz(pr) / 2) * (size_t)f.CA * f.a.Ls;
-            const size_t ldb = 2 * (esz(pr) / 2) * (size_t)f.CB * f.b.Ls;
+            const size_t ldb = fsb_tw_offset(esz(pr) / 2, f.CB, f.b.Ls) +
+                               (size_t)tw3_entries(f.tbits, f.tnc) * esz(pr);
             for (const XformRef& x : xs) {
                 const uint64_t mid = x.scratch ? x.scratch : x.out;
                 const int mid_sel = x.scratch ? kAbs : x.out_sel;
@@ -300,7 +299,8 @@ void WorkList::add_transforms(const PlanEntry& e, const std::vector<XformRef>& x
                 const FourStepPlan& f = s.fs;
                 const int ta = (f.N2 + f.CA - 1) / f.CA, tb = (f.N1 + f.CB - 1) / f.CB;
                 const size_t lda = 2 * (esz(cp) / 2) * (size_t)f.CA * f.a.Ls;
-                const size_t ldb = 2 * (esz(cp) / 2) * (size_t)f.CB * f.b.Ls;
+                const size_t ldb = fsb_tw_offset(esz(cp) / 2, f.CB, f.b.Ls) +
+                                   (size_t)tw3_entries(f.tbits, f.tnc) * esz(cp);
                 for (const XformRef& x : xs) {
                     for (int pass = 0; pass < 2; ++pass) {
                         const uint64_t src = pass == 0 ? x.w1 : x.w2;

@@ -208,11 +208,14 @@ def test_emulated_wisdom_round_trip(E, tmp_path):
 
 @pytest.mark.parametrize("n,batch,dt", [(2520, 11, np.complex64), (2520, 9, np.complex128),
                                         (64, 150, np.complex64), (12, 500, np.complex128),
-                                        (11 * 13 * 8, 7, np.complex64)])
-def test_emulated_pipelined_rows_match_plain(E, tmp_path, n, batch, dt):
-    """The persistent pipelined rows kernel (its few workgroups each loop over several line
-    groups here) against the plain rows kernel, chosen through wisdom: identical bits, and the
-    oracle, forward and backward, out of place and in place."""
+                                        (11 * 13 * 8, 7, np.complex64), (1 << 16, 3, np.complex64),
+                                        (1 << 17, 1, np.complex128), (3 << 15, 2, np.complex64),
+                                        (7 << 14, 1, np.complex128)])
+def test_emulated_pipelined_match_plain(E, tmp_path, n, batch, dt):
+    """The persistent pipelined kernels (rows, and four-step passes A and B; their few
+    workgroups each loop over several line groups or tiles here) against the plain kernels,
+    chosen through wisdom: identical bits, and the oracle, forward and backward, out of place and
+    in place."""
     lib, ctx = E
     dcode = 0 if dt == np.complex64 else 1
     outs = {}

@@ -601,8 +601,14 @@ static std::string describe(const Plan& p) {
 }
 
 // ------------------------------------------------------------------------------- execution
+static int run_fft_s(const Plan& p, const void* in, void* out, int sgn, double scale,
+                     ConvEpilogue epi, hipStream_t st, const ExecShape& sh);
 static int run_fft(const Plan& p, const void* in, void* out, int sgn, double scale,
-                   ConvEpilogue epi, hipStream_t st);
+                   ConvEpilogue epi, hipStream_t st) {
+    ExecShape sh;
+    sh.batch = p.batch;
+    return run_fft_s(p, in, out, sgn, scale, epi, st, sh);
+}
 
 // Runs the two passes over the batch in chunks of fs.chunk transforms (0 = whole batch).  Pass B
 // overwrites the middle in place with the output, so when a chunk's middle fits the Infinity
@@ -610,12 +616,12 @@ static int run_fft(const Plan& p, const void* in, void* out, int sgn, double sca
 // and one write per element.
 static int run_fourstep(const Plan& p, const void* in, void* mid, void* out, int64_t idist,
                         int64_t mdist, int64_t odist, int sgn, double scale, ConvEpilogue epi,
-                        hipStream_t st) {
+                        hipStream_t st, int64_t B) {
     const FourStepPlan& fs = p.fs;
-    const int64_t chunk = fs.chunk > 0 ? std::min<int64_t>(fs.chunk, p.batch) : p.batch;
+    const int64_t chunk = fs.chunk > 0 ? std::min<int64_t>(fs.chunk, B) : B;
     const size_t es = esize(p.prec);
-    for (int64_t b0 = 0; b0 < p.batch; b0 += chunk) {
-        const int64_t nb = std::min<int64_t>(chunk, p.batch - b0);
+    for (int64_t b0 = 0; b0 < B; b0 += chunk) {
+        const int64_t nb = std::min<int64_t>(chunk, B - b0);
         const char* inb = (const char*)in + (size_t)b0 * idist * es;
         char* midb = (char*)mid + (size_t)b0 * mdist * es;
         char* outb = (char*)out + (size_t)b0 * odist * es;
@@ -667,14 +673,15 @@ static int run_fourstep(const Plan& p, const void* in, void* mid, void* out, int
     return CFFT_OK;
 }
 
-static int run_fft(const Plan& p, const void* in, void* out, int sgn, double scale,
-                   ConvEpilogue epi, hipStream_t st) {
+static int run_fft_s(const Plan& p, const void* in, void* out, int sgn, double scale,
+                     ConvEpilogue epi, hipStream_t st, const ExecShape& sh) {
+    const int64_t B = sh.batch;
     switch (p.kind) {
         case Kind::Trivial: {
             const int64_t es = esize(p.prec);
-            if (p.batch == 0) return CFFT_OK;
+            if (B == 0) return CFFT_OK;
             if (in == out && p.idist == p.odist) return CFFT_OK;
-            CFFT_HIP(hipMemcpy2DAsync(out, p.odist * es, in, p.idist * es, es, p.batch,
+            CFFT_HIP(hipMemcpy2DAsync(out, p.odist * es, in, p.idist * es, es, B,
                                       hipMemcpyDeviceToDevice, st));
             return CFFT_OK;
         }
@@ -685,7 +692,7 @@ static int run_fft(const Plan& p, const void* in, void* out, int sgn, double sca
             a.Tt = p.line.Tt;
             a.Ls = p.line.Ls;
             a.E = p.line.E;
-            a.nlines = p.batch;
+            a.nlines = B;
             a.idist = p.idist;
             a.odist = p.odist;
             a.sgn_in = sgn;
@@ -693,7 +700,7 @@ static int run_fft(const Plan& p, const void* in, void* out, int sgn, double sca
             a.scale = scale;
             a.epi = epi;
             if (p.pipe_res > 0 && !epi.K) {
-                const int64_t groups = (p.batch + p.C - 1) / p.C;
+                const int64_t groups = (B + p.C - 1) / p.C;
                 a.grid = (int)std::min<int64_t>(groups, p.pipe_res);
                 if (a.grid == 0) return CFFT_OK;
                 CFFT_HIP(launch_rows_pipe(p.prec, in, out, a, p.line.pt, p.line.dtw.p,
@@ -706,15 +713,15 @@ static int run_fft(const Plan& p, const void* in, void* out, int sgn, double sca
         case Kind::FourStep: {
             if (in == out) {
                 std::lock_guard<std::mutex> g(p.scratch_mu);
-                const size_t need = (size_t)p.batch * p.n * esize(p.prec);
+                const size_t need = (size_t)B * p.n * esize(p.prec);
                 if (p.scratch.bytes < need) {
                     int rc = p.scratch.alloc(need);
                     if (rc) return rc;
                 }
                 return run_fourstep(p, in, p.scratch.p, out, p.idist, p.n, p.odist, sgn, scale,
-                                    epi, st);
+                                    epi, st, B);
             }
-            return run_fourstep(p, in, out, out, p.idist, p.odist, p.odist, sgn, scale, epi, st);
+            return run_fourstep(p, in, out, out, p.idist, p.odist, p.odist, sgn, scale, epi, st, B);
         }
         case Kind::Cols: {  // fs.N1 adjacent lines of length fs.N2 (stride N1), batch blocks
             FourStepBArgs b{};
@@ -724,7 +731,7 @@ static int run_fft(const Plan& p, const void* in, void* out, int sgn, double sca
             b.Tt = p.fs.b.Tt;
             b.Ls = p.fs.b.Ls;
             b.E = p.fs.b.E;
-            b.batch = p.batch;
+            b.batch = B;
             b.idist = p.idist;
             b.odist = p.odist;
             b.sgn_in = sgn;
@@ -754,7 +761,7 @@ static int run_fft(const Plan& p, const void* in, void* out, int sgn, double sca
             a.LsA = p.LsA;
             a.E = p.line.E;
             a.mode = (int)p.mode;
-            a.nlines = p.batch;
+            a.nlines = B;
             a.idist = p.idist;
             a.odist = p.odist;
             a.sgn_in = sgn;
@@ -771,24 +778,29 @@ static int run_fft(const Plan& p, const void* in, void* out, int sgn, double sca
             ConvGatherArgs g{};
             g.n = p.n;
             g.L = p.L;
-            g.nlines = p.batch;
+            g.nlines = B;
             g.idist = p.idist;
             g.wdist = p.L;
             g.mode = (int)p.mode;
             g.sgn_in = sgn;
             g.perm = (const int*)p.perm.p;
             g.chirp = p.chirp.p;
-            g.dc = p.dc.p;
-            CFFT_HIP(launch_conv_gather(p.prec, p.cprec, in, p.w1.p, g, st));
-            ConvEpilogue e{p.K.p, p.dc.p};
-            int rc = run_fft(*p.sub, p.w1.p, p.w2.p, 1, 1.0, e, st);
+            void* w1 = sh.w1 ? sh.w1 : p.w1.p;
+            void* w2 = sh.w1 ? sh.w2 : p.w2.p;
+            void* dc = sh.w1 ? sh.dc : p.dc.p;
+            if (!w1 || !w2 || !dc) return fail(CFFT_ERR_INVALID, "no convolution work buffers");
+            g.dc = dc;
+            CFFT_HIP(launch_conv_gather(p.prec, p.cprec, in, w1, g, st));
+            ExecShape ss;
+            ss.batch = B;
+            int rc = run_fft_s(*p.sub, w1, w2, 1, 1.0, ConvEpilogue{p.K.p, dc}, st, ss);
             if (rc) return rc;
-            rc = run_fft(*p.sub, p.w2.p, p.w1.p, 1, 1.0, ConvEpilogue{nullptr, nullptr}, st);
+            rc = run_fft_s(*p.sub, w2, w1, 1, 1.0, ConvEpilogue{nullptr, nullptr}, st, ss);
             if (rc) return rc;
             ConvScatterArgs s{};
             s.n = p.n;
             s.L = p.L;
-            s.nlines = p.batch;
+            s.nlines = B;
             s.odist = p.odist;
             s.idist = p.idist;
             s.wdist = p.L;
@@ -799,8 +811,8 @@ static int run_fft(const Plan& p, const void* in, void* out, int sgn, double sca
             s.scale = scale;
             s.iperm = (const int*)p.iperm.p;
             s.chirp = p.chirp.p;
-            s.dc = p.dc.p;
-            CFFT_HIP(launch_conv_scatter(p.prec, p.cprec, p.w1.p, out, s, st));
+            s.dc = dc;
+            CFFT_HIP(launch_conv_scatter(p.prec, p.cprec, w1, out, s, st));
             return CFFT_OK;
         }
         default: break;
@@ -810,6 +822,19 @@ static int run_fft(const Plan& p, const void* in, void* out, int sgn, double sca
 
 static int real_exec(const Plan& p, const void* in, void* out, int direction, int normalize,
                      hipStream_t st);
+
+int plan_exec_shaped(const Plan& p, const void* in, void* out, int direction, int normalize,
+                     hipStream_t st, const ExecShape& sh) {
+    if (p.kind == Kind::Uneven || p.real || p.kind == Kind::ND || p.kind == Kind::Cols)
+        return fail(CFFT_ERR_UNSUPPORTED, "shaped execution needs a 1-D complex plan");
+    if (sh.batch <= 0) return CFFT_OK;
+    if (!in || !out) return fail(CFFT_ERR_INVALID, "null buffer");
+    if (in == out && p.kind == Kind::FourStep)
+        return fail(CFFT_ERR_UNSUPPORTED, "shaped four-step execution is out of place");
+    const int sgn = direction > 0 ? -1 : 1;
+    const double scale = normalize ? 1.0 / (double)p.n : 1.0;
+    return run_fft_s(p, in, out, sgn, scale, ConvEpilogue{nullptr, nullptr}, st, sh);
+}
 
 int plan_exec(const Plan& p, const void* in, void* out, int direction, int normalize,
               hipStream_t st) {

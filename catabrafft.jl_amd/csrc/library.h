@@ -85,6 +85,16 @@ struct Plan {
     std::string desc;
 };
 
+// Execution of a 1-D complex plan on `batch` contiguous rows (idist / odist of the plan), other
+// than the batch it was made for; a large convolution then uses the caller's work buffers
+// (w1, w2: batch * L elements of its convolution precision each, dc: 2 per line).
+struct ExecShape {
+    int64_t batch = 0;
+    void* w1 = nullptr;
+    void* w2 = nullptr;
+    void* dc = nullptr;
+};
+
 // Planner knobs; the autotuner enumerates them, NO_FLAG uses the defaults.
 struct PlanOptions {
     int E = 0;                 // per-thread complex capacity (8 or 16), 0 = default
@@ -129,6 +139,8 @@ int plan_create(Ctx* ctx, int64_t n, int64_t batch, int64_t idist, int64_t odist
                 int flags, const PlanOptions& opts, std::shared_ptr<Plan>* out);
 int plan_exec(const Plan& p, const void* in, void* out, int direction, int normalize,
               hipStream_t st);
+int plan_exec_shaped(const Plan& p, const void* in, void* out, int direction, int normalize,
+                     hipStream_t st, const ExecShape& sh);
 int plan_nd_create(Ctx* ctx, int rank, const int64_t* dims, int64_t axes_mask, int dtype,
                    int flags, std::shared_ptr<Plan>* out);
 int plan_real_create(Ctx* ctx, int64_t n, int64_t batch, int64_t rdist, int64_t cdist, int dtype,

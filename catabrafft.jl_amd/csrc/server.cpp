@@ -12,10 +12,14 @@
 // A request that arrives while earlier rounds run is admitted into the next round as soon as
 // memory and a stream slot free up, so its kernels overlap the running ones on the device.
 // Each transform's arithmetic depends only on its own plan, so a result is the same alone and in
-// any mix.
+// any mix.  In the steady state the loop allocates and frees no device memory (hipFree
+// synchronizes the device and would serialize the rounds): each stream slot keeps its work-list
+// tables, pinned staging and completion event, and request buffers come from a pool.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -37,6 +41,53 @@ static double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// Device buffers recycled across requests, in power-of-two size classes of at least 64 KiB.
+struct DevPool {
+    std::multimap<size_t, void*> idle;
+    size_t idle_bytes = 0;
+    DevPool() = default;
+    DevPool(const DevPool&) = delete;
+    DevPool& operator=(const DevPool&) = delete;
+    ~DevPool() { trim(); }
+    static size_t size_class(size_t n) {
+        size_t c = (size_t)64 << 10;
+        while (c < n) c <<= 1;
+        return c;
+    }
+    void* get(size_t n, size_t* cls) {
+        const size_t c = size_class(n);
+        *cls = c;
+        auto it = idle.find(c);
+        if (it != idle.end()) {
+            void* p = it->second;
+            idle.erase(it);
+            idle_bytes -= c;
+            return p;
+        }
+        void* p = nullptr;
+        if (hipMalloc(&p, c) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        return p;
+    }
+    void put(void* p, size_t c) {
+        if (!p) return;
+        idle.emplace(c, p);
+        idle_bytes += c;
+    }
+    void trim() {
+        for (auto& kv : idle) (void)hipFree(kv.second);
+        idle.clear();
+        idle_bytes = 0;
+    }
+};
+
+struct PoolBuf {
+    void* p = nullptr;
+    size_t cls = 0;
+};
+
 struct SrvRequest {
     int64_t id = 0;
     int64_t n = 0, batch = 0;
@@ -47,22 +98,24 @@ struct SrvRequest {
     double t_submit = 0, t_done = 0;
     size_t bytes = 0;
     int status = 1;  // 1 queued / running, 0 done, < 0 error
-    DevBuf din, dout, work;
+    PoolBuf din, dout, work;
     std::string err;
 };
 
 struct SrvRound {
     std::vector<SrvRequest*> reqs;
-    DevBuf dplans, ditems, dtable;
     WorkList wl;
     std::vector<MultiLaunch> launches;
-    hipEvent_t done = nullptr;
     int slot = 0;
 };
 
+// A stream with the resources its rounds reuse (one round at a time per slot).
 struct SrvSlot {
     hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
     bool busy = false;
+    DevBuf dplans, ditems, dtable;
+    HostStage stage;
 };
 
 struct Server {
@@ -76,9 +129,11 @@ struct Server {
     std::map<int64_t, std::unique_ptr<SrvRequest>> reqs;
     std::map<std::pair<int64_t, int>, std::unique_ptr<Plan>> plans;
     std::deque<std::unique_ptr<SrvRound>> inflight;
-    size_t resident = 0;
+    DevPool pool;
+    size_t resident = 0;  // plans + admitted requests (the pool's idle buffers are counted apart)
     int64_t next_id = 1;
     bool stop = false;
+    bool trace = false;  // CFFT_SERVER_TRACE: one line per round on stderr
     // statistics
     std::vector<double> lat;
     double first_submit = -1, last_done = 0, flops = 0;
@@ -129,15 +184,37 @@ static size_t request_bytes(const SrvRequest& r, const Plan& p) {
     return b;
 }
 
-// Builds and enqueues one round on slot `slot`.  Called with the lock held.
-static int srv_launch_round(Server* s, std::vector<SrvRequest*>& admitted, int slot) {
+// Returns a request's pooled buffers and its share of the HBM cap.  Called with the lock held.
+static void srv_release(Server* s, SrvRequest* r) {
+    s->pool.put(r->din.p, r->din.cls);
+    s->pool.put(r->dout.p, r->dout.cls);
+    s->pool.put(r->work.p, r->work.cls);
+    r->din = r->dout = r->work = PoolBuf{};
+    s->resident -= r->bytes;
+    r->bytes = 0;
+}
+
+// Builds and enqueues one round on slot `slot`.  Called with the lock held; the lock is released
+// while the work list is built and launched (that touches only the round, the slot's buffers and
+// the immutable plans, all owned by the server thread).
+static int srv_launch_round(Server* s, std::vector<SrvRequest*>& admitted, int slot,
+                            std::unique_lock<std::mutex>& lk) {
     std::unique_ptr<SrvRound> R(new SrvRound());
     R->slot = slot;
     hipStream_t st = s->slots[slot].stream;
     int rc = CFFT_OK;
-    // group transforms by plan
+    // group transforms by plan; large transforms (four-step, large convolution) run as their
+    // plan's own batched launches instead (their work needs no per-workgroup items)
     std::map<const Plan*, std::vector<XformRef>> groups;
     std::map<const Plan*, PlanEntry> entries;
+    struct Direct {
+        const Plan* p;
+        SrvRequest* r;
+        const void* in;
+        void* out;
+        ExecShape sh;
+    };
+    std::vector<Direct> direct;
     for (SrvRequest* r : admitted) {
         Plan* p = s->plans[std::make_pair(r->n, r->dtype)].get();
         const size_t es = esz_of(r->dtype);
@@ -145,8 +222,10 @@ static int srv_launch_round(Server* s, std::vector<SrvRequest*>& admitted, int s
         const void* din = r->in;
         void* dout = r->out;
         if (!r->on_device) {
-            if ((rc = r->din.alloc(bytes)) || (rc = r->dout.alloc(bytes))) {
-                r->status = rc;
+            r->din.p = s->pool.get(bytes, &r->din.cls);
+            r->dout.p = s->pool.get(bytes, &r->dout.cls);
+            if (!r->din.p || !r->dout.p) {
+                r->status = CFFT_ERR_NOMEM;
                 r->err = "device allocation failed";
                 continue;
             }
@@ -161,11 +240,25 @@ static int srv_launch_round(Server* s, std::vector<SrvRequest*>& admitted, int s
         size_t ces = 0;
         if (p->kind == Kind::ConvLarge) {
             ces = p->cprec == Prec::F32 ? 8 : 16;
-            if ((rc = r->work.alloc((size_t)r->batch * (2 * (size_t)p->L + 2) * ces))) {
-                r->status = rc;
+            r->work.p = s->pool.get((size_t)r->batch * (2 * (size_t)p->L + 2) * ces, &r->work.cls);
+            if (!r->work.p) {
+                r->status = CFFT_ERR_NOMEM;
                 r->err = "device allocation failed";
                 continue;
             }
+        }
+        if (p->kind == Kind::FourStep || p->kind == Kind::ConvLarge) {
+            Direct d{p, r, din, dout, ExecShape{}};
+            d.sh.batch = r->batch;
+            if (p->kind == Kind::ConvLarge) {
+                const size_t wl_bytes = (size_t)r->batch * p->L * ces;
+                d.sh.w1 = r->work.p;
+                d.sh.w2 = (char*)r->work.p + wl_bytes;
+                d.sh.dc = (char*)r->work.p + 2 * wl_bytes;
+            }
+            direct.push_back(d);
+            R->reqs.push_back(r);
+            continue;
         }
         if (!entries.count(p)) entries[p] = R->wl.add_plan(*p);
         for (int64_t b = 0; b < r->batch; ++b) {
@@ -187,23 +280,36 @@ static int srv_launch_round(Server* s, std::vector<SrvRequest*>& admitted, int s
     }
     for (SrvRequest* r : admitted)  // requests that could not be staged leave now
         if (r->status < 0) {
-            s->resident -= r->bytes;
-            r->bytes = 0;
+            srv_release(s, r);
             s->cv_done.notify_all();
         }
+    SrvSlot& sl = s->slots[slot];
+    sl.busy = true;  // claimed while the round is built without the lock
+    lk.unlock();
+    rc = CFFT_OK;
     for (auto& kv : groups) R->wl.add_transforms(entries[kv.first], kv.second);
-    if ((rc = R->wl.finalize(R->dplans, R->ditems, R->dtable, R->launches, st, true))) return rc;
+    rc = R->wl.finalize(sl.dplans, sl.ditems, sl.dtable, R->launches, st, true, &sl.stage);
     MultiArgs ma{0, 0, 1, 0};
-    for (const MultiLaunch& L : R->launches)
-        if (launch_multi(L, ma, st) != hipSuccess) return fail(CFFT_ERR_HIP, "work-list launch failed");
+    for (size_t i = 0; rc == CFFT_OK && i < R->launches.size(); ++i)
+        if (launch_multi(R->launches[i], ma, st) != hipSuccess)
+            rc = fail(CFFT_ERR_HIP, "work-list launch failed");
+    for (size_t i = 0; rc == CFFT_OK && i < direct.size(); ++i) {
+        const Direct& d = direct[i];
+        rc = plan_exec_shaped(*d.p, d.in, d.out, d.r->dir, d.r->norm, st, d.sh);
+    }
     for (SrvRequest* r : R->reqs)
-        if (!r->on_device && r->status > 0)
+        if (rc == CFFT_OK && !r->on_device && r->status > 0)
             (void)hipMemcpyAsync(r->out, r->dout.p, (size_t)r->batch * r->n * esz_of(r->dtype),
                                  hipMemcpyDeviceToHost, st);
-    if (hipEventCreateWithFlags(&R->done, hipEventDisableTiming) != hipSuccess ||
-        hipEventRecord(R->done, st) != hipSuccess)
-        return fail(CFFT_ERR_HIP, "event record failed");
-    s->slots[slot].busy = true;
+    if (rc == CFFT_OK && hipEventRecord(sl.done, st) != hipSuccess)
+        rc = fail(CFFT_ERR_HIP, "event record failed");
+    lk.lock();
+    if (rc != CFFT_OK) {
+        // nothing of this round may still run when its buffers return to the pool
+        (void)hipStreamSynchronize(st);
+        sl.busy = false;
+        return rc;
+    }
     s->inflight.push_back(std::move(R));
     return CFFT_OK;
 }
@@ -211,16 +317,13 @@ static int srv_launch_round(Server* s, std::vector<SrvRequest*>& admitted, int s
 static void srv_retire(Server* s) {
     for (auto it = s->inflight.begin(); it != s->inflight.end();) {
         SrvRound* R = it->get();
-        if (hipEventQuery(R->done) != hipSuccess) {
+        if (hipEventQuery(s->slots[R->slot].done) != hipSuccess) {
             ++it;
             continue;
         }
         const double t = now_s();
         for (SrvRequest* r : R->reqs) {
-            r->din.reset();
-            r->dout.reset();
-            r->work.reset();
-            s->resident -= r->bytes;
+            srv_release(s, r);
             if (r->status > 0) r->status = 0;
             r->t_done = t;
             s->lat.push_back(t - r->t_submit);
@@ -228,7 +331,6 @@ static void srv_retire(Server* s) {
             s->transforms += r->batch;
             s->flops += 5.0 * r->n * std::log2((double)std::max<int64_t>(r->n, 2)) * r->batch;
         }
-        (void)hipEventDestroy(R->done);
         s->slots[R->slot].busy = false;
         it = s->inflight.erase(it);
         s->cv_done.notify_all();
@@ -272,7 +374,9 @@ static void srv_loop(Server* s) {
                     s->cv_done.notify_all();
                     continue;
                 }
-                if (s->resident + need > s->cap) break;  // wait for memory to free up
+                if (s->resident + s->pool.idle_bytes + need > s->cap && s->pool.idle_bytes)
+                    s->pool.trim();  // give idle pooled buffers back first
+                if (s->resident + s->pool.idle_bytes + need > s->cap) break;  // wait for memory
                 r->bytes = need;
                 s->resident += need;
                 round_bytes += need;
@@ -282,13 +386,22 @@ static void srv_loop(Server* s) {
             }
         }
         if (!admitted.empty()) {
-            const int rc = srv_launch_round(s, admitted, slot);
+            const double t0 = now_s();
+            const int rc = srv_launch_round(s, admitted, slot, lk);
+            if (s->trace) {
+                int64_t nt = 0;
+                for (SrvRequest* r : admitted) nt += r->batch;
+                const SrvRound* R = s->inflight.empty() ? nullptr : s->inflight.back().get();
+                std::fprintf(stderr, "round slot=%d requests=%zu transforms=%lld launches=%zu "
+                             "build_ms=%.3f queued=%zu inflight=%zu\n", slot, admitted.size(),
+                             (long long)nt, R ? R->launches.size() : (size_t)0,
+                             (now_s() - t0) * 1e3, s->queue.size(), s->inflight.size());
+            }
             if (rc) {
                 for (SrvRequest* r : admitted) {
                     if (r->status > 0) r->status = rc;
                     r->err = cfft_last_error();
-                    s->resident -= r->bytes;
-                    r->bytes = 0;
+                    srv_release(s, r);
                 }
                 s->cv_done.notify_all();
             }
@@ -319,9 +432,15 @@ int cfft_server_create(cfft_ctx ctx, int64_t hbm_cap_bytes, int nstreams, cfft_s
     Server* s = new Server();
     s->ctx = ctx->c;
     s->cap = (size_t)hbm_cap_bytes;
-    s->slots.resize(nstreams);
+    s->trace = std::getenv("CFFT_SERVER_TRACE") != nullptr;
+    s->slots = std::vector<SrvSlot>(nstreams);
     for (auto& sl : s->slots)
-        if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess) {
+        if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess) {
+            for (auto& x : s->slots) {
+                if (x.stream) (void)hipStreamDestroy(x.stream);
+                if (x.done) (void)hipEventDestroy(x.done);
+            }
             delete s;
             return fail(CFFT_ERR_HIP, "stream creation failed");
         }
@@ -429,8 +548,10 @@ int cfft_server_destroy(cfft_server srv) {
     s->cv_work.notify_all();
     if (s->worker.joinable()) s->worker.join();
     (void)hipSetDevice(s->ctx->device);
-    for (auto& sl : s->slots)
+    for (auto& sl : s->slots) {
         if (sl.stream) (void)hipStreamDestroy(sl.stream);
+        if (sl.done) (void)hipEventDestroy(sl.done);
+    }
     delete s;
     delete srv;
     return CFFT_OK;

@@ -64,7 +64,15 @@ void WorkList::add_item(int phase, MultiKernel k, Prec prec, Prec cprec, int blo
     const int bc = (k == MultiKernel::Gather || k == MultiKernel::Scatter) ? 256 : pow2_ceil(block);
     const size_t lc = lds_class(lds);
     const Key key{phase, (int)k, (int)prec, (int)cprec, bc, lc};
-    Group& g = groups_[key];
+    Group* gp = nullptr;
+    for (auto& r : recent_)
+        if (r.second && r.first == key) gp = r.second;
+    if (!gp) {
+        gp = &groups_[key];
+        recent_[recent_next_] = {key, gp};
+        recent_next_ = (recent_next_ + 1) & 3;
+    }
+    Group& g = *gp;
     if (g.items.empty()) {
         g.phase = phase;
         g.kernel = (int)k;
@@ -74,8 +82,8 @@ void WorkList::add_item(int phase, MultiKernel k, Prec prec, Prec cprec, int blo
         g.lds = 0;
     }
     g.lds = std::max(g.lds, lds);
+    if (g.runs.empty() || g.runs.back().first != cost) g.runs.emplace_back(cost, g.items.size());
     g.items.push_back(it);
-    g.cost.push_back(cost);
 }
 
 size_t WorkList::item_count() const {
@@ -341,7 +349,8 @@ void WorkList::add_transforms(const PlanEntry& e, const std::vector<XformRef>& x
 }
 
 int WorkList::finalize(DevBuf& dplans, DevBuf& ditems, DevBuf& dtable,
-                       std::vector<MultiLaunch>& out, hipStream_t st, bool async) {
+                       std::vector<MultiLaunch>& out, hipStream_t st, bool async,
+                       HostStage* stage) {
     int rc;
     out.clear();
     // order: phase, then kernel; inside a group, most expensive items first
@@ -355,16 +364,26 @@ int WorkList::finalize(DevBuf& dplans, DevBuf& ditems, DevBuf& dtable,
     const size_t bp = mplans_.size() * sizeof(MPlan);
     const size_t bt = std::max<size_t>(1, table_.size()) * sizeof(uint64_t);
     const size_t bi = std::max<size_t>(1, nitems) * sizeof(MItem);
-    if (dplans.bytes < bp && (rc = dplans.alloc(bp))) return rc;
-    if (dtable.bytes < bt && (rc = dtable.alloc(bt))) return rc;
-    if (ditems.bytes < bi && (rc = ditems.alloc(bi))) return rc;
+    // grow with headroom, so that buffers reused round after round are rarely reallocated
+    if (dplans.bytes < bp && (rc = dplans.alloc(bp + bp / 2))) return rc;
+    if (dtable.bytes < bt && (rc = dtable.alloc(bt + bt / 2))) return rc;
+    if (ditems.bytes < bi && (rc = ditems.alloc(bi + bi / 2))) return rc;
     std::vector<MItem> all;
     all.reserve(nitems);
     for (const Group* g : gs) {
-        std::vector<size_t> order(g->items.size());
-        std::iota(order.begin(), order.end(), 0);
-        std::stable_sort(order.begin(), order.end(),
-                         [g](size_t a, size_t b) { return g->cost[a] > g->cost[b]; });
+        // most expensive items first, equal costs in insertion order: a stable sort of the runs
+        // (run k ends where run k + 1 starts)
+        std::vector<size_t> ri(g->runs.size());
+        std::iota(ri.begin(), ri.end(), 0);
+        std::stable_sort(ri.begin(), ri.end(), [g](size_t a, size_t b) {
+            return g->runs[a].first > g->runs[b].first;
+        });
+        std::vector<size_t> order;
+        order.reserve(g->items.size());
+        for (size_t k : ri) {
+            const size_t end = k + 1 < g->runs.size() ? g->runs[k + 1].second : g->items.size();
+            for (size_t i = g->runs[k].second; i < end; ++i) order.push_back(i);
+        }
         MultiLaunch L{};
         L.kernel = g->kernel;
         L.prec = g->prec;
@@ -382,13 +401,20 @@ int WorkList::finalize(DevBuf& dplans, DevBuf& ditems, DevBuf& dtable,
         out.push_back(L);
     }
     if (async) {
-        host_.resize(bp + bt + bi);
-        std::memcpy(host_.data(), mplans_.data(), bp);
-        if (!table_.empty()) std::memcpy(host_.data() + bp, table_.data(), table_.size() * 8);
-        if (!all.empty()) std::memcpy(host_.data() + bp + bt, all.data(), all.size() * sizeof(MItem));
-        CFFT_HIP(hipMemcpyAsync(dplans.p, host_.data(), bp, hipMemcpyHostToDevice, st));
-        CFFT_HIP(hipMemcpyAsync(dtable.p, host_.data() + bp, bt, hipMemcpyHostToDevice, st));
-        CFFT_HIP(hipMemcpyAsync(ditems.p, host_.data() + bp + bt, bi, hipMemcpyHostToDevice, st));
+        unsigned char* h;
+        if (stage) {
+            if (stage->reserve(bp + bt + bi)) return fail(CFFT_ERR_NOMEM, "pinned staging failed");
+            h = static_cast<unsigned char*>(stage->p);
+        } else {
+            host_.resize(bp + bt + bi);
+            h = host_.data();
+        }
+        std::memcpy(h, mplans_.data(), bp);
+        if (!table_.empty()) std::memcpy(h + bp, table_.data(), table_.size() * 8);
+        if (!all.empty()) std::memcpy(h + bp + bt, all.data(), all.size() * sizeof(MItem));
+        CFFT_HIP(hipMemcpyAsync(dplans.p, h, bp, hipMemcpyHostToDevice, st));
+        CFFT_HIP(hipMemcpyAsync(dtable.p, h + bp, bt, hipMemcpyHostToDevice, st));
+        CFFT_HIP(hipMemcpyAsync(ditems.p, h + bp + bt, bi, hipMemcpyHostToDevice, st));
         return CFFT_OK;
     }
     if (bp) CFFT_HIP(hipMemcpy(dplans.p, mplans_.data(), bp, hipMemcpyHostToDevice));

@@ -18,6 +18,32 @@ struct PlanEntry {
 };
 
 // Where one transform of a plan entry reads and writes.
+// Pinned host staging buffer for asynchronous table uploads (grown on demand).
+struct HostStage {
+    void* p = nullptr;
+    size_t cap = 0;
+    HostStage() = default;
+    HostStage(const HostStage&) = delete;
+    HostStage& operator=(const HostStage&) = delete;
+    ~HostStage() {
+        if (p) (void)hipHostFree(p);
+    }
+    int reserve(size_t n) {
+        if (n <= cap) return 0;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        const size_t want = n + n / 2;
+        if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            return -1;
+        }
+        cap = want;
+        return 0;
+    }
+};
+
 struct XformRef {
     uint64_t in = 0, out = 0;  // addresses or offsets, see in_sel / out_sel
     int in_sel = kAbs, out_sel = kAbs;
@@ -37,8 +63,11 @@ class WorkList {
     // Adds every item of the given transforms of one plan entry.
     void add_transforms(const PlanEntry& e, const std::vector<XformRef>& xs);
     // Uploads descriptors, line tables and items; returns the launches in execution order.
+    // Uploads the tables (growing the device buffers when too small) and fills `out`.  With
+    // async, the uploads are enqueued on `st` from `stage` (pinned host memory that must stay
+    // untouched until they complete) or, without one, from this work list's own host buffer.
     int finalize(DevBuf& dplans, DevBuf& ditems, DevBuf& dtable, std::vector<MultiLaunch>& out,
-                 hipStream_t st = nullptr, bool async = false);
+                 hipStream_t st = nullptr, bool async = false, HostStage* stage = nullptr);
     size_t item_count() const;
 
    private:
@@ -47,11 +76,16 @@ class WorkList {
         int block;
         size_t lds;
         std::vector<MItem> items;
-        std::vector<double> cost;
+        // runs of consecutive items of equal cost: (cost, first item); a launch orders its
+        // items by decreasing cost, equal costs in insertion order
+        std::vector<std::pair<double, size_t>> runs;
     };
     using Key = std::tuple<int, int, int, int, int, size_t>;
     std::vector<MPlan> mplans_;
     std::map<Key, Group> groups_;
+    // the groups found by the latest lookups (std::map nodes do not move)
+    std::pair<Key, Group*> recent_[4];
+    int recent_next_ = 0;
     std::vector<uint64_t> table_;
     std::vector<unsigned char> host_;  // staging for async uploads (kept alive by the caller)
 

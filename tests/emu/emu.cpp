@@ -132,6 +132,14 @@ hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int* n, const void*, int
     *n = v < 2 ? v : 2;
     return hipSuccess;
 }
+hipError_t hipHostMalloc(void** p, size_t n, unsigned) {
+    *p = std::malloc(n ? n : 1);
+    return *p ? hipSuccess : hipErrorOutOfMemory;
+}
+hipError_t hipHostFree(void* p) {
+    std::free(p);
+    return hipSuccess;
+}
 hipError_t hipGetLastError() { return hipSuccess; }
 const char* hipGetErrorString(hipError_t) { return "emulated error"; }
 hipError_t hipDeviceSynchronize() { return hipSuccess; }

@@ -87,6 +87,9 @@ hipError_t hipSetDevice(int d);
 hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t a, int d);
 hipError_t hipFuncSetAttribute(const void* f, hipFuncAttribute a, int v);
 hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int* n, const void* f, int block, size_t lds);
+#define hipHostMallocDefault 0
+hipError_t hipHostMalloc(void** p, size_t n, unsigned flags);
+hipError_t hipHostFree(void* p);
 hipError_t hipGetLastError();
 const char* hipGetErrorString(hipError_t e);
 hipError_t hipDeviceSynchronize();

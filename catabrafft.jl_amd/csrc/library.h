@@ -156,7 +156,7 @@ struct ConvData {
 };
 int conv_data(int64_t n, ConvData& d);
 int conv_spectrum(Ctx* ctx, const std::vector<std::complex<long double>>& b, Prec cprec, DevBuf& K);
-Prec conv_precision(int64_t n, Prec io);
+Prec conv_precision(int64_t n, Prec io, ConvMode mode);
 int upload_cld(DevBuf& buf, const std::vector<std::complex<long double>>& v, Prec pr);
 int upload_ints(DevBuf& buf, const std::vector<int>& v);
 bool line_fits_lds(Ctx* ctx, int64_t N, Prec pr, int E);

@@ -63,6 +63,10 @@ struct GPlainDst {
 // then covers U loads; a plain loop would wait once per element, because each element's store
 // may alias the next element's load.
 constexpr int kStageU = 8;
+// Staged loads of plain data issue 16 per thread per batch: a workgroup then has up to 16 KiB
+// (F32) or 32 KiB (F64) per 256 threads in flight, which the memory system needs to cover its
+// latency when only one or two workgroups fit a CU.
+constexpr int kStageLoadU = 16;
 
 template <int U, class LD, class ST>
 __device__ __forceinline__ void batched_for(int tid, int step, int total, const LD& ld,
@@ -178,7 +182,7 @@ __device__ __forceinline__ void fs_a_body(unsigned char* smem, const V_t<T>* src
     const T sg = (T)sgn;
     // Staged load: rows j1 of C contiguous columns (one run of C elements per row); columns past
     // N2 (a partial last tile) load column N2 - 1 and are zeroed.
-    batched_for<kStageU>(
+    batched_for<kStageLoadU>(
         threadIdx.x, blockDim.x, C * N1,
         [&](int idx) {
             const int c = idx & (C - 1);
@@ -256,7 +260,7 @@ __device__ __forceinline__ void fs_b_body(unsigned char* smem, const V_t<T>* src
     __syncthreads();
     // Staged load of runs of C elements (k1) per middle row j2, times the four-step twiddle.
     // Lines past N1 (a partial last tile) load line N1 - 1 and are zeroed.
-    batched_for<kStageU>(
+    batched_for<kStageLoadU>(
         threadIdx.x, blockDim.x, total,
         [&](int idx) {
             const int c = idx & (C - 1);
@@ -387,7 +391,7 @@ __device__ __forceinline__ void conv_body(unsigned char* smem, const LINES& line
     V_t<T>* sdc = reinterpret_cast<V_t<T>*>(aim + C * LsA);
     const T sg = (T)sgn;
     // 1. staged, coalesced load of the nl input lines (flat over lines x elements)
-    batched_for<kStageU>(
+    batched_for<kStageLoadU>(
         threadIdx.x, blockDim.x, nl * n,
         [&](int idx) {
             const int c = idx / n;

@@ -57,6 +57,29 @@ struct GPlainDst {
     __device__ __forceinline__ void put(int e, V_t<T> v) const { p[e] = v; }
 };
 
+// Global-memory line with elements `stride` apart (a column of a row-major matrix); sg = -1
+// conjugates on input, (sx, sy) scale and conjugate on output.
+template <typename T>
+struct GStrideSrc {
+    const V_t<T>* p;
+    int64_t stride;
+    T sg;
+    __device__ __forceinline__ V_t<T> operator()(int e) const {
+        V_t<T> v = p[(int64_t)e * stride];
+        v.y *= sg;
+        return v;
+    }
+};
+template <typename T>
+struct GStrideDst {
+    V_t<T>* p;
+    int64_t stride;
+    T sx, sy;
+    __device__ __forceinline__ void put(int e, V_t<T> v) const {
+        p[(int64_t)e * stride] = V_t<T>{v.x * sx, v.y * sy};
+    }
+};
+
 // Staged copy with batched memory latency: for idx = tid, tid + step, ... < total, every thread
 // first runs U loads LD(idx) (indices past the end are clamped to total - 1, so every load is
 // valid and none is predicated), then the U stores ST(idx, value) of the valid indices.  One wait
@@ -174,38 +197,24 @@ k_rows(const V_t<T>* __restrict__ in, V_t<T>* __restrict__ out, RowsArgs a, Pass
 //   out[j2*N1 + k1] = DFT_N1(in[j1*N2 + j2])[k1]
 // The twiddle w_n^{j2 k1} is applied by pass B while it loads the middle.
 // ------------------------------------------------------------------------------------------
+// Threads are mapped column first (lane = c + C t): a wave's accesses to a row of the matrix
+// cover C adjacent columns, so the first pass reads global memory directly in runs of C
+// elements, with no staging through LDS.
 template <typename T, int E>
 __device__ __forceinline__ void fs_a_body(unsigned char* smem, const V_t<T>* src, V_t<T>* dst,
                                           int N1, int N2, int C, int Tt, int Ls, int j20,
                                           int sgn, const PassTable& pt, const V_t<T>* tw) {
     const int cshift = __ffs(C) - 1;
-    const T sg = (T)sgn;
-    // Staged load: rows j1 of C contiguous columns (one run of C elements per row); columns past
-    // N2 (a partial last tile) load column N2 - 1 and are zeroed.
-    batched_for<kStageLoadU>(
-        threadIdx.x, blockDim.x, C * N1,
-        [&](int idx) {
-            const int c = idx & (C - 1);
-            const int e = idx >> cshift;
-            const int j2 = j20 + c < N2 ? j20 + c : N2 - 1;
-            return src[(int64_t)e * N2 + j2];
-        },
-        [&](int idx, V_t<T> v) {
-            const int c = idx & (C - 1);
-            const int e = idx >> cshift;
-            if (j20 + c >= N2) v = V_t<T>{0, 0};
-            v.y *= sg;
-            lds_line<T>(smem, C, Ls, c).put(e, v);
-        });
-    __syncthreads();
     const int tid = threadIdx.x;
-    const int c = tid / Tt;
-    const int t = tid - c * Tt;
+    const int c = tid & (C - 1);
+    const int t = tid >> cshift;
     const int j2 = j20 + c;
-    const bool active = (c < C) && (j2 < N2);
-    const LdsLine<T> L = lds_line<T>(smem, C, Ls, c < C ? c : 0);
-    GPlainDst<T> d{dst + (int64_t)(active ? j2 : 0) * N1};
-    run_line<T, E>(pt, N1, t, Tt, active, tw, L, d, L, false, true);
+    const bool active = j2 < N2;
+    const int j2c = active ? j2 : 0;
+    const LdsLine<T> L = lds_line<T>(smem, C, Ls, c);
+    GStrideSrc<T> gs{src + j2c, N2, (T)sgn};
+    GPlainDst<T> d{dst + (int64_t)j2c * N1};
+    run_line<T, E>(pt, N1, t, Tt, active, tw, gs, d, L, true, true);
 }
 
 template <typename T, int E>
@@ -277,14 +286,22 @@ __device__ __forceinline__ void fs_b_body(unsigned char* smem, const V_t<T>* src
             lds_line<T>(smem, C, Ls, c).put(e, v);
         });
     __syncthreads();
+    // threads column first (lane = c + C t), as in pass A
     const int tid = threadIdx.x;
-    const int c = tid / Tt;
-    const int t = tid - c * Tt;
-    const bool active = (c < C) && (k10 + c < N1);
-    const LdsLine<T> L = lds_line<T>(smem, C, Ls, c < C ? c : 0);
+    const int c = tid & (C - 1);
+    const int t = tid >> cshift;
+    const bool active = k10 + c < N1;
+    const LdsLine<T> L = lds_line<T>(smem, C, Ls, c);
+    const T sx = (T)scale, sy = (T)(scale * sgn);
+    if (!K) {
+        // the last pass writes out[k1 + N1 k2] directly, in runs of C (k1) per row k2; the
+        // tile's reads all happened in the staged load, so in == out is safe
+        GStrideDst<T> d{dst + (active ? k10 + c : 0), N1, sx, sy};
+        run_line<T, E>(pt, N2, t, Tt, active, tw, L, d, L, false, true);
+        return;
+    }
     run_line<T, E>(pt, N2, t, Tt, active, tw, L, L, L, false, false);
     // run_line ended with a barrier: the lines' outputs are in LDS.
-    const T sx = (T)scale, sy = (T)(scale * sgn);
     // Staged store in runs of C (k1) per output row k2; the epilogue spectrum is loaded in the
     // same batch.  Indices of lines past N1 are skipped.
     batched_for<kStageU>(

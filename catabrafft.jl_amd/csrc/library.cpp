@@ -285,6 +285,70 @@ static bool line_fits(Ctx* ctx, int64_t N, Prec pr, int E) {
 
 // Lines per four-step workgroup: a power of two, at most `want`, within 1024 threads and the LDS
 // left after `reserve` bytes (pass B's twiddle table).
+// LDS line stride of a four-step workgroup of C lines, for its column-first thread mapping
+// (lane = c + C t): of the strides >= the line's minimum, the one whose modelled bank conflicts
+// (one wave of each pass: its first read and first two writes, 64 and 32 banks of 4 bytes) are
+// lowest, the smallest on ties.  Results are cached per (precision, N, C).
+static int bank_cost(std::vector<int>& words, int banks) {
+    // words: the 4-byte LDS words one wave touches; cost = most distinct words on one bank
+    std::sort(words.begin(), words.end());
+    int cnt[64] = {0}, worst = 0;
+    for (size_t i = 0; i < words.size(); ++i)
+        if (i == 0 || words[i] != words[i - 1]) worst = std::max(worst, ++cnt[words[i] % banks]);
+    return worst;
+}
+
+static int fs_line_stride(Ctx* ctx, Prec pr, const LinePlan& lp, int C, size_t reserve) {
+    static std::mutex mu;
+    static std::map<std::tuple<int, int, int, int, size_t>, int> cache;
+    const auto key = std::make_tuple((int)pr, lp.N, C, lp.E, ctx->lds_max - std::min(ctx->lds_max, reserve));
+    {
+        std::lock_guard<std::mutex> g(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) return it->second;
+    }
+    const int ew = pr == Prec::F32 ? 1 : 2;
+    auto lidx = [pr](int e) { return pr == Prec::F32 ? lds_index<float>(e) : lds_index<double>(e); };
+    const int N = lp.N, Tt = lp.Tt;
+    const int base = lidx(N - 1) + 1;
+    int best = base;
+    long best_cost = -1;
+    std::vector<int> acc;
+    for (int Ls = base; Ls < base + 64; ++Ls) {
+        if (fsb_tw_offset(ew * 4, C, Ls) + reserve > ctx->lds_max) break;
+        long cost = 0;
+        int s = 1;
+        for (int ip = 0; ip < lp.pt.npass; ++ip) {
+            const int R = lp.pt.radix[ip], nb = N / R;
+            for (int pat = 0; pat < 3; ++pat) {  // read j = 0, writes k = 0, 1
+                acc.clear();
+                for (int lane = 0; lane < 64 && lane < C * Tt; ++lane) {
+                    const int c = lane % C, b = lane / C;
+                    if (b >= nb) continue;
+                    int e = b;
+                    if (pat) {
+                        const int p = b / s, q = b - p * s;
+                        e = q + s * R * p + s * (pat - 1);
+                    }
+                    for (int w = 0; w < ew; ++w) acc.push_back((c * Ls + lidx(e)) * ew + w);
+                }
+                if (!acc.empty()) {
+                    cost += bank_cost(acc, 64);
+                    cost += bank_cost(acc, 32);
+                }
+            }
+            s *= R;
+        }
+        if (best_cost < 0 || cost < best_cost) {
+            best_cost = cost;
+            best = Ls;
+        }
+    }
+    std::lock_guard<std::mutex> g(mu);
+    cache[key] = best;
+    return best;
+}
+
 static int pick_tile(Ctx* ctx, const LinePlan& lp, Prec pr, int want, size_t reserve = 0) {
     const size_t per = 2 * (size_t)(pr == Prec::F32 ? 4 : 8) * lp.Ls;
     const size_t room = ctx->lds_max > reserve + 16 ? ctx->lds_max - reserve - 16 : 0;
@@ -332,6 +396,8 @@ static int plan_fourstep_factors(Ctx* ctx, FourStepPlan& fs, int64_t n, Prec pr,
     const int want = pr == Prec::F32 ? 16 : 8;
     fs.CA = ca > 0 ? ca : pick_tile(ctx, fs.a, pr, want);
     fs.CB = cb > 0 ? cb : pick_tile(ctx, fs.b, pr, want, table);
+    fs.a.Ls = fs_line_stride(ctx, pr, fs.a, fs.CA, 16);
+    fs.b.Ls = fs_line_stride(ctx, pr, fs.b, fs.CB, table + 16);
     // a requested tile must be a power of two that fits 1024 threads and the LDS
     for (int pass = 0; pass < 2; ++pass) {
         const int C = pass ? fs.CB : fs.CA;
@@ -914,6 +980,7 @@ int plan_nd_create(Ctx* ctx, int rank, const int64_t* dims, int64_t axes_mask, i
             s->fs.N2 = (int)N2;
             if ((rc = make_line(s->fs.b, N2, p->prec, 16, 1024))) return rc;
             s->fs.CB = pick_tile(ctx, s->fs.b, p->prec, p->prec == Prec::F32 ? 16 : 8);
+            s->fs.b.Ls = fs_line_stride(ctx, p->prec, s->fs.b, s->fs.CB, 16);
             os << " | axis " << a << ": strided lines N=" << N2 << " x " << inner << " C="
                << s->fs.CB;
         }

@@ -35,6 +35,7 @@ struct FourStepAArgs {
     int64_t batch, idist, odist;
     int sgn_in;
     int grid;              // pipelined kernel: workgroups launched (each loops over tiles)
+    int xcd;               // nonzero: consecutive tiles on one XCD
 };
 
 // Four-step pass B (lines = the N1 values of k1, elements j2 at stride N1):
@@ -52,6 +53,7 @@ struct FourStepBArgs {
     int tbits, tnc;
     const void* tw3;
     int grid;              // pipelined kernel: workgroups launched (each loops over tiles)
+    int xcd;               // nonzero: consecutive tiles on one XCD
 };
 
 // Entries of a three-level twiddle table, and the LDS bytes of a pass-B workgroup: C lines of

@@ -217,14 +217,23 @@ __device__ __forceinline__ void fs_a_body(unsigned char* smem, const V_t<T>* src
     run_line<T, E>(pt, N1, t, Tt, active, tw, gs, d, L, true, true);
 }
 
+// Workgroups are dispatched to the eight XCDs round robin (block b on XCD b mod 8).  With
+// remap, XCD x takes the consecutive block range [x G/8, (x+1) G/8) of a grid of G blocks.
+__device__ __forceinline__ int xcd_block(int remap) {
+    const int b = blockIdx.x, G = gridDim.x;
+    if (!remap || (G & 7)) return b;
+    return (b & 7) * (G >> 3) + (b >> 3);
+}
+
 template <typename T, int E>
 __global__ void __launch_bounds__(1024)
 k_fs_a(const V_t<T>* __restrict__ in, V_t<T>* __restrict__ out, FourStepAArgs a, PassTable pt,
        const V_t<T>* __restrict__ tw) {
     CFFT_DYN_LDS(smem);
     const int tiles = (a.N2 + a.C - 1) / a.C;
-    const int64_t bt = blockIdx.x / tiles;
-    const int tile = blockIdx.x - (int)(bt * tiles);
+    const int bid = xcd_block(a.xcd);
+    const int64_t bt = bid / tiles;
+    const int tile = bid - (int)(bt * tiles);
     fs_a_body<T, E>(smem, in + bt * a.idist, out + bt * a.odist, a.N1, a.N2, a.C, a.Tt, a.Ls,
                     tile * a.C, a.sgn_in, pt, tw);
 }
@@ -336,8 +345,9 @@ k_fs_b(const V_t<T>* in, V_t<T>* out, FourStepBArgs a, PassTable pt,
        const V_t<T>* __restrict__ tw) {
     CFFT_DYN_LDS(smem);
     const int tiles = (a.N1 + a.C - 1) / a.C;
-    const int64_t bt = blockIdx.x / tiles;
-    const int tile = blockIdx.x - (int)(bt * tiles);
+    const int bid = xcd_block(a.xcd);
+    const int64_t bt = bid / tiles;
+    const int tile = bid - (int)(bt * tiles);
     fs_b_body<T, E>(smem, in + bt * a.idist, out + bt * a.odist, a.N1, a.N2, a.C, a.Tt, a.Ls,
                     tile * a.C, a.sgn_in, a.sgn_out, a.scale, (const V_t<T>*)a.epi.K,
                     a.epi.dc ? (V_t<T>*)a.epi.dc + 2 * bt : nullptr, (const V_t<T>*)a.tw3,

@@ -463,6 +463,36 @@ def bench_serve(args):
                       "capacity_speedup_vs_baseline": round(cap["transforms_per_s"] / base_tps, 2)}))
 
 
+def bench_f16(args):
+    """The batched cases in ComplexF16 storage (float arithmetic): half the bytes of F32.  The
+    check is bitwise: the F16 output equals the F32 plan's output rounded to float16."""
+    import torch
+
+    import catabrafft as C
+
+    s = torch.cuda.current_stream().cuda_stream
+    out = []
+    for n, batch in ((2520, 4096), (1 << 20, 64)):
+        g = torch.Generator(device="cuda").manual_seed(5)
+        x32 = torch.randn((batch, n, 2), generator=g, device="cuda", dtype=torch.float32)
+        xh = x32.to(torch.float16)
+        x32 = xh.to(torch.float32)  # the same values in both precisions
+        yh, y32 = torch.empty_like(xh), torch.empty_like(x32)
+        p16, p32 = C.Plan(n, batch, C.C16), C.Plan(n, batch, np.complex64)
+        ms = _time_ms(lambda: p16.exec_ptr(xh.data_ptr(), yh.data_ptr(), -1, False, s),
+                      args.steps, args.warmup)
+        p32.exec_ptr(x32.data_ptr(), y32.data_ptr(), -1, False, s)
+        torch.cuda.synchronize()
+        same = bool(torch.equal(yh.view(torch.int16), y32.to(torch.float16).view(torch.int16)))
+        gbs = 2 * n * batch * 4 / (ms * 1e-3) / 1e9
+        out.append({"n": n, "batch": batch, "ms": round(ms, 4),
+                    "gflops": round(flops(n, batch) / (ms * 1e-3) / 1e9, 1),
+                    "algo_GBs": round(gbs, 1), "hbm_frac": round(gbs / 8000, 4),
+                    "bits_equal_rounded_f32": same, "plan": p16.describe()})
+    print(json.dumps({"workload": "batched ComplexF16 storage (float arithmetic), forward",
+                      "cases": out}))
+
+
 def bench_copy(args):
     """A plain device-to-device copy of 1 GiB (torch): the reference kernel whose FETCH_SIZE /
     WRITE_SIZE counters calibrate the HBM-traffic figures of the FFT kernels."""
@@ -484,7 +514,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cases", default="all", help="comma list of case indices (batched)")
     ap.add_argument("--workload", default="batched",
-                    choices=["batched", "single", "rader", "uneven", "serve", "copy"])
+                    choices=["batched", "single", "rader", "uneven", "serve", "copy", "f16"])
     ap.add_argument("--requests", type=int, default=4000, help="serve: trace length")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -493,7 +523,8 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     {"batched": bench_batched, "single": bench_single, "rader": bench_rader,
-     "uneven": bench_uneven, "serve": bench_serve, "copy": bench_copy}[args.workload](args)
+     "uneven": bench_uneven, "serve": bench_serve, "copy": bench_copy,
+     "f16": bench_f16}[args.workload](args)
 
 
 if __name__ == "__main__":

@@ -10,8 +10,11 @@ Same names, argument meaning and error behaviour as the reference's module (Cata
     empty_cache()     CatabraFFT.jl:39-43
     NO_FLAG, MEASURE, ENCHANT  spells.jl:4-8
 
-x is a complex64 / complex128 numpy array (host, staged through the device) or a complex torch
-tensor on a GPU (executed in place on the device, on torch's current stream).  A 1-D x is one
+x is a complex64 / complex128 / complex32 numpy array (host, staged through the device) or a
+complex torch tensor on a GPU (executed in place on the device, on torch's current stream).
+complex32 is Julia's ComplexF16: numpy has no complex half type, so it is the structured dtype
+`complex32` below ((re, im) float16 pairs), and torch's complex32 on the device; it is computed
+in float32 and rounded to nearest even on output.  A 1-D x is one
 transform; a 2-D x is a batch of rows (an extension: the reference has no batch API).
 
 Differences from the reference, all fixes: a plan's flag is honoured by p * x (the reference
@@ -27,7 +30,7 @@ import threading
 import numpy as np
 
 from . import _lib
-from ._lib import (BACKWARD, C32, C64, FORWARD, CatabraFFTArgumentError, CatabraFFTError,
+from ._lib import (BACKWARD, C16, C32, C64, FORWARD, CatabraFFTArgumentError, CatabraFFTError,
                    check, lib)
 
 NO_FLAG, MEASURE, ENCHANT = 0, 1, 2
@@ -36,7 +39,7 @@ __all__ = ["fft", "ifft", "bfft", "plan_fft", "plan_bfft", "plan_ifft", "mul", "
            "Spell", "Plan", "UnevenPlan", "Server", "Context", "context", "NO_FLAG", "MEASURE",
            "ENCHANT", "CatabraFFTError", "CatabraFFTArgumentError", "export_wisdom",
            "import_wisdom", "rfft", "irfft", "brfft", "RealPlan", "NDPlan", "fftn", "ifftn",
-           "bfftn"]
+           "bfftn", "complex32", "to_complex32", "from_complex32"]
 
 
 # ------------------------------------------------------------------------------ context
@@ -74,13 +77,38 @@ def context(device: int = 0) -> Context:
 
 
 # ------------------------------------------------------------------------------ dtypes
+# Julia's ComplexF16 on the host: (re, im) float16 pairs, 4 bytes, as the library stores them.
+complex32 = np.dtype([("re", np.float16), ("im", np.float16)])
+
+
+def to_complex32(z) -> np.ndarray:
+    """A complex array rounded to ComplexF16 (each part to the nearest float16)."""
+    z = np.asarray(z)
+    out = np.empty(z.shape, dtype=complex32)
+    out["re"] = z.real.astype(np.float16)
+    out["im"] = z.imag.astype(np.float16)
+    return out
+
+
+def from_complex32(h) -> np.ndarray:
+    """A ComplexF16 array widened (exactly) to complex64."""
+    h = np.asarray(h)
+    return h["re"].astype(np.float32) + 1j * h["im"].astype(np.float32)
+
+
+_CODE_DTYPE = {C32: np.dtype(np.complex64), C64: np.dtype(np.complex128), C16: complex32}
+
+
 def _dtype_code(dt) -> int:
     dt = np.dtype(dt)
     if dt == np.complex64:
         return C32
     if dt == np.complex128:
         return C64
-    raise TypeError(f"CatabraFFT transforms complex vectors (ComplexF32 / ComplexF64), got {dt}")
+    if dt == complex32:
+        return C16
+    raise TypeError("CatabraFFT transforms complex vectors (ComplexF32 / ComplexF64 / "
+                    f"ComplexF16), got {dt}")
 
 
 def _is_torch(x) -> bool:
@@ -94,7 +122,10 @@ def _torch_dtype_code(x) -> int:
         return C32
     if x.dtype == t.complex128:
         return C64
-    raise TypeError(f"CatabraFFT transforms complex vectors (ComplexF32 / ComplexF64), got {x.dtype}")
+    if x.dtype == t.complex32:
+        return C16
+    raise TypeError("CatabraFFT transforms complex vectors (ComplexF32 / ComplexF64 / "
+                    f"ComplexF16), got {x.dtype}")
 
 
 # ------------------------------------------------------------------------------ plans
@@ -108,7 +139,7 @@ class Plan:
         self.idist = int(idist if idist is not None else n)
         self.odist = int(odist if odist is not None else n)
         self.dtype_code = dtype if isinstance(dtype, int) else _dtype_code(dtype)
-        self.dtype = np.complex64 if self.dtype_code == C32 else np.complex128
+        self.dtype = _CODE_DTYPE[self.dtype_code]
         self.flags = int(flags)
         h = ctypes.c_void_p()
         check(lib().cfft_plan_1d(self.ctx.h, self.n, self.batch, self.idist, self.odist,
@@ -132,7 +163,11 @@ class Plan:
         if not x.is_contiguous():
             x = x.contiguous()
         if out is None:
-            out = t.empty_like(x)
+            if x.dtype == t.complex32:  # float16 pairs viewed as complex32 (no ComplexHalf alloc)
+                out = t.empty(x.shape[:-1] + (2 * x.shape[-1],), dtype=t.float16,
+                              device=x.device).view(t.complex32)
+            else:
+                out = t.empty_like(x)
         stream = t.cuda.current_stream(x.device).cuda_stream
         self.exec_ptr(x.data_ptr(), out.data_ptr(), direction, normalize, stream)
         return out
@@ -226,7 +261,7 @@ class NDPlan(Plan):
         for a in axes:
             mask |= 1 << a
         self.dtype_code = dtype if isinstance(dtype, int) else _dtype_code(dtype)
-        self.dtype = np.complex64 if self.dtype_code == C32 else np.complex128
+        self.dtype = _CODE_DTYPE[self.dtype_code]
         self.n = int(np.prod([self.shape[a] for a in set(axes)])) if axes else 1
         self.batch, self.flags = 1, int(flags)
         dims = (ctypes.c_int64 * rank)(*self.shape)
@@ -444,8 +479,7 @@ class Spell:
 def plan_fft(x, flags: int = NO_FLAG) -> Spell:
     """A forward plan for x's length and element type (CatabraFFT.jl:149-171)."""
     n, batch = _shape(x)
-    dt = np.complex64 if (_is_torch(x) and _torch_dtype_code(x) == C32) else (
-        np.complex128 if _is_torch(x) else np.asarray(x).dtype)
+    dt = _CODE_DTYPE[_torch_dtype_code(x)] if _is_torch(x) else np.asarray(x).dtype
     return Spell(n, dt, flags, FORWARD, False, batch)
 
 

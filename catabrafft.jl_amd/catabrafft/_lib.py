@@ -28,7 +28,7 @@ CFFT_ERR_UNSUPPORTED = -4
 CFFT_ERR_ADMISSION = -5
 CFFT_ERR_NOTREADY = -6
 
-C32, C64 = 0, 1
+C32, C64, C16 = 0, 1, 2
 FORWARD, BACKWARD = -1, 1
 
 _lib = None

@@ -121,6 +121,19 @@ void init_kernels();
 
 hipError_t launch_rows(Prec p, const void* in, void* out, const RowsArgs& a, const PassTable& pt,
                        const void* tw, hipStream_t st);
+// ComplexF16 storage, float arithmetic: the rows and four-step kernels with half-precision
+// global lines (the four-step middle stays float), and conversions of rows x n elements between
+// half and float storage (row r at r * sdist in the source, r * ddist in the destination).
+hipError_t launch_rows_f16(const void* in, void* out, const RowsArgs& a, const PassTable& pt,
+                           const void* tw, hipStream_t st);
+hipError_t launch_fourstep_a_f16(const void* in, void* out, const FourStepAArgs& a,
+                                 const PassTable& pt, const void* tw, hipStream_t st);
+hipError_t launch_fourstep_b_f16(const void* in, void* out, const FourStepBArgs& a,
+                                 const PassTable& pt, const void* tw, hipStream_t st);
+hipError_t launch_convert_h2f(const void* src, void* dst, int64_t rows, int64_t n, int64_t sdist,
+                              int64_t ddist, hipStream_t st);
+hipError_t launch_convert_f2h(const void* src, void* dst, int64_t rows, int64_t n, int64_t sdist,
+                              int64_t ddist, hipStream_t st);
 // Persistent pipelined rows kernel (needs npass >= 2, no epilogue); a.grid workgroups.  twn =
 // entries of the twiddle table, which the kernel copies into LDS after the work lines.
 hipError_t launch_rows_pipe(Prec p, const void* in, void* out, const RowsArgs& a,

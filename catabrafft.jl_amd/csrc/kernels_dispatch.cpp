@@ -8,6 +8,7 @@ namespace cfft {
 void init_kernels() {
     init_kernels_f32();
     init_kernels_f64();
+    init_kernels_f16();
     init_kernels_pipe_f32();
     init_kernels_pipe_f64();
 }
@@ -52,6 +53,26 @@ int rows_pipe_blocks_per_cu(Prec p, int E, int R0, int block, size_t lds) {
 hipError_t launch_rows(Prec p, const void* in, void* out, const RowsArgs& a, const PassTable& pt,
                        const void* tw, hipStream_t st) {
     return p == Prec::F32 ? rows_f32(in, out, a, pt, tw, st) : rows_f64(in, out, a, pt, tw, st);
+}
+hipError_t launch_rows_f16(const void* in, void* out, const RowsArgs& a, const PassTable& pt,
+                           const void* tw, hipStream_t st) {
+    return rows_f16(in, out, a, pt, tw, st);
+}
+hipError_t launch_fourstep_a_f16(const void* in, void* out, const FourStepAArgs& a,
+                                 const PassTable& pt, const void* tw, hipStream_t st) {
+    return fsa_f16(in, out, a, pt, tw, st);
+}
+hipError_t launch_fourstep_b_f16(const void* in, void* out, const FourStepBArgs& a,
+                                 const PassTable& pt, const void* tw, hipStream_t st) {
+    return fsb_f16(in, out, a, pt, tw, st);
+}
+hipError_t launch_convert_h2f(const void* src, void* dst, int64_t rows, int64_t n, int64_t sdist,
+                              int64_t ddist, hipStream_t st) {
+    return convert_h2f(src, dst, rows, n, sdist, ddist, st);
+}
+hipError_t launch_convert_f2h(const void* src, void* dst, int64_t rows, int64_t n, int64_t sdist,
+                              int64_t ddist, hipStream_t st) {
+    return convert_f2h(src, dst, rows, n, sdist, ddist, st);
 }
 hipError_t launch_fourstep_a(Prec p, const void* in, void* out, const FourStepAArgs& a,
                              const PassTable& pt, const void* tw, hipStream_t st) {

@@ -15,6 +15,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "kernels.h"
 #include "multi.h"
 
@@ -22,22 +24,47 @@ namespace cfft {
 
 template <typename T> using V_t = typename Vec2<T>::type;
 
-// Global-memory line input; sg = -1 conjugates.
+// ComplexF16 storage: (re, im) as two IEEE binary16 values in 4 bytes.  Lines stored this way
+// are converted to float at their global loads and rounded (to nearest even) at their global
+// stores; all arithmetic in between runs in float.  CFFT_F16 is a macro so that the CPU
+// emulation harness of the tests can supply a software binary16 type.
+#ifndef CFFT_F16
+#define CFFT_F16 _Float16
+#endif
+struct alignas(4) ch16 {
+    CFFT_F16 x, y;
+};
+
+// Global element <-> compute vector, for the storage types V_t<T> (no conversion) and ch16.
 template <typename T>
+__device__ __forceinline__ V_t<T> load_elem(const V_t<T>& g) { return g; }
+template <typename T>
+__device__ __forceinline__ V_t<T> load_elem(const ch16& g) {
+    return V_t<T>{(T)(float)g.x, (T)(float)g.y};
+}
+template <typename T>
+__device__ __forceinline__ void store_elem(V_t<T>& g, V_t<T> v) { g = v; }
+template <typename T>
+__device__ __forceinline__ void store_elem(ch16& g, V_t<T> v) {
+    g = ch16{(CFFT_F16)(float)v.x, (CFFT_F16)(float)v.y};
+}
+
+// Global-memory line input of storage type G; sg = -1 conjugates.
+template <typename T, typename G = V_t<T>>
 struct GSrc {
-    const V_t<T>* p;
+    const G* p;
     T sg;
     __device__ __forceinline__ V_t<T> operator()(int e) const {
-        V_t<T> v = p[e];
+        V_t<T> v = load_elem<T>(p[e]);
         v.y *= sg;
         return v;
     }
 };
 
 // Global-memory line output with scale, conjugation and the optional convolution epilogue.
-template <typename T>
+template <typename T, typename G = V_t<T>>
 struct GRowDst {
-    V_t<T>* p;
+    G* p;
     T sx, sy;
     const V_t<T>* K;
     V_t<T>* dc;
@@ -46,7 +73,7 @@ struct GRowDst {
             if (dc && e == 0) *dc = v;
             v = cconj(cmul(v, K[e]));
         }
-        p[e] = V_t<T>{v.x * sx, v.y * sy};
+        store_elem<T>(p[e], V_t<T>{v.x * sx, v.y * sy});
     }
 };
 
@@ -59,24 +86,24 @@ struct GPlainDst {
 
 // Global-memory line with elements `stride` apart (a column of a row-major matrix); sg = -1
 // conjugates on input, (sx, sy) scale and conjugate on output.
-template <typename T>
+template <typename T, typename G = V_t<T>>
 struct GStrideSrc {
-    const V_t<T>* p;
+    const G* p;
     int64_t stride;
     T sg;
     __device__ __forceinline__ V_t<T> operator()(int e) const {
-        V_t<T> v = p[(int64_t)e * stride];
+        V_t<T> v = load_elem<T>(p[(int64_t)e * stride]);
         v.y *= sg;
         return v;
     }
 };
-template <typename T>
+template <typename T, typename G = V_t<T>>
 struct GStrideDst {
-    V_t<T>* p;
+    G* p;
     int64_t stride;
     T sx, sy;
     __device__ __forceinline__ void put(int e, V_t<T> v) const {
-        p[(int64_t)e * stride] = V_t<T>{v.x * sx, v.y * sy};
+        store_elem<T>(p[(int64_t)e * stride], V_t<T>{v.x * sx, v.y * sy});
     }
 };
 
@@ -119,14 +146,14 @@ __device__ __forceinline__ LdsLine<T> lds_line(unsigned char* smem, int C, int L
 // ------------------------------------------------------------------------------------------
 // Line addressing policies: where line c of a workgroup's lines lives.
 // ------------------------------------------------------------------------------------------
-template <typename T>
+template <typename T, typename GI = V_t<T>, typename GO = V_t<T>>
 struct UniformLines {  // line c at in0 + c*idist / out0 + c*odist, slots dc0 + 2c
-    const V_t<T>* in0;
-    V_t<T>* out0;
+    const GI* in0;
+    GO* out0;
     int64_t idist, odist;
     V_t<T>* dc0;
-    __device__ __forceinline__ const V_t<T>* in(int c) const { return in0 + c * idist; }
-    __device__ __forceinline__ V_t<T>* out(int c) const { return out0 + c * odist; }
+    __device__ __forceinline__ const GI* in(int c) const { return in0 + c * idist; }
+    __device__ __forceinline__ GO* out(int c) const { return out0 + c * odist; }
     __device__ __forceinline__ V_t<T>* dc(int c) const { return dc0 ? dc0 + 2 * c : nullptr; }
 };
 
@@ -168,8 +195,10 @@ __device__ __forceinline__ void rows_body(unsigned char* smem, const LINES& line
     const int t = tid - c * Tt;
     const bool active = (c < C) && (c < count);
     const int li = active ? c : 0;
-    GSrc<T> gs{lines.in(li), (T)sgn};
-    GRowDst<T> gd{lines.out(li), (T)scale, (T)(scale * sgn), K, K ? lines.dc(li) : nullptr};
+    using GI = std::remove_cv_t<std::remove_pointer_t<decltype(lines.in(0))>>;
+    using GO = std::remove_pointer_t<decltype(lines.out(0))>;
+    GSrc<T, GI> gs{lines.in(li), (T)sgn};
+    GRowDst<T, GO> gd{lines.out(li), (T)scale, (T)(scale * sgn), K, K ? lines.dc(li) : nullptr};
     if (pt.npass == 0) {
         if (active && t == 0) gd.put(0, gs(0));
         return;
@@ -178,16 +207,16 @@ __device__ __forceinline__ void rows_body(unsigned char* smem, const LINES& line
     run_line<T, E>(pt, N, t, Tt, active, tw, gs, gd, L, true, true);
 }
 
-template <typename T, int E>
+template <typename T, int E, typename G = V_t<T>>
 __global__ void __launch_bounds__(1024)
-k_rows(const V_t<T>* __restrict__ in, V_t<T>* __restrict__ out, RowsArgs a, PassTable pt,
+k_rows(const G* __restrict__ in, G* __restrict__ out, RowsArgs a, PassTable pt,
        const V_t<T>* __restrict__ tw) {
     CFFT_DYN_LDS(smem);
     const int64_t line0 = (int64_t)blockIdx.x * a.C;
     const int64_t left = a.nlines - line0;
     const int count = (int)(left < a.C ? left : a.C);
-    UniformLines<T> lines{in + line0 * a.idist, out + line0 * a.odist, a.idist, a.odist,
-                          a.epi.dc ? (V_t<T>*)a.epi.dc + 2 * line0 : nullptr};
+    UniformLines<T, G, G> lines{in + line0 * a.idist, out + line0 * a.odist, a.idist, a.odist,
+                                a.epi.dc ? (V_t<T>*)a.epi.dc + 2 * line0 : nullptr};
     rows_body<T, E>(smem, lines, count, a.N, a.C, a.Tt, a.Ls, pt, tw, a.sgn_in, a.scale,
                     (const V_t<T>*)a.epi.K);
 }
@@ -200,8 +229,8 @@ k_rows(const V_t<T>* __restrict__ in, V_t<T>* __restrict__ out, RowsArgs a, Pass
 // Threads are mapped column first (lane = c + C t): a wave's accesses to a row of the matrix
 // cover C adjacent columns, so the first pass reads global memory directly in runs of C
 // elements, with no staging through LDS.
-template <typename T, int E>
-__device__ __forceinline__ void fs_a_body(unsigned char* smem, const V_t<T>* src, V_t<T>* dst,
+template <typename T, int E, typename GI>
+__device__ __forceinline__ void fs_a_body(unsigned char* smem, const GI* src, V_t<T>* dst,
                                           int N1, int N2, int C, int Tt, int Ls, int j20,
                                           int sgn, const PassTable& pt, const V_t<T>* tw) {
     const int cshift = __ffs(C) - 1;
@@ -212,7 +241,7 @@ __device__ __forceinline__ void fs_a_body(unsigned char* smem, const V_t<T>* src
     const bool active = j2 < N2;
     const int j2c = active ? j2 : 0;
     const LdsLine<T> L = lds_line<T>(smem, C, Ls, c);
-    GStrideSrc<T> gs{src + j2c, N2, (T)sgn};
+    GStrideSrc<T, GI> gs{src + j2c, N2, (T)sgn};
     GPlainDst<T> d{dst + (int64_t)j2c * N1};
     run_line<T, E>(pt, N1, t, Tt, active, tw, gs, d, L, true, true);
 }
@@ -225,9 +254,9 @@ __device__ __forceinline__ int xcd_block(int remap) {
     return (b & 7) * (G >> 3) + (b >> 3);
 }
 
-template <typename T, int E>
+template <typename T, int E, typename GI = V_t<T>>
 __global__ void __launch_bounds__(1024)
-k_fs_a(const V_t<T>* __restrict__ in, V_t<T>* __restrict__ out, FourStepAArgs a, PassTable pt,
+k_fs_a(const GI* __restrict__ in, V_t<T>* __restrict__ out, FourStepAArgs a, PassTable pt,
        const V_t<T>* __restrict__ tw) {
     CFFT_DYN_LDS(smem);
     const int tiles = (a.N2 + a.C - 1) / a.C;
@@ -264,8 +293,8 @@ __device__ __forceinline__ void stage_table(const V_t<T>* g, V_t<T>* l, int n) {
                          [&](int i, V_t<T> v) { l[i] = v; });
 }
 
-template <typename T, int E>
-__device__ __forceinline__ void fs_b_body(unsigned char* smem, const V_t<T>* src, V_t<T>* dst,
+template <typename T, int E, typename GO>
+__device__ __forceinline__ void fs_b_body(unsigned char* smem, const V_t<T>* src, GO* dst,
                                           int N1, int N2, int C, int Tt, int Ls, int k10,
                                           int sgn_in, int sgn, double scale, const V_t<T>* K,
                                           V_t<T>* dc, const V_t<T>* tw3, int tbits, int tnc,
@@ -305,7 +334,7 @@ __device__ __forceinline__ void fs_b_body(unsigned char* smem, const V_t<T>* src
     if (!K) {
         // the last pass writes out[k1 + N1 k2] directly, in runs of C (k1) per row k2; the
         // tile's reads all happened in the staged load, so in == out is safe
-        GStrideDst<T> d{dst + (active ? k10 + c : 0), N1, sx, sy};
+        GStrideDst<T, GO> d{dst + (active ? k10 + c : 0), N1, sx, sy};
         run_line<T, E>(pt, N2, t, Tt, active, tw, L, d, L, false, true);
         return;
     }
@@ -335,13 +364,13 @@ __device__ __forceinline__ void fs_b_body(unsigned char* smem, const V_t<T>* src
                 if (dc && o == 0) *dc = v;
                 v = cconj(cmul(v, r.a));
             }
-            dst[o] = V_t<T>{v.x * sx, v.y * sy};
+            store_elem<T>(dst[o], V_t<T>{v.x * sx, v.y * sy});
         });
 }
 
-template <typename T, int E>
+template <typename T, int E, typename GO = V_t<T>>
 __global__ void __launch_bounds__(1024)
-k_fs_b(const V_t<T>* in, V_t<T>* out, FourStepBArgs a, PassTable pt,
+k_fs_b(const V_t<T>* in, GO* out, FourStepBArgs a, PassTable pt,
        const V_t<T>* __restrict__ tw) {
     CFFT_DYN_LDS(smem);
     const int tiles = (a.N1 + a.C - 1) / a.C;
@@ -880,34 +909,35 @@ void allow_big_lds(K kernel) {
                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
 }
 
-template <typename T, int E>
+// Launchers; G is the storage type of the lines' global input / output (V_t<T> or ch16).
+template <typename T, int E, typename G = V_t<T>>
 hipError_t rows_t(const void* in, void* out, const RowsArgs& a, const PassTable& pt,
                   const void* tw, hipStream_t st) {
     const size_t lds = 2 * sizeof(T) * (size_t)a.C * a.Ls;
     const int64_t blocks = (a.nlines + a.C - 1) / a.C;
-    hipLaunchKernelGGL((k_rows<T, E>), dim3((unsigned)blocks), dim3(a.C * a.Tt), lds, st,
-                       (const V_t<T>*)in, (V_t<T>*)out, a, pt, (const V_t<T>*)tw);
+    hipLaunchKernelGGL((k_rows<T, E, G>), dim3((unsigned)blocks), dim3(a.C * a.Tt), lds, st,
+                       (const G*)in, (G*)out, a, pt, (const V_t<T>*)tw);
     return hipGetLastError();
 }
 
-template <typename T, int E>
+template <typename T, int E, typename G = V_t<T>>
 hipError_t fsa_t(const void* in, void* out, const FourStepAArgs& a, const PassTable& pt,
                  const void* tw, hipStream_t st) {
     const size_t lds = 2 * sizeof(T) * (size_t)a.C * a.Ls;
     const int64_t blocks = (int64_t)((a.N2 + a.C - 1) / a.C) * a.batch;
-    hipLaunchKernelGGL((k_fs_a<T, E>), dim3((unsigned)blocks), dim3(a.C * a.Tt), lds, st,
-                       (const V_t<T>*)in, (V_t<T>*)out, a, pt, (const V_t<T>*)tw);
+    hipLaunchKernelGGL((k_fs_a<T, E, G>), dim3((unsigned)blocks), dim3(a.C * a.Tt), lds, st,
+                       (const G*)in, (V_t<T>*)out, a, pt, (const V_t<T>*)tw);
     return hipGetLastError();
 }
 
-template <typename T, int E>
+template <typename T, int E, typename G = V_t<T>>
 hipError_t fsb_t(const void* in, void* out, const FourStepBArgs& a, const PassTable& pt,
                  const void* tw, hipStream_t st) {
     const size_t lds = fsb_tw_offset(sizeof(T), a.C, a.Ls) +
                        (a.tw3 ? (size_t)tw3_entries(a.tbits, a.tnc) * sizeof(V_t<T>) : 0);
     const int64_t blocks = (int64_t)((a.N1 + a.C - 1) / a.C) * a.batch;
-    hipLaunchKernelGGL((k_fs_b<T, E>), dim3((unsigned)blocks), dim3(a.C * a.Tt), lds, st,
-                       (const V_t<T>*)in, (V_t<T>*)out, a, pt, (const V_t<T>*)tw);
+    hipLaunchKernelGGL((k_fs_b<T, E, G>), dim3((unsigned)blocks), dim3(a.C * a.Tt), lds, st,
+                       (const V_t<T>*)in, (G*)out, a, pt, (const V_t<T>*)tw);
     return hipGetLastError();
 }
 

@@ -1,5 +1,5 @@
-// kernels_prec.h -- per-precision launch entry points (defined in kernels_f32.hip and
-// kernels_f64.hip, dispatched on Prec by kernels_dispatch.cpp).
+// kernels_prec.h -- per-precision launch entry points (defined in kernels_f32.hip,
+// kernels_f64.hip and kernels_f16.hip, dispatched on Prec by kernels_dispatch.cpp).
 #pragma once
 #include "kernels.h"
 
@@ -27,6 +27,18 @@ hipError_t real_aux_f32(const void* in, void* out, const RealAuxArgs& a, hipStre
 hipError_t real_aux_f64(const void* in, void* out, const RealAuxArgs& a, hipStream_t st);
 void init_kernels_f32();
 void init_kernels_f64();
+
+hipError_t rows_f16(const void* in, void* out, const RowsArgs& a, const PassTable& pt,
+                    const void* tw, hipStream_t st);
+hipError_t fsa_f16(const void* in, void* out, const FourStepAArgs& a, const PassTable& pt,
+                   const void* tw, hipStream_t st);
+hipError_t fsb_f16(const void* in, void* out, const FourStepBArgs& a, const PassTable& pt,
+                   const void* tw, hipStream_t st);
+hipError_t convert_h2f(const void* src, void* dst, int64_t rows, int64_t n, int64_t sdist,
+                       int64_t ddist, hipStream_t st);
+hipError_t convert_f2h(const void* src, void* dst, int64_t rows, int64_t n, int64_t sdist,
+                       int64_t ddist, hipStream_t st);
+void init_kernels_f16();
 
 hipError_t rows_pipe_f32(const void* in, void* out, const RowsArgs& a, const PassTable& pt,
                          const void* tw, size_t twn, hipStream_t st);

@@ -228,6 +228,8 @@ int make_line(LinePlan& lp, int64_t N, Prec pr, int E, int max_threads) {
 }
 
 static int64_t esize(Prec p) { return p == Prec::F32 ? 8 : 16; }
+// Bytes of one element of the plan's input and output (ComplexF16: 4).
+static int64_t io_esize(const Plan& p) { return p.half ? 4 : esize(p.prec); }
 
 // ------------------------------------------------------------------------------- planning
 static int pow2_floor(int64_t v) {
@@ -675,6 +677,7 @@ static std::string describe(const Plan& p) {
             break;
         default: os << "uneven"; break;
     }
+    if (p.half) os << " io=f16";
     return os.str();
 }
 
@@ -697,12 +700,12 @@ static int run_fourstep(const Plan& p, const void* in, void* mid, void* out, int
                         hipStream_t st, int64_t B) {
     const FourStepPlan& fs = p.fs;
     const int64_t chunk = fs.chunk > 0 ? std::min<int64_t>(fs.chunk, B) : B;
-    const size_t es = esize(p.prec);
+    const size_t es = esize(p.prec), eio = io_esize(p);
     for (int64_t b0 = 0; b0 < B; b0 += chunk) {
         const int64_t nb = std::min<int64_t>(chunk, B - b0);
-        const char* inb = (const char*)in + (size_t)b0 * idist * es;
+        const char* inb = (const char*)in + (size_t)b0 * idist * eio;
         char* midb = (char*)mid + (size_t)b0 * mdist * es;
-        char* outb = (char*)out + (size_t)b0 * odist * es;
+        char* outb = (char*)out + (size_t)b0 * odist * eio;
         FourStepAArgs a{};
         a.N1 = fs.N1;
         a.N2 = fs.N2;
@@ -716,7 +719,9 @@ static int run_fourstep(const Plan& p, const void* in, void* mid, void* out, int
         a.sgn_in = sgn;
         a.xcd = 1;
         const int64_t es_ = esize(p.prec);
-        if (fs.pipe_a > 0) {
+        if (p.half) {
+            CFFT_HIP(launch_fourstep_a_f16(inb, midb, a, fs.a.pt, fs.a.dtw.p, st));
+        } else if (fs.pipe_a > 0) {
             a.grid = (int)std::min<int64_t>(fs.pipe_a, (int64_t)((fs.N2 + fs.CA - 1) / fs.CA) * nb);
             CFFT_HIP(launch_fourstep_a_pipe(p.prec, inb, midb, a, fs.a.pt, fs.a.dtw.p,
                                             fs.a.dtw.bytes / (size_t)es_, st));
@@ -742,7 +747,9 @@ static int run_fourstep(const Plan& p, const void* in, void* mid, void* out, int
         b.tnc = fs.tnc;
         b.tw3 = fs.tw3.p;
         b.xcd = 1;
-        if (fs.pipe_b > 0 && !epi.K) {
+        if (p.half) {
+            CFFT_HIP(launch_fourstep_b_f16(midb, outb, b, fs.b.pt, fs.b.dtw.p, st));
+        } else if (fs.pipe_b > 0 && !epi.K) {
             b.grid = (int)std::min<int64_t>(fs.pipe_b, (int64_t)((fs.N1 + fs.CB - 1) / fs.CB) * nb);
             CFFT_HIP(launch_fourstep_b_pipe(p.prec, midb, outb, b, fs.b.pt, fs.b.dtw.p,
                                             fs.b.dtw.bytes / (size_t)es_, st));
@@ -758,7 +765,7 @@ static int run_fft_s(const Plan& p, const void* in, void* out, int sgn, double s
     const int64_t B = sh.batch;
     switch (p.kind) {
         case Kind::Trivial: {
-            const int64_t es = esize(p.prec);
+            const int64_t es = io_esize(p);
             if (B == 0) return CFFT_OK;
             if (in == out && p.idist == p.odist) return CFFT_OK;
             CFFT_HIP(hipMemcpy2DAsync(out, p.odist * es, in, p.idist * es, es, B,
@@ -779,6 +786,10 @@ static int run_fft_s(const Plan& p, const void* in, void* out, int sgn, double s
             a.sgn_out = sgn;
             a.scale = scale;
             a.epi = epi;
+            if (p.half) {
+                CFFT_HIP(launch_rows_f16(in, out, a, p.line.pt, p.line.dtw.p, st));
+                return CFFT_OK;
+            }
             if (p.pipe_res > 0 && !epi.K) {
                 const int64_t groups = (B + p.C - 1) / p.C;
                 a.grid = (int)std::min<int64_t>(groups, p.pipe_res);
@@ -791,7 +802,7 @@ static int run_fft_s(const Plan& p, const void* in, void* out, int sgn, double s
             return CFFT_OK;
         }
         case Kind::FourStep: {
-            if (in == out) {
+            if (in == out || p.half) {  // the middle in scratch (float for ComplexF16 data)
                 std::lock_guard<std::mutex> g(p.scratch_mu);
                 const size_t need = (size_t)B * p.n * esize(p.prec);
                 if (p.scratch.bytes < need) {
@@ -906,8 +917,8 @@ static int real_exec(const Plan& p, const void* in, void* out, int direction, in
 
 int plan_exec_shaped(const Plan& p, const void* in, void* out, int direction, int normalize,
                      hipStream_t st, const ExecShape& sh) {
-    if (p.kind == Kind::Uneven || p.real || p.kind == Kind::ND || p.kind == Kind::Cols)
-        return fail(CFFT_ERR_UNSUPPORTED, "shaped execution needs a 1-D complex plan");
+    if (p.kind == Kind::Uneven || p.real || p.kind == Kind::ND || p.kind == Kind::Cols || p.half)
+        return fail(CFFT_ERR_UNSUPPORTED, "shaped execution needs a 1-D float/double plan");
     if (sh.batch <= 0) return CFFT_OK;
     if (!in || !out) return fail(CFFT_ERR_INVALID, "null buffer");
     if (in == out && p.kind == Kind::FourStep)
@@ -915,6 +926,25 @@ int plan_exec_shaped(const Plan& p, const void* in, void* out, int direction, in
     const int sgn = direction > 0 ? -1 : 1;
     const double scale = normalize ? 1.0 / (double)p.n : 1.0;
     return run_fft_s(p, in, out, sgn, scale, ConvEpilogue{nullptr, nullptr}, st, sh);
+}
+
+// ComplexF16 data on a plan kind without fused half-precision kernels: the input converted to
+// float in a work buffer, the float transform, the result rounded into the output.
+static int half_exec_staged(const Plan& p, const void* in, void* out, int sgn, double scale,
+                            hipStream_t st) {
+    const bool nd = p.kind == Kind::ND;
+    const int64_t rows = nd ? 1 : p.batch, n = nd ? p.idist : p.n;
+    const int64_t idist = p.idist, odist = p.odist;
+    std::lock_guard<std::mutex> g(p.scratch_mu);
+    const size_t ib = (size_t)rows * idist * 8, ob = (size_t)rows * odist * 8;
+    int rc;
+    if (p.hbuf_in.bytes < ib && (rc = p.hbuf_in.alloc(ib))) return rc;
+    if (p.hbuf_out.bytes < ob && (rc = p.hbuf_out.alloc(ob))) return rc;
+    CFFT_HIP(launch_convert_h2f(in, p.hbuf_in.p, rows, n, idist, idist, st));
+    if ((rc = run_fft(p, p.hbuf_in.p, p.hbuf_out.p, sgn, scale, ConvEpilogue{nullptr, nullptr}, st)))
+        return rc;
+    CFFT_HIP(launch_convert_f2h(p.hbuf_out.p, out, rows, n, odist, odist, st));
+    return CFFT_OK;
 }
 
 int plan_exec(const Plan& p, const void* in, void* out, int direction, int normalize,
@@ -925,6 +955,8 @@ int plan_exec(const Plan& p, const void* in, void* out, int direction, int norma
     if (!in || !out) return fail(CFFT_ERR_INVALID, "null buffer");
     const int sgn = direction > 0 ? -1 : 1;
     const double scale = normalize ? 1.0 / (double)p.n : 1.0;
+    if (p.half && p.kind != Kind::Direct && p.kind != Kind::FourStep && p.kind != Kind::Trivial)
+        return half_exec_staged(p, in, out, sgn, scale, st);
     return run_fft(p, in, out, sgn, scale, ConvEpilogue{nullptr, nullptr}, st);
 }
 
@@ -936,7 +968,8 @@ int plan_exec(const Plan& p, const void* in, void* out, int direction, int norma
 int plan_nd_create(Ctx* ctx, int rank, const int64_t* dims, int64_t axes_mask, int dtype,
                    int flags, std::shared_ptr<Plan>* out) {
     if (rank < 1 || rank > 8 || !dims) return fail(CFFT_ERR_INVALID, "rank must be 1..8");
-    if (dtype != CFFT_C32 && dtype != CFFT_C64) return fail(CFFT_ERR_INVALID, "bad dtype");
+    if (dtype != CFFT_C32 && dtype != CFFT_C64 && dtype != CFFT_C16)
+        return fail(CFFT_ERR_INVALID, "bad dtype");
     if (flags < CFFT_NO_FLAG || flags > CFFT_ENCHANT) return fail(CFFT_ERR_INVALID, "bad flags");
     int64_t total = 1, ntrans = 1;
     for (int a = 0; a < rank; ++a) {
@@ -953,7 +986,9 @@ int plan_nd_create(Ctx* ctx, int rank, const int64_t* dims, int64_t axes_mask, i
     p->idist = p->odist = total;
     p->dtype = dtype;
     p->flags = flags;
-    p->prec = dtype == CFFT_C32 ? Prec::F32 : Prec::F64;
+    p->prec = dtype == CFFT_C64 ? Prec::F64 : Prec::F32;
+    p->half = dtype == CFFT_C16;  // float parts between two conversions (half_exec_staged)
+    const int sdtype = dtype == CFFT_C16 ? CFFT_C32 : dtype;
     std::ostringstream os;
     os << "nd dims=[";
     for (int a = 0; a < rank; ++a) os << (a ? "," : "") << dims[a];
@@ -967,7 +1002,7 @@ int plan_nd_create(Ctx* ctx, int rank, const int64_t* dims, int64_t axes_mask, i
         std::unique_ptr<Plan> s(new Plan());
         s->ctx = ctx;
         s->prec = p->prec;
-        s->dtype = dtype;
+        s->dtype = sdtype;
         s->flags = flags;
         if (inner == 1) {  // contiguous axis: any 1-D plan
             s->n = dims[a];
@@ -1005,6 +1040,7 @@ int plan_nd_create(Ctx* ctx, int rank, const int64_t* dims, int64_t axes_mask, i
         s->idist = s->odist = 1;
         p->parts.push_back(std::move(s));
     }
+    if (p->half) os << " io=f16";
     p->desc = os.str();
     *out = p;
     return CFFT_OK;
@@ -1154,7 +1190,7 @@ size_t plan_device_bytes(const Plan& p) {
     if (p.rsub_fwd) b += plan_device_bytes(*p.rsub_fwd);
     if (p.rsub_bwd) b += plan_device_bytes(*p.rsub_bwd);
     for (const auto& s : p.parts) b += plan_device_bytes(*s);
-    b += p.uneven_bytes;
+    b += p.uneven_bytes + p.hbuf_in.bytes + p.hbuf_out.bytes;
     return b;
 }
 
@@ -1162,7 +1198,8 @@ int plan_create(Ctx* ctx, int64_t n, int64_t batch, int64_t idist, int64_t odist
                 int flags, const PlanOptions& opts, std::shared_ptr<Plan>* out) {
     if (n < 1) return fail(CFFT_ERR_INVALID, "n must be >= 1");
     if (batch < 0) return fail(CFFT_ERR_INVALID, "batch must be >= 0");
-    if (dtype != CFFT_C32 && dtype != CFFT_C64) return fail(CFFT_ERR_INVALID, "bad dtype");
+    if (dtype != CFFT_C32 && dtype != CFFT_C64 && dtype != CFFT_C16)
+        return fail(CFFT_ERR_INVALID, "bad dtype");
     if (flags < CFFT_NO_FLAG || flags > CFFT_ENCHANT) return fail(CFFT_ERR_INVALID, "bad flags");
     if (idist < n || odist < n) return fail(CFFT_ERR_INVALID, "idist/odist must be >= n");
     if (n > ((int64_t)1 << 30)) return fail(CFFT_ERR_UNSUPPORTED, "n above 2^30");
@@ -1174,7 +1211,7 @@ int plan_create(Ctx* ctx, int64_t n, int64_t batch, int64_t idist, int64_t odist
     p->odist = odist;
     p->dtype = dtype;
     p->flags = flags;
-    p->prec = dtype == CFFT_C32 ? Prec::F32 : Prec::F64;
+    p->prec = dtype == CFFT_C64 ? Prec::F64 : Prec::F32;
     CFFT_HIP(hipSetDevice(ctx->device));
     int rc;
     if (flags == CFFT_NO_FLAG || opts.explicit_opts) {
@@ -1183,6 +1220,12 @@ int plan_create(Ctx* ctx, int64_t n, int64_t batch, int64_t idist, int64_t odist
         rc = autotune(ctx, *p, flags, opts);
     }
     if (rc) return rc;
+    if (dtype == CFFT_C16) {  // the float plan with half-precision lines (plain rows kernel)
+        p->half = true;
+        p->pipe_res = 0;
+        p->fs.pipe_a = p->fs.pipe_b = 0;
+        p->desc.clear();
+    }
     if (p->desc.empty()) p->desc = describe(*p);
     *out = p;
     return CFFT_OK;
@@ -1324,7 +1367,7 @@ int cfft_exec_host(cfft_plan plan, const void* in, void* out, int direction, int
     const Plan& p = *plan->p;
     Ctx* c = p.ctx;
     CFFT_HIP(hipSetDevice(c->device));
-    const int64_t es = esize(p.prec);
+    const int64_t es = io_esize(p);
     size_t ib, ob;
     if (p.kind == Kind::Uneven) {
         ib = ob = (size_t)p.uneven_span * es;

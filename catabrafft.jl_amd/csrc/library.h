@@ -64,6 +64,10 @@ struct Plan {
     FourStepPlan fs;
     mutable std::mutex scratch_mu;
     mutable DevBuf scratch;
+    // ComplexF16 input and output (dtype CFFT_C16): the plan computes in float; direct and
+    // four-step plans convert inside their kernels, other kinds through hbuf_in / hbuf_out
+    bool half = false;
+    mutable DevBuf hbuf_in, hbuf_out;
     // convolution (Rader / Bluestein)
     ConvMode mode = ConvMode::Rader;
     int64_t L = 0, g = 0;

@@ -45,7 +45,8 @@ end
 
 cfft_dtype(::Type{Float32}) = Cint(0)
 cfft_dtype(::Type{Float64}) = Cint(1)
-cfft_dtype(::Type{T}) where {T} = throw(ArgumentError("CatabraFFT on MI355X supports ComplexF32 and ComplexF64, got Complex{$T}"))
+cfft_dtype(::Type{Float16}) = Cint(2)  # ComplexF16 storage, Float32 arithmetic (complex plans only)
+cfft_dtype(::Type{T}) where {T} = throw(ArgumentError("CatabraFFT on MI355X supports ComplexF16, ComplexF32 and ComplexF64, got Complex{$T}"))
 
 const PLANS = Dict{Tuple{Int,Int,DataType,FLAG},Ptr{Cvoid}}()
 

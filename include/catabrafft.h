@@ -3,7 +3,8 @@
  *
  * Every entry point takes plain pointers and sizes, returns an int status (0 = OK, < 0 = error,
  * message in cfft_last_error()) and never aborts.  Complex data is interleaved (re, im) of
- * float (CFFT_C32, Julia ComplexF32) or double (CFFT_C64, Julia ComplexF64), rows contiguous,
+ * float (CFFT_C32, Julia ComplexF32), double (CFFT_C64, Julia ComplexF64) or, for 1-D and N-D
+ * complex plans, half (CFFT_C16, Julia ComplexF16), rows contiguous,
  * 0-based.  Forward uses e^{-2 pi i jk/n}; backward is unnormalised (CatabraFFT.bfft) and
  * normalize = 1 divides by n (CatabraFFT.ifft).  Plans are immutable once created and may be
  * executed concurrently on distinct streams.  Out-of-place execution never modifies the input;
@@ -48,6 +49,10 @@ extern "C" {
 /* dtypes */
 #define CFFT_C32 0
 #define CFFT_C64 1
+/* ComplexF16: (re, im) IEEE binary16 pairs, 4 bytes per element; computed in float and rounded
+ * to nearest even on output.  cfft_plan_1d and cfft_plan_nd only (runtests.jl:39-46 round-trips
+ * ComplexF16 through fft / ifft). */
+#define CFFT_C16 2
 
 /* planner flags (spells.jl:4-8) */
 #define CFFT_NO_FLAG 0
@@ -74,8 +79,8 @@ CFFT_API int cfft_init(int device, cfft_ctx* ctx);
 CFFT_API int cfft_finalize(cfft_ctx ctx);
 
 /* A plan for `batch` transforms of length n: row b of the input starts at element b*idist,
- * row b of the output at element b*odist (idist, odist >= n).  dtype CFFT_C32 / CFFT_C64,
- * flags CFFT_NO_FLAG (default heuristics) / CFFT_MEASURE (time the candidate kernel
+ * row b of the output at element b*odist (idist, odist >= n).  dtype CFFT_C32 / CFFT_C64 /
+ * CFFT_C16, flags CFFT_NO_FLAG (default heuristics) / CFFT_MEASURE (time the candidate kernel
  * configurations on the device and keep the fastest) / CFFT_ENCHANT (wider search). */
 CFFT_API int cfft_plan_1d(cfft_ctx ctx, int64_t n, int64_t batch, int64_t idist, int64_t odist,
                  int dtype, int flags, cfft_plan* plan);

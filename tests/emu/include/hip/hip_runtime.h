@@ -22,6 +22,53 @@
 struct float2 {
     float x, y;
 };
+
+// IEEE binary16 in software, standing in for the device's _Float16: conversion from float rounds
+// to nearest, ties to even (subnormals kept, overflow to infinity), as v_cvt_f16_f32 does.
+struct emu_half {
+    uint16_t bits;
+    emu_half() = default;
+    emu_half(float f) : bits(from_float(f)) {}
+    operator float() const { return to_float(bits); }
+    static uint16_t from_float(float f) {
+        uint32_t x;
+        std::memcpy(&x, &f, 4);
+        const uint32_t sign = (x >> 16) & 0x8000u, ax = x & 0x7fffffffu;
+        if (ax >= 0x7f800000u)  // infinity, NaN (quiet)
+            return (uint16_t)(sign | 0x7c00u | (ax > 0x7f800000u ? 0x200u | ((ax >> 13) & 0x3ffu) : 0u));
+        if (ax >= 0x477ff000u) return (uint16_t)(sign | 0x7c00u);  // >= 65520 rounds to infinity
+        if (ax < 0x38800000u) {                                    // below 2^-14: subnormal
+            if (ax < 0x33000000u) return (uint16_t)sign;           // below 2^-25: zero
+            const uint32_t e = ax >> 23, m = (ax & 0x7fffffu) | 0x800000u;
+            const uint32_t shift = 126u - e;                        // units of 2^-24
+            uint32_t q = m >> shift;
+            const uint32_t rem = m & ((1u << shift) - 1u), half = 1u << (shift - 1u);
+            if (rem > half || (rem == half && (q & 1u))) ++q;
+            return (uint16_t)(sign | q);
+        }
+        uint32_t q = (((ax >> 23) - 112u) << 10) | ((ax & 0x7fffffu) >> 13);
+        const uint32_t rem = ax & 0x1fffu;
+        if (rem > 0x1000u || (rem == 0x1000u && (q & 1u))) ++q;
+        return (uint16_t)(sign | q);
+    }
+    static float to_float(uint16_t h) {
+        const uint32_t sign = (uint32_t)(h & 0x8000u) << 16, e = (h >> 10) & 0x1fu, m = h & 0x3ffu;
+        uint32_t x;
+        if (e == 0x1f) {
+            x = sign | 0x7f800000u | (m << 13);
+        } else if (e == 0) {
+            const float f = (float)m * 5.9604644775390625e-08f;  // m * 2^-24, exact
+            std::memcpy(&x, &f, 4);
+            x |= sign;
+        } else {
+            x = sign | ((e + 112u) << 23) | (m << 13);
+        }
+        float f;
+        std::memcpy(&f, &x, 4);
+        return f;
+    }
+};
+#define CFFT_F16 emu_half
 struct double2 {
     double x, y;
 };

@@ -179,21 +179,36 @@ __device__ __forceinline__ void stockham_pass(const int N, const int s, const in
 }
 
 // Dispatch of one pass on its radix.  The four source/destination combinations are
-// instantiated by the callers below.
-#define CFFT_RADIX_SWITCH(R_, CALL)                                  \
-    switch (R_) {                                                    \
-        case 16: CALL(16); break;                                    \
-        case 13: CALL(13); break;                                    \
-        case 11: CALL(11); break;                                    \
-        case 9: CALL(9); break;                                      \
-        case 8: CALL(8); break;                                      \
-        case 7: CALL(7); break;                                      \
-        case 5: CALL(5); break;                                      \
-        case 4: CALL(4); break;                                      \
-        case 3: CALL(3); break;                                      \
-        case 2: CALL(2); break;                                      \
-        default: break;                                              \
+// instantiated by the callers below.  Radices above RM_, a compile-time bound of the kernel, are
+// left out: a kernel built for lines of short radices holds fewer registers per lane, so more
+// waves fit a SIMD.
+#define CFFT_RADIX_CASE(RM_, RR, CALL) \
+    case RR:                          \
+        if constexpr ((RM_) >= RR) {  \
+            CALL(RR);                 \
+        }                             \
+        break;
+#define CFFT_RADIX_SWITCH(RM_, R_, CALL)   \
+    switch (R_) {                          \
+        CFFT_RADIX_CASE(RM_, 16, CALL)     \
+        CFFT_RADIX_CASE(RM_, 13, CALL)     \
+        CFFT_RADIX_CASE(RM_, 11, CALL)     \
+        CFFT_RADIX_CASE(RM_, 9, CALL)      \
+        CFFT_RADIX_CASE(RM_, 8, CALL)      \
+        CFFT_RADIX_CASE(RM_, 7, CALL)      \
+        CFFT_RADIX_CASE(RM_, 5, CALL)      \
+        CFFT_RADIX_CASE(RM_, 4, CALL)      \
+        CFFT_RADIX_CASE(RM_, 3, CALL)      \
+        CFFT_RADIX_CASE(RM_, 2, CALL)      \
+        default: break;                    \
     }
+
+// The largest radix a kernel must hold for a pass table (16: every radix).
+inline int pass_table_rmax(const PassTable& pt) {
+    int r = 2;
+    for (int i = 0; i < pt.npass; ++i) r = pt.radix[i] > r ? pt.radix[i] : r;
+    return r;
+}
 
 // Runs passes ip0.. of one line (s0 = product of the radices before ip0).  GSRC / GDST are the
 // input and output functors of the line (HBM, or a gather / scatter over another LDS buffer);
@@ -201,7 +216,7 @@ __device__ __forceinline__ void stockham_pass(const int N, const int s, const in
 // in LDS).  dst_aliases_lds asks for a barrier between the reads and writes of the last pass
 // because GDST writes the line's own LDS.  The caller provides the barrier before the first pass
 // when it staged the input into LDS.  TWT is a twiddle table (TwGlobal or TwLds).
-template <typename T, int E, class TWT, class GSRC, class GDST>
+template <typename T, int E, int RM = 16, class TWT, class GSRC, class GDST>
 __device__ __forceinline__ void run_line_t(const PassTable& pt, const int N, const int t,
                                            const int Tt, const bool active, const TWT& twt,
                                            const GSRC& gsrc, const GDST& gdst,
@@ -217,19 +232,19 @@ __device__ __forceinline__ void run_line_t(const PassTable& pt, const int N, con
         const bool dg = last && to_global;
         if (sg && dg) {
 #define CFFT_CALL_GG(RR) stockham_pass<T, RR, bfly_per_thread(E, RR)>(N, s, t, Tt, active, twa, gsrc, gdst, dst_aliases_lds)
-            CFFT_RADIX_SWITCH(R, CFFT_CALL_GG)
+            CFFT_RADIX_SWITCH(RM, R, CFFT_CALL_GG)
 #undef CFFT_CALL_GG
         } else if (sg) {
 #define CFFT_CALL_GL(RR) stockham_pass<T, RR, bfly_per_thread(E, RR)>(N, s, t, Tt, active, twa, gsrc, lds, false)
-            CFFT_RADIX_SWITCH(R, CFFT_CALL_GL)
+            CFFT_RADIX_SWITCH(RM, R, CFFT_CALL_GL)
 #undef CFFT_CALL_GL
         } else if (dg) {
 #define CFFT_CALL_LG(RR) stockham_pass<T, RR, bfly_per_thread(E, RR)>(N, s, t, Tt, active, twa, lds, gdst, dst_aliases_lds)
-            CFFT_RADIX_SWITCH(R, CFFT_CALL_LG)
+            CFFT_RADIX_SWITCH(RM, R, CFFT_CALL_LG)
 #undef CFFT_CALL_LG
         } else {
 #define CFFT_CALL_LL(RR) stockham_pass<T, RR, bfly_per_thread(E, RR)>(N, s, t, Tt, active, twa, lds, lds, true)
-            CFFT_RADIX_SWITCH(R, CFFT_CALL_LL)
+            CFFT_RADIX_SWITCH(RM, R, CFFT_CALL_LL)
 #undef CFFT_CALL_LL
         }
         // The pass output sits in LDS for the next pass (or for the caller's staged store).
@@ -239,7 +254,7 @@ __device__ __forceinline__ void run_line_t(const PassTable& pt, const int N, con
 }
 
 // Every pass of one line with the plan's twiddle table in global memory.
-template <typename T, int E, class GSRC, class GDST>
+template <typename T, int E, int RM = 16, class GSRC, class GDST>
 __device__ __forceinline__ void run_line(const PassTable& pt, const int N, const int t,
                                          const int Tt, const bool active,
                                          const typename Vec2<T>::type* __restrict__ tw,
@@ -247,7 +262,7 @@ __device__ __forceinline__ void run_line(const PassTable& pt, const int N, const
                                          const LdsLine<T>& lds, const bool from_global,
                                          const bool to_global,
                                          const bool dst_aliases_lds = false) {
-    run_line_t<T, E>(pt, N, t, Tt, active, TwGlobal<typename Vec2<T>::type>{tw}, gsrc, gdst, lds,
+    run_line_t<T, E, RM>(pt, N, t, Tt, active, TwGlobal<typename Vec2<T>::type>{tw}, gsrc, gdst, lds,
                      from_global, to_global, dst_aliases_lds, 0, 1);
 }
 

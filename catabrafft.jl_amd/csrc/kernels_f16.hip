@@ -15,7 +15,7 @@ using namespace detail;
 
 hipError_t rows_f16(const void* in, void* out, const RowsArgs& a, const PassTable& pt,
                     const void* tw, hipStream_t st) {
-    return CFFT_E(rows_t, float, a.E, in, out, a, pt, tw, st);
+    return rows_rm_t<float, ch16>(in, out, a, pt, tw, st);
 }
 hipError_t fsa_f16(const void* in, void* out, const FourStepAArgs& a, const PassTable& pt,
                    const void* tw, hipStream_t st) {
@@ -68,6 +68,8 @@ hipError_t convert_f2h(const void* src, void* dst, int64_t rows, int64_t n, int6
 void init_kernels_f16() {
     allow_big_lds(k_rows<float, 8, ch16>);
     allow_big_lds(k_rows<float, 16, ch16>);
+    allow_big_lds(k_rows<float, 8, ch16, 9>);
+    allow_big_lds(k_rows<float, 16, ch16, 9>);
     allow_big_lds(k_fs_a<float, 8, ch16>);
     allow_big_lds(k_fs_a<float, 16, ch16>);
     allow_big_lds(k_fs_b<float, 8, ch16>);

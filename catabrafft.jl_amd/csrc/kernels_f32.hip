@@ -7,7 +7,7 @@ using namespace detail;
 #define CFFT_E(FN, T, E_, ...) ((E_) <= 8 ? FN<T, 8>(__VA_ARGS__) : FN<T, 16>(__VA_ARGS__))
 hipError_t rows_f32(const void* in, void* out, const RowsArgs& a, const PassTable& pt,
                     const void* tw, hipStream_t st) {
-    return CFFT_E(rows_t, float, a.E, in, out, a, pt, tw, st);
+    return rows_rm_t<float>(in, out, a, pt, tw, st);
 }
 hipError_t fsa_f32(const void* in, void* out, const FourStepAArgs& a, const PassTable& pt,
                    const void* tw, hipStream_t st) {
@@ -44,6 +44,8 @@ hipError_t real_aux_f32(const void* in, void* out, const RealAuxArgs& a, hipStre
 void init_kernels_f32() {
     allow_big_lds(k_rows<float, 8>);
     allow_big_lds(k_rows<float, 16>);
+    allow_big_lds(k_rows<float, 8, float2, 9>);
+    allow_big_lds(k_rows<float, 16, float2, 9>);
     allow_big_lds(k_fs_a<float, 8>);
     allow_big_lds(k_fs_a<float, 16>);
     allow_big_lds(k_fs_b<float, 8>);

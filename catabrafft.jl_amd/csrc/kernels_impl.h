@@ -185,7 +185,7 @@ struct ItemLines {  // an MItem's lines: uniform distances or a table of {in, ou
 // rows body: up to C lines of length N (`count` valid).  A line with no passes (N = 1) is a
 // scaled copy.
 // ------------------------------------------------------------------------------------------
-template <typename T, int E, class LINES>
+template <typename T, int E, int RM = 16, class LINES>
 __device__ __forceinline__ void rows_body(unsigned char* smem, const LINES& lines, int count,
                                           int N, int C, int Tt, int Ls, const PassTable& pt,
                                           const V_t<T>* tw, int sgn, double scale,
@@ -204,10 +204,11 @@ __device__ __forceinline__ void rows_body(unsigned char* smem, const LINES& line
         return;
     }
     const LdsLine<T> L = lds_line<T>(smem, C, Ls, c < C ? c : 0);
-    run_line<T, E>(pt, N, t, Tt, active, tw, gs, gd, L, true, true);
+    run_line<T, E, RM>(pt, N, t, Tt, active, tw, gs, gd, L, true, true);
 }
 
-template <typename T, int E, typename G = V_t<T>>
+// RM bounds the radices the kernel holds (lines of radices <= 9 run on a lighter instance).
+template <typename T, int E, typename G = V_t<T>, int RM = 16>
 __global__ void __launch_bounds__(1024)
 k_rows(const G* __restrict__ in, G* __restrict__ out, RowsArgs a, PassTable pt,
        const V_t<T>* __restrict__ tw) {
@@ -217,8 +218,8 @@ k_rows(const G* __restrict__ in, G* __restrict__ out, RowsArgs a, PassTable pt,
     const int count = (int)(left < a.C ? left : a.C);
     UniformLines<T, G, G> lines{in + line0 * a.idist, out + line0 * a.odist, a.idist, a.odist,
                                 a.epi.dc ? (V_t<T>*)a.epi.dc + 2 * line0 : nullptr};
-    rows_body<T, E>(smem, lines, count, a.N, a.C, a.Tt, a.Ls, pt, tw, a.sgn_in, a.scale,
-                    (const V_t<T>*)a.epi.K);
+    rows_body<T, E, RM>(smem, lines, count, a.N, a.C, a.Tt, a.Ls, pt, tw, a.sgn_in, a.scale,
+                        (const V_t<T>*)a.epi.K);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -910,14 +911,26 @@ void allow_big_lds(K kernel) {
 }
 
 // Launchers; G is the storage type of the lines' global input / output (V_t<T> or ch16).
-template <typename T, int E, typename G = V_t<T>>
+template <typename T, int E, typename G = V_t<T>, int RM = 16>
 hipError_t rows_t(const void* in, void* out, const RowsArgs& a, const PassTable& pt,
                   const void* tw, hipStream_t st) {
     const size_t lds = 2 * sizeof(T) * (size_t)a.C * a.Ls;
     const int64_t blocks = (a.nlines + a.C - 1) / a.C;
-    hipLaunchKernelGGL((k_rows<T, E, G>), dim3((unsigned)blocks), dim3(a.C * a.Tt), lds, st,
+    hipLaunchKernelGGL((k_rows<T, E, G, RM>), dim3((unsigned)blocks), dim3(a.C * a.Tt), lds, st,
                        (const G*)in, (G*)out, a, pt, (const V_t<T>*)tw);
     return hipGetLastError();
+}
+
+// Rows launch on the instance for the plan's radices: lines whose radices are all <= 9 use the
+// RM = 9 kernel (float: 60 instead of 92 registers per lane, 8 instead of 5 waves per SIMD).
+template <typename T, typename G = V_t<T>>
+hipError_t rows_rm_t(const void* in, void* out, const RowsArgs& a, const PassTable& pt,
+                     const void* tw, hipStream_t st) {
+    if (pass_table_rmax(pt) <= 9)
+        return a.E <= 8 ? rows_t<T, 8, G, 9>(in, out, a, pt, tw, st)
+                        : rows_t<T, 16, G, 9>(in, out, a, pt, tw, st);
+    return a.E <= 8 ? rows_t<T, 8, G>(in, out, a, pt, tw, st)
+                    : rows_t<T, 16, G>(in, out, a, pt, tw, st);
 }
 
 template <typename T, int E, typename G = V_t<T>>

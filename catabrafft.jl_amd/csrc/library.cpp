@@ -261,9 +261,14 @@ static int pipe_resident(Ctx* ctx, const Plan& p) {
     return bpc > 0 ? bpc * ctx->num_cu : 0;
 }
 
+// pipe: 1 = the pipelined rows kernel, 0 = the plain one, kPipeAuto = the default (pipelined
+// for F32 lines with a radix above 9; lines of radices <= 9 run faster on the plain kernel's
+// radix-bounded instance, which fits more waves per SIMD).
+constexpr int kPipeAuto = 2;
 static int plan_direct(Ctx* ctx, Plan& p, int E, int target_threads, int pipe = -1) {
     int rc = make_line(p.line, p.n, p.prec, E, 1024);
     if (rc) return rc;
+    if (pipe == kPipeAuto) pipe = p.prec == Prec::F32 && pass_table_rmax(p.line.pt) > 9;
     const size_t per = 2 * (size_t)esize(p.prec) / 2 * p.line.Ls;
     if (per > ctx->lds_max) return fail(CFFT_ERR_UNSUPPORTED, "line does not fit LDS");
     p.C = rows_lines_per_block(p.line, p.prec, p.batch, std::max<size_t>(per, ctx->lds_rows_budget),
@@ -616,11 +621,11 @@ static int plan_fft_core(Ctx* ctx, Plan& p, const PlanOptions& o) {
     const int E = o.E ? o.E : 16;
     if (o.force_conv == 0 && is_smooth(p.n)) {
         // Direct lines default to E = 8 (more threads per line; measured faster than 16 on
-        // MI355X for F32 and F64) when the line fits a workgroup that way, and F32 to the
-        // pipelined rows kernel.
+        // MI355X for F32 and F64) when the line fits a workgroup that way, and F32 lines with a
+        // radix above 9 to the pipelined rows kernel.
         if (o.n1 == 0) {
             const int Ed = o.E ? o.E : (line_fits(ctx, p.n, p.prec, 8) ? 8 : 16);
-            const int pipe = o.pipe >= 0 ? o.pipe : (!o.explicit_opts && p.prec == Prec::F32);
+            const int pipe = o.pipe >= 0 ? o.pipe : (o.explicit_opts ? 0 : kPipeAuto);
             if (line_fits(ctx, p.n, p.prec, Ed))
                 return plan_direct(ctx, p, Ed, o.target_threads, pipe);
         }

@@ -14,6 +14,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "codelets.h"
 
 // Dynamic LDS of a kernel, and an opaque register copy (an empty asm the compiler cannot see
@@ -115,6 +117,12 @@ struct TwLds {
     __device__ __forceinline__ TwColAcc<V> pass(int off, int n1) const { return {tw + off, n1}; }
 };
 
+// HasPost<S>: S has a member post(e, v), applied to element e after the pass's loads.
+template <class S, class = void>
+struct HasPost : std::false_type {};
+template <class S>
+struct HasPost<S, std::void_t<decltype(&S::post)>> : std::true_type {};
+
 // One Stockham pass.  SRC(e) returns element e of the pass input, DST.put(e, v) stores element e
 // of its output.  mid_barrier separates the reads from the writes when both go to the same LDS.
 template <typename T, int R, int MB, class SRC, class DST, class TWA>
@@ -155,6 +163,18 @@ __device__ __forceinline__ void stockham_pass(const int N, const int s, const in
         if (active && b < nb) {
 #pragma unroll
             for (int j = 0; j < R; ++j) v[ib][j] = src(b + j * nb);
+        }
+    }
+    // A source with a post step (work on each loaded value, e.g. a twiddle from a table) applies
+    // it once all of the pass's loads are issued, so one wait covers them.
+    if constexpr (HasPost<SRC>::value) {
+#pragma unroll
+        for (int ib = 0; ib < MB; ++ib) {
+            const int b = t + ib * Tt;
+            if (active && b < nb) {
+#pragma unroll
+                for (int j = 0; j < R; ++j) v[ib][j] = src.post(b + j * nb, v[ib][j]);
+            }
         }
     }
     if (mid_barrier) __syncthreads();

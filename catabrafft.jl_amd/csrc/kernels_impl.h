@@ -294,6 +294,25 @@ __device__ __forceinline__ void stage_table(const V_t<T>* g, V_t<T>* l, int n) {
                          [&](int i, V_t<T> v) { l[i] = v; });
 }
 
+// Pass B's line input straight from the middle: element e of column k1 at p[e * stride] (p =
+// middle + k1, stride = N1), then (post, once the pass's loads are issued) times the four-step
+// twiddle w_n^{e k1} from the three-level table in LDS (t3 == nullptr: no twiddle); sg = -1
+// conjugates.
+template <typename T>
+struct GTwColSrc {
+    const V_t<T>* p;
+    int64_t stride;
+    const V_t<T>* t3;
+    int tbits, k1;
+    T sg;
+    __device__ __forceinline__ V_t<T> operator()(int e) const { return p[(int64_t)e * stride]; }
+    __device__ __forceinline__ V_t<T> post(int e, V_t<T> v) const {
+        if (t3) v = cmul(v, tw3_get<T>(t3, tbits, e * k1));
+        v.y *= sg;
+        return v;
+    }
+};
+
 template <typename T, int E, typename GO>
 __device__ __forceinline__ void fs_b_body(unsigned char* smem, const V_t<T>* src, GO* dst,
                                           int N1, int N2, int C, int Tt, int Ls, int k10,
@@ -302,44 +321,29 @@ __device__ __forceinline__ void fs_b_body(unsigned char* smem, const V_t<T>* src
                                           const PassTable& pt, const V_t<T>* tw) {
     const int cshift = __ffs(C) - 1;
     const int total = C * N2;
-    const T sgi = (T)sgn_in;
     V_t<T>* t3 = reinterpret_cast<V_t<T>*>(smem + fsb_tw_offset(sizeof(T), C, Ls));
     if (tw3) stage_table<T>(tw3, t3, tw3_entries(tbits, tnc));
     __syncthreads();
-    // Staged load of runs of C elements (k1) per middle row j2, times the four-step twiddle.
-    // Lines past N1 (a partial last tile) load line N1 - 1 and are zeroed.
-    batched_for<kStageLoadU>(
-        threadIdx.x, blockDim.x, total,
-        [&](int idx) {
-            const int c = idx & (C - 1);
-            const int e = idx >> cshift;
-            const int k1 = k10 + c < N1 ? k10 + c : N1 - 1;
-            return src[(int64_t)e * N1 + k1];
-        },
-        [&](int idx, V_t<T> v) {
-            const int c = idx & (C - 1);
-            const int e = idx >> cshift;
-            if (tw3) v = cmul(v, tw3_get<T>(t3, tbits, e * (k10 + c)));
-            if (k10 + c >= N1) v = V_t<T>{0, 0};
-            v.y *= sgi;
-            lds_line<T>(smem, C, Ls, c).put(e, v);
-        });
-    __syncthreads();
-    // threads column first (lane = c + C t), as in pass A
+    // threads column first (lane = c + C t), as in pass A: the first pass reads the middle
+    // straight from global memory in runs of C elements (k1) per row j2, applying the four-step
+    // twiddle in registers.  Every element of the tile is read by the first pass, before any
+    // write (the last pass's writes come after the passes' barriers; a single-pass line gets a
+    // barrier between its reads and writes), so in == out is safe.
     const int tid = threadIdx.x;
     const int c = tid & (C - 1);
     const int t = tid >> cshift;
     const bool active = k10 + c < N1;
+    const int k1 = active ? k10 + c : 0;
     const LdsLine<T> L = lds_line<T>(smem, C, Ls, c);
+    GTwColSrc<T> gs{src + k1, N1, tw3 ? t3 : nullptr, tbits, k1, (T)sgn_in};
     const T sx = (T)scale, sy = (T)(scale * sgn);
     if (!K) {
-        // the last pass writes out[k1 + N1 k2] directly, in runs of C (k1) per row k2; the
-        // tile's reads all happened in the staged load, so in == out is safe
-        GStrideDst<T, GO> d{dst + (active ? k10 + c : 0), N1, sx, sy};
-        run_line<T, E>(pt, N2, t, Tt, active, tw, L, d, L, false, true);
+        // the last pass writes out[k1 + N1 k2] directly, in runs of C (k1) per row k2
+        GStrideDst<T, GO> d{dst + k1, N1, sx, sy};
+        run_line<T, E>(pt, N2, t, Tt, active, tw, gs, d, L, true, true, pt.npass == 1);
         return;
     }
-    run_line<T, E>(pt, N2, t, Tt, active, tw, L, L, L, false, false);
+    run_line<T, E>(pt, N2, t, Tt, active, tw, gs, L, L, true, false);
     // run_line ended with a barrier: the lines' outputs are in LDS.
     // Staged store in runs of C (k1) per output row k2; the epilogue spectrum is loaded in the
     // same batch.  Indices of lines past N1 are skipped.

@@ -293,14 +293,17 @@ static bool line_fits(Ctx* ctx, int64_t N, Prec pr, int E) {
 // Lines per four-step workgroup: a power of two, at most `want`, within 1024 threads and the LDS
 // left after `reserve` bytes (pass B's twiddle table).
 // LDS line stride of a four-step workgroup of C lines, for its column-first thread mapping
-// (lane = c + C t): of the strides >= the line's minimum, the one whose modelled bank conflicts
-// (one wave of each pass: its first read and first two writes, 64 and 32 banks of 4 bytes) are
-// lowest, the smallest on ties.  Results are cached per (precision, N, C).
-static int bank_cost(std::vector<int>& words, int banks) {
-    // words: the 4-byte LDS words one wave touches; cost = most distinct words on one bank
-    std::sort(words.begin(), words.end());
+// (lane = c + C t): of the strides >= the line's minimum, the one whose bank conflicts are
+// lowest, the smallest on ties.  The model replays every LDS instruction of the workgroup's
+// first wave -- each pass's reads of its inputs and writes of its outputs, except the first
+// pass's reads and the last pass's writes, which go to global memory -- and charges each
+// instruction the bank conflicts of 32 four-byte banks per half wave (weight 4) plus those of 64
+// banks over the whole wave (tie-break).  Results are cached per (precision, N, E, C, LDS room).
+static int bank_cost(int* words, int n, int banks) {
+    // words: the 4-byte LDS words of one group of lanes; cost = most distinct words on one bank
+    std::sort(words, words + n);
     int cnt[64] = {0}, worst = 0;
-    for (size_t i = 0; i < words.size(); ++i)
+    for (int i = 0; i < n; ++i)
         if (i == 0 || words[i] != words[i - 1]) worst = std::max(worst, ++cnt[words[i] % banks]);
     return worst;
 }
@@ -308,7 +311,8 @@ static int bank_cost(std::vector<int>& words, int banks) {
 static int fs_line_stride(Ctx* ctx, Prec pr, const LinePlan& lp, int C, size_t reserve) {
     static std::mutex mu;
     static std::map<std::tuple<int, int, int, int, size_t>, int> cache;
-    const auto key = std::make_tuple((int)pr, lp.N, C, lp.E, ctx->lds_max - std::min(ctx->lds_max, reserve));
+    const auto key = std::make_tuple((int)pr, lp.N, lp.E, C,
+                                     ctx->lds_max - std::min(ctx->lds_max, reserve));
     {
         std::lock_guard<std::mutex> g(mu);
         auto it = cache.find(key);
@@ -316,35 +320,59 @@ static int fs_line_stride(Ctx* ctx, Prec pr, const LinePlan& lp, int C, size_t r
     }
     const int ew = pr == Prec::F32 ? 1 : 2;
     auto lidx = [pr](int e) { return pr == Prec::F32 ? lds_index<float>(e) : lds_index<double>(e); };
-    const int N = lp.N, Tt = lp.Tt;
+    const int N = lp.N, Tt = lp.Tt, E = lp.E, threads = C * Tt;
+    const int waves = 1;
     const int base = lidx(N - 1) + 1;
     int best = base;
     long best_cost = -1;
-    std::vector<int> acc;
+    int wl[64][2], acc[128], half[64];
     for (int Ls = base; Ls < base + 64; ++Ls) {
         if (fsb_tw_offset(ew * 4, C, Ls) + reserve > ctx->lds_max) break;
         long cost = 0;
-        int s = 1;
-        for (int ip = 0; ip < lp.pt.npass; ++ip) {
-            const int R = lp.pt.radix[ip], nb = N / R;
-            for (int pat = 0; pat < 3; ++pat) {  // read j = 0, writes k = 0, 1
-                acc.clear();
-                for (int lane = 0; lane < 64 && lane < C * Tt; ++lane) {
-                    const int c = lane % C, b = lane / C;
-                    if (b >= nb) continue;
-                    int e = b;
-                    if (pat) {
-                        const int p = b / s, q = b - p * s;
-                        e = q + s * R * p + s * (pat - 1);
-                    }
-                    for (int w = 0; w < ew; ++w) acc.push_back((c * Ls + lidx(e)) * ew + w);
-                }
-                if (!acc.empty()) {
-                    cost += bank_cost(acc, 64);
-                    cost += bank_cost(acc, 32);
-                }
+        // one instruction: lane L touches element e(L) of column c(L) (e < 0: inactive)
+        auto charge = [&](const int (*lane_ce)[2]) {
+            int n = 0, nh = 0;
+            for (int L = 0; L < 64; ++L) {
+                if (lane_ce[L][1] < 0) continue;
+                const int w0 = (lane_ce[L][0] * Ls + lidx(lane_ce[L][1])) * ew;
+                for (int x = 0; x < ew; ++x) acc[n++] = w0 + x;
             }
-            s *= R;
+            if (!n) return;
+            for (int h = 0; h < 2; ++h) {
+                nh = 0;
+                for (int L = 32 * h; L < 32 * h + 32; ++L) {
+                    if (lane_ce[L][1] < 0) continue;
+                    const int w0 = (lane_ce[L][0] * Ls + lidx(lane_ce[L][1])) * ew;
+                    for (int x = 0; x < ew; ++x) half[nh++] = w0 + x;
+                }
+                if (nh) cost += 4 * bank_cost(half, nh, 32);
+            }
+            cost += bank_cost(acc, n, 64);
+        };
+        for (int w = 0; w < waves; ++w) {
+            int s = 1;
+            for (int ip = 0; ip < lp.pt.npass; ++ip) {
+                const int R = lp.pt.radix[ip], nb = N / R, MB = std::max(1, E / R);
+                const bool rd = ip > 0, wr = ip + 1 < lp.pt.npass;
+                for (int ib = 0; ib < MB; ++ib)
+                    for (int j = 0; j < R; ++j)
+                        for (int pass_wr = 0; pass_wr < 2; ++pass_wr) {
+                            if (pass_wr ? !wr : !rd) continue;
+                            for (int L = 0; L < 64; ++L) {
+                                const int tid = 64 * w + L, c = tid % C;
+                                const int b = tid / C + ib * Tt;
+                                wl[L][0] = c;
+                                if (tid >= threads || b >= nb) {
+                                    wl[L][1] = -1;
+                                    continue;
+                                }
+                                const int p = b / s, q = b - p * s;
+                                wl[L][1] = pass_wr ? q + s * R * p + s * j : b + j * nb;
+                            }
+                            charge(wl);
+                        }
+                s *= R;
+            }
         }
         if (best_cost < 0 || cost < best_cost) {
             best_cost = cost;
@@ -664,7 +692,8 @@ static std::string describe(const Plan& p) {
             break;
         case Kind::FourStep:
             os << "fourstep N1=" << p.fs.N1 << " " << rad(p.fs.a) << " CA=" << p.fs.CA
-               << " N2=" << p.fs.N2 << " " << rad(p.fs.b) << " CB=" << p.fs.CB;
+               << " Ls=" << p.fs.a.Ls << " N2=" << p.fs.N2 << " " << rad(p.fs.b)
+               << " CB=" << p.fs.CB << " Ls=" << p.fs.b.Ls;
             if (p.fs.chunk) os << " chunk=" << p.fs.chunk;
             if (p.fs.pipe_a || p.fs.pipe_b)
                 os << " pipe=" << (p.fs.pipe_a ? "A" : "") << (p.fs.pipe_b ? "B" : "");

@@ -145,9 +145,10 @@ size_t rows_pipe_lds(Prec p, int C, int Ls, size_t twn);
 int rows_pipe_blocks_per_cu(Prec p, int E, int R0, int block, size_t lds);
 hipError_t launch_fourstep_a(Prec p, const void* in, void* out, const FourStepAArgs& a,
                              const PassTable& pt, const void* tw, hipStream_t st);
-// Persistent pipelined four-step passes (a.grid workgroups of at most kPipeMaxThreads, no
-// epilogue); twn = entries of the line's twiddle table, copied into LDS.  fs_pipe_lds gives the
-// LDS bytes (pass 1 = B: lines, pass twiddles, then the three-level table of t3n entries).
+// Persistent pipelined four-step passes (a.grid workgroups of at most kPipeMaxThreads, lines of
+// at least two passes, no epilogue); twn = entries of the line's twiddle table, copied into LDS.
+// fs_pipe_lds gives the LDS bytes (pass 1 = B: lines, pass twiddles, then the three-level table
+// of t3n entries); fs_pipe_blocks_per_cu the resident workgroups per CU for a first radix R0.
 hipError_t launch_fourstep_a_pipe(Prec p, const void* in, void* out, const FourStepAArgs& a,
                                   const PassTable& pt, const void* tw, size_t twn,
                                   hipStream_t st);
@@ -155,7 +156,7 @@ hipError_t launch_fourstep_b_pipe(Prec p, const void* in, void* out, const FourS
                                   const PassTable& pt, const void* tw, size_t twn,
                                   hipStream_t st);
 size_t fs_pipe_lds(Prec p, int C, int Ls, size_t twn, size_t t3n);
-int fs_pipe_blocks_per_cu(Prec p, int E, int pass, int block, size_t lds);
+int fs_pipe_blocks_per_cu(Prec p, int E, int pass, int R0, int block, size_t lds);
 hipError_t launch_fourstep_b(Prec p, const void* in, void* out, const FourStepBArgs& a,
                              const PassTable& pt, const void* tw, hipStream_t st);
 hipError_t launch_conv_small(Prec p, const void* in, void* out, const ConvSmallArgs& a,

@@ -41,9 +41,9 @@ size_t fs_pipe_lds(Prec p, int C, int Ls, size_t twn, size_t t3n) {
     const size_t t3_off = (tw_off + twn * v + 15) & ~(size_t)15;
     return t3_off + t3n * v;
 }
-int fs_pipe_blocks_per_cu(Prec p, int E, int pass, int block, size_t lds) {
-    return p == Prec::F32 ? fs_pipe_occ_f32(E, pass, block, lds)
-                          : fs_pipe_occ_f64(E, pass, block, lds);
+int fs_pipe_blocks_per_cu(Prec p, int E, int pass, int R0, int block, size_t lds) {
+    return p == Prec::F32 ? fs_pipe_occ_f32(E, pass, R0, block, lds)
+                          : fs_pipe_occ_f64(E, pass, R0, block, lds);
 }
 int rows_pipe_blocks_per_cu(Prec p, int E, int R0, int block, size_t lds) {
     return p == Prec::F32 ? rows_pipe_occ_f32(E, R0, block, lds)

@@ -24,8 +24,9 @@ hipError_t fsb_pipe_f32(const void* in, void* out, const FourStepBArgs& a, const
     return a.E <= 8 ? fsb_pipe_t<float, 8>(in, out, a, pt, tw, twn, st)
                     : fsb_pipe_t<float, 16>(in, out, a, pt, tw, twn, st);
 }
-int fs_pipe_occ_f32(int E, int pass, int block, size_t lds) {
-    return E <= 8 ? fs_pipe_occ_t<float, 8>(pass, block, lds) : fs_pipe_occ_t<float, 16>(pass, block, lds);
+int fs_pipe_occ_f32(int E, int pass, int R0, int block, size_t lds) {
+    return E <= 8 ? fs_pipe_occ_t<float, 8>(pass, R0, block, lds)
+                  : fs_pipe_occ_t<float, 16>(pass, R0, block, lds);
 }
 void init_kernels_pipe_f32() {
     rows_pipe_init_t<float, 8>();

@@ -54,8 +54,8 @@ hipError_t fsb_pipe_f32(const void* in, void* out, const FourStepBArgs& a, const
                         const void* tw, size_t twn, hipStream_t st);
 hipError_t fsb_pipe_f64(const void* in, void* out, const FourStepBArgs& a, const PassTable& pt,
                         const void* tw, size_t twn, hipStream_t st);
-int fs_pipe_occ_f32(int E, int pass, int block, size_t lds);
-int fs_pipe_occ_f64(int E, int pass, int block, size_t lds);
+int fs_pipe_occ_f32(int E, int pass, int R0, int block, size_t lds);
+int fs_pipe_occ_f64(int E, int pass, int R0, int block, size_t lds);
 void init_kernels_pipe_f32();
 void init_kernels_pipe_f64();
 

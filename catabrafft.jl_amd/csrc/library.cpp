@@ -631,12 +631,12 @@ static void fs_pipe_setup(Ctx* ctx, Plan& p) {
     for (int pass = 0; pass < 2; ++pass) {
         const LinePlan& lp = pass ? fs.b : fs.a;
         const int C = pass ? fs.CB : fs.CA;
-        if (C * lp.Tt > kPipeMaxThreads || (int64_t)C * lp.N > 16 * (int64_t)C * lp.Tt) continue;
+        if (C * lp.Tt > kPipeMaxThreads || lp.pt.npass < 2) continue;
         const size_t twn = lp.dtw.bytes / (size_t)esize(p.prec);
         const size_t t3n = pass ? (size_t)tw3_entries(fs.tbits, fs.tnc) : 0;
         const size_t lds = fs_pipe_lds(p.prec, C, lp.Ls, twn, t3n);
         if (lds > ctx->lds_max) continue;
-        const int bpc = fs_pipe_blocks_per_cu(p.prec, lp.E, pass, C * lp.Tt, lds);
+        const int bpc = fs_pipe_blocks_per_cu(p.prec, lp.E, pass, lp.pt.radix[0], C * lp.Tt, lds);
         (pass ? fs.pipe_b : fs.pipe_a) = bpc > 0 ? bpc * ctx->num_cu : 0;
     }
 }

@@ -121,11 +121,21 @@ k_rows_pipe(const V_t<T>* __restrict__ in, V_t<T>* __restrict__ out, RowsArgs a,
 
 // ------------------------------------------------------------------------------------------
 // Pipelined four-step passes.  A workgroup loops over tiles (pass A: C columns of one transform;
-// pass B: C rows of the middle); the next tile's staged elements are loaded into registers while
-// the current tile runs its passes and stores.  Same staged layouts, passes and twiddle values
-// as k_fs_a / k_fs_b, hence the same results.
+// pass B: C rows of the middle) with the plain kernels' column-first lanes (lane = c + C t).  The
+// first Stockham pass's inputs of the NEXT tile are loaded straight into registers while the
+// current tile runs its remaining passes, so each CU keeps reads in flight through the whole pass
+// sequence.  The pass twiddles (and pass B's three-level table) live in LDS: vmcnt counts in
+// order, so a global twiddle load issued after the prefetch would wait for it.  Same passes,
+// twiddle values and operation order as k_fs_a / k_fs_b, hence the same bits.
+//
+// Tile order: iteration i of workgroup g runs tile i G + (g mod 8) (G / 8) + g / 8 (G = gridDim.x,
+// a multiple of 8; else i G + g), so the tiles in flight on one XCD at a time are consecutive.
 // ------------------------------------------------------------------------------------------
-constexpr int kPipeStage = 16;  // staged elements per thread (C * N / threads <= 16)
+__device__ __forceinline__ int64_t pipe_tile(int64_t i, int remap) {
+    const int g = blockIdx.x, G = gridDim.x;
+    const int gg = (!remap || (G & 7)) ? g : (g & 7) * (G >> 3) + (g >> 3);
+    return i * G + gg;
+}
 
 // LDS offset of pass B's three-level twiddle table: after the lines and the pass twiddles.
 template <typename T>
@@ -133,144 +143,124 @@ __host__ __device__ constexpr size_t pipe_t3_offset(int C, int Ls, int twn) {
     return (pipe_tw_offset<T>(C, Ls) + (size_t)twn * sizeof(V_t<T>) + 15) & ~(size_t)15;
 }
 
-template <typename T, int E>
+template <typename T, int E, int R0>
 __global__ void __launch_bounds__(kPipeMaxThreads)
 k_fs_a_pipe(const V_t<T>* __restrict__ in, V_t<T>* __restrict__ out, FourStepAArgs a,
             PassTable pt, const V_t<T>* __restrict__ tw) {
     CFFT_DYN_LDS(smem);
     using V = V_t<T>;
+    constexpr int MB = bfly_per_thread(E, R0);
     const int C = a.C, N1 = a.N1, N2 = a.N2, Tt = a.Tt;
     const int cshift = __ffs(C) - 1;
-    const int total = C * N1;
+    const int tid = threadIdx.x;
+    const int c = tid & (C - 1);
+    const int t = tid >> cshift;
     const int tiles = (N2 + C - 1) / C;
     const int64_t ntiles = (int64_t)tiles * a.batch;
+    const int nb0 = N1 / R0;
     V* twl = reinterpret_cast<V*>(smem + pipe_tw_offset<T>(C, a.Ls));
-    V x[kPipeStage];
-    // this thread's staged elements of tile q: idx = tid + u * blockDim (row e, column c)
+    V x[MB][R0];
+    // this thread's first-pass inputs of tile q: x[ib][j] = column j2 element b + j nb0
     auto load_tile = [&](int64_t q) {
         const int64_t bt = q / tiles;
         const int j20 = (int)(q - bt * tiles) * C;
-        const V* src = in + bt * a.idist;
+        const int j2 = j20 + c < N2 ? j20 + c : N2 - 1;
+        const V* src = in + bt * a.idist + j2;
 #pragma unroll
-        for (int u = 0; u < kPipeStage; ++u) {
-            const int idx = threadIdx.x + u * blockDim.x;
-            if (idx < total) {
-                const int c = idx & (C - 1);
-                const int e = idx >> cshift;
-                const int j2 = j20 + c < N2 ? j20 + c : N2 - 1;
-                x[u] = src[(int64_t)e * N2 + j2];
-            }
+        for (int ib = 0; ib < MB; ++ib) {
+            const int bt_ = t + ib * Tt;
+            const int b = bt_ < nb0 ? bt_ : nb0 - 1;
+#pragma unroll
+            for (int j = 0; j < R0; ++j) x[ib][j] = src[(int64_t)(b + j * nb0) * N2];
         }
     };
-    int64_t q = blockIdx.x;
+    int64_t i = 0;
+    int64_t q = pipe_tile(0, a.xcd);
     if (q < ntiles) load_tile(q);
     stage_twiddles<T>(pt, N1, tw, twl);
-    const int tid = threadIdx.x;
-    const int c = tid / Tt;
-    const int t = tid - c * Tt;
+    __syncthreads();  // the twiddle copy is complete
     const LdsLine<T> L = lds_line<T>(smem, C, a.Ls, c);
     const TwLds<V> twt{twl};
     const T sg = (T)a.sgn_in;
-    for (; q < ntiles; q += gridDim.x) {
+    while (q < ntiles) {
         const int64_t bt = q / tiles;
-        const int j20 = (int)(q - bt * tiles) * C;
-#pragma unroll
-        for (int u = 0; u < kPipeStage; ++u) {
-            const int idx = threadIdx.x + u * blockDim.x;
-            if (idx < total) {
-                const int cc = idx & (C - 1);
-                V v = x[u];
-                if (j20 + cc >= N2) v = V{0, 0};
-                v.y *= sg;
-                lds_line<T>(smem, C, a.Ls, cc).put(idx >> cshift, v);
-            }
-        }
-        __syncthreads();
-        if (q + gridDim.x < ntiles) load_tile(q + gridDim.x);
-        const int j2 = j20 + c;
+        const int j2 = (int)(q - bt * tiles) * C + c;
         const bool active = j2 < N2;
-        GPlainDst<T> d{out + bt * a.odist + (int64_t)(active ? j2 : 0) * N1};
-        run_line_t<T, E>(pt, N1, t, Tt, active, twt, L, d, L, false, true, false, 0, 1);
+        first_pass_regs<T, R0, MB>(N1, t, Tt, active, twt.pass(pt.twoff[0], nb0), x, sg, L);
         __syncthreads();
+        const int64_t qn = pipe_tile(++i, a.xcd);
+        if (qn < ntiles) load_tile(qn);
+        GPlainDst<T> d{out + bt * a.odist + (int64_t)(active ? j2 : 0) * N1};
+        run_line_t<T, E>(pt, N1, t, Tt, active, twt, L, d, L, false, true, false, 1, R0);
+        __syncthreads();  // the last pass's LDS reads precede the next tile's first pass
+        q = qn;
     }
 }
 
-template <typename T, int E>
+template <typename T, int E, int R0>
 __global__ void __launch_bounds__(kPipeMaxThreads)
 k_fs_b_pipe(const V_t<T>* in, V_t<T>* out, FourStepBArgs a, PassTable pt,
             const V_t<T>* __restrict__ tw, int twn) {
     CFFT_DYN_LDS(smem);
     using V = V_t<T>;
+    constexpr int MB = bfly_per_thread(E, R0);
     const int C = a.C, N1 = a.N1, N2 = a.N2, Tt = a.Tt;
     const int cshift = __ffs(C) - 1;
-    const int total = C * N2;
+    const int tid = threadIdx.x;
+    const int c = tid & (C - 1);
+    const int t = tid >> cshift;
     const int tiles = (N1 + C - 1) / C;
     const int64_t ntiles = (int64_t)tiles * a.batch;
+    const int nb0 = N2 / R0;
     V* twl = reinterpret_cast<V*>(smem + pipe_tw_offset<T>(C, a.Ls));
     V* t3 = reinterpret_cast<V*>(smem + pipe_t3_offset<T>(C, a.Ls, twn));
     const V* tw3 = (const V*)a.tw3;
-    V x[kPipeStage];
-    // this thread's staged elements of tile q: runs of C lines (k1) per middle row j2 = e
+    V x[MB][R0];
+    // this thread's first-pass inputs of tile q: x[ib][j] = middle row b + j nb0, column k1
     auto load_tile = [&](int64_t q) {
         const int64_t bt = q / tiles;
         const int k10 = (int)(q - bt * tiles) * C;
-        const V* src = in + bt * a.idist;
+        const int k1 = k10 + c < N1 ? k10 + c : N1 - 1;
+        const V* src = in + bt * a.idist + k1;
 #pragma unroll
-        for (int u = 0; u < kPipeStage; ++u) {
-            const int idx = threadIdx.x + u * blockDim.x;
-            if (idx < total) {
-                const int cc = idx & (C - 1);
-                const int e = idx >> cshift;
-                const int k1 = k10 + cc < N1 ? k10 + cc : N1 - 1;
-                x[u] = src[(int64_t)e * N1 + k1];
-            }
+        for (int ib = 0; ib < MB; ++ib) {
+            const int bt_ = t + ib * Tt;
+            const int b = bt_ < nb0 ? bt_ : nb0 - 1;
+#pragma unroll
+            for (int j = 0; j < R0; ++j) x[ib][j] = src[(int64_t)(b + j * nb0) * N1];
         }
     };
-    int64_t q = blockIdx.x;
+    int64_t i = 0;
+    int64_t q = pipe_tile(0, a.xcd);
     if (q < ntiles) load_tile(q);
     stage_twiddles<T>(pt, N2, tw, twl);
     if (tw3) stage_table<T>(tw3, t3, tw3_entries(a.tbits, a.tnc));
-    const int tid = threadIdx.x;
-    const int c = tid / Tt;
-    const int t = tid - c * Tt;
+    __syncthreads();  // the tables are in LDS
     const LdsLine<T> L = lds_line<T>(smem, C, a.Ls, c);
     const TwLds<V> twt{twl};
-    const T sgi = (T)a.sgn_in;
+    const T sg = (T)a.sgn_in;
     const T sx = (T)a.scale, sy = (T)(a.scale * a.sgn_out);
-    __syncthreads();  // the tables are in LDS
-    for (; q < ntiles; q += gridDim.x) {
+    while (q < ntiles) {
         const int64_t bt = q / tiles;
-        const int k10 = (int)(q - bt * tiles) * C;
+        const int k1 = (int)(q - bt * tiles) * C + c;
+        const bool active = k1 < N1;
+        if (tw3) {  // the four-step twiddle w_n^{e k1}, before the conjugation in the first pass
 #pragma unroll
-        for (int u = 0; u < kPipeStage; ++u) {
-            const int idx = threadIdx.x + u * blockDim.x;
-            if (idx < total) {
-                const int cc = idx & (C - 1);
-                const int e = idx >> cshift;
-                V v = x[u];
-                if (tw3) v = cmul(v, tw3_get<T>(t3, a.tbits, e * (k10 + cc)));
-                if (k10 + cc >= N1) v = V{0, 0};
-                v.y *= sgi;
-                lds_line<T>(smem, C, a.Ls, cc).put(e, v);
+            for (int ib = 0; ib < MB; ++ib) {
+                const int b = t + ib * Tt;
+#pragma unroll
+                for (int j = 0; j < R0; ++j)
+                    x[ib][j] = cmul(x[ib][j], tw3_get<T>(t3, a.tbits, (b + j * nb0) * k1));
             }
         }
+        first_pass_regs<T, R0, MB>(N2, t, Tt, active, twt.pass(pt.twoff[0], nb0), x, sg, L);
         __syncthreads();
-        if (q + gridDim.x < ntiles) load_tile(q + gridDim.x);
-        const bool active = k10 + c < N1;
-        run_line_t<T, E>(pt, N2, t, Tt, active, twt, L, L, L, false, false, false, 0, 1);
-        // run_line_t ended with a barrier: the lines' outputs are in LDS.  Staged store in runs
-        // of C (k1) per output row k2.
-        V* dst = out + bt * a.odist;
-        for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-            const int cc = idx & (C - 1);
-            const int e = idx >> cshift;
-            const int k1 = k10 + cc;
-            if (k1 < N1) {
-                const V v = lds_line<T>(smem, C, a.Ls, cc)(e);
-                dst[(int64_t)k1 + (int64_t)N1 * e] = V{v.x * sx, v.y * sy};
-            }
-        }
+        const int64_t qn = pipe_tile(++i, a.xcd);
+        if (qn < ntiles) load_tile(qn);
+        GStrideDst<T> d{out + bt * a.odist + (active ? k1 : 0), N1, sx, sy};
+        run_line_t<T, E>(pt, N2, t, Tt, active, twt, L, d, L, false, true, false, 1, R0);
         __syncthreads();
+        q = qn;
     }
 }
 
@@ -296,6 +286,8 @@ int rows_pipe_occ_r(int block, size_t lds) {
 template <typename T, int E, int R0>
 void rows_pipe_init_r() {
     allow_big_lds(k_rows_pipe<T, E, R0>);
+    allow_big_lds(k_fs_a_pipe<T, E, R0>);
+    allow_big_lds(k_fs_b_pipe<T, E, R0>);
 }
 
 // Dispatch of a pipelined rows launch on its first radix.
@@ -332,13 +324,29 @@ int rows_pipe_occ_t(int R0, int block, size_t lds) {
     return 0;
 }
 
+template <typename T, int E, int R0>
+hipError_t fsa_pipe_r(const void* in, void* out, const FourStepAArgs& a, const PassTable& pt,
+                      const void* tw, size_t lds, hipStream_t st) {
+    hipLaunchKernelGGL((k_fs_a_pipe<T, E, R0>), dim3((unsigned)a.grid), dim3(a.C * a.Tt), lds, st,
+                       (const V_t<T>*)in, (V_t<T>*)out, a, pt, (const V_t<T>*)tw);
+    return hipGetLastError();
+}
+template <typename T, int E, int R0>
+hipError_t fsb_pipe_r(const void* in, void* out, const FourStepBArgs& a, const PassTable& pt,
+                      const void* tw, size_t twn, size_t lds, hipStream_t st) {
+    hipLaunchKernelGGL((k_fs_b_pipe<T, E, R0>), dim3((unsigned)a.grid), dim3(a.C * a.Tt), lds, st,
+                       (const V_t<T>*)in, (V_t<T>*)out, a, pt, (const V_t<T>*)tw, (int)twn);
+    return hipGetLastError();
+}
+
 template <typename T, int E>
 hipError_t fsa_pipe_t(const void* in, void* out, const FourStepAArgs& a, const PassTable& pt,
                       const void* tw, size_t twn, hipStream_t st) {
     const size_t lds = pipe_tw_offset<T>(a.C, a.Ls) + twn * sizeof(V_t<T>);
-    hipLaunchKernelGGL((k_fs_a_pipe<T, E>), dim3((unsigned)a.grid), dim3(a.C * a.Tt), lds, st,
-                       (const V_t<T>*)in, (V_t<T>*)out, a, pt, (const V_t<T>*)tw);
-    return hipGetLastError();
+#define CFFT_PIPE_CALL(RR) fsa_pipe_r<T, E, RR>(in, out, a, pt, tw, lds, st)
+    CFFT_PIPE_R0_SWITCH(pt.radix[0], CFFT_PIPE_CALL)
+#undef CFFT_PIPE_CALL
+    return hipErrorInvalidValue;
 }
 
 template <typename T, int E>
@@ -346,24 +354,31 @@ hipError_t fsb_pipe_t(const void* in, void* out, const FourStepBArgs& a, const P
                       const void* tw, size_t twn, hipStream_t st) {
     const size_t lds = pipe_t3_offset<T>(a.C, a.Ls, (int)twn) +
                        (a.tw3 ? (size_t)tw3_entries(a.tbits, a.tnc) * sizeof(V_t<T>) : 0);
-    hipLaunchKernelGGL((k_fs_b_pipe<T, E>), dim3((unsigned)a.grid), dim3(a.C * a.Tt), lds, st,
-                       (const V_t<T>*)in, (V_t<T>*)out, a, pt, (const V_t<T>*)tw, (int)twn);
-    return hipGetLastError();
+#define CFFT_PIPE_CALL(RR) fsb_pipe_r<T, E, RR>(in, out, a, pt, tw, twn, lds, st)
+    CFFT_PIPE_R0_SWITCH(pt.radix[0], CFFT_PIPE_CALL)
+#undef CFFT_PIPE_CALL
+    return hipErrorInvalidValue;
 }
 
-template <typename T, int E>
-int fs_pipe_occ_t(int pass, int block, size_t lds) {
+template <typename T, int E, int R0>
+int fs_pipe_occ_r(int pass, int block, size_t lds) {
     int nb = 0;
-    const void* k = pass ? reinterpret_cast<const void*>(k_fs_b_pipe<T, E>)
-                         : reinterpret_cast<const void*>(k_fs_a_pipe<T, E>);
+    const void* k = pass ? reinterpret_cast<const void*>(k_fs_b_pipe<T, E, R0>)
+                         : reinterpret_cast<const void*>(k_fs_a_pipe<T, E, R0>);
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, block, lds) != hipSuccess) return 0;
     return nb;
 }
 
 template <typename T, int E>
+int fs_pipe_occ_t(int pass, int R0, int block, size_t lds) {
+#define CFFT_PIPE_CALL(RR) fs_pipe_occ_r<T, E, RR>(pass, block, lds)
+    CFFT_PIPE_R0_SWITCH(R0, CFFT_PIPE_CALL)
+#undef CFFT_PIPE_CALL
+    return 0;
+}
+
+template <typename T, int E>
 void rows_pipe_init_t() {
-    allow_big_lds(k_fs_a_pipe<T, E>);
-    allow_big_lds(k_fs_b_pipe<T, E>);
     rows_pipe_init_r<T, E, 16>();
     rows_pipe_init_r<T, E, 13>();
     rows_pipe_init_r<T, E, 11>();

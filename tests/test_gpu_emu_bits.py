@@ -69,9 +69,12 @@ def test_gpu_bits_equal_cpu_emulation(gpu_ready, emu, n, batch, dt):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("n,batch,dt", [(2520, 9, np.complex64), (1024, 5, np.complex128),
-                                        (64, 300, np.complex64), (720, 3, np.complex64)])
-def test_gpu_pipelined_rows_bits_equal_cpu_emulation(gpu_ready, emu, tmp_path, n, batch, dt):
-    """The persistent pipelined rows kernel (chosen through wisdom) gives the emulation's bits."""
+                                        (64, 300, np.complex64), (720, 3, np.complex64),
+                                        (1 << 16, 3, np.complex64), (1 << 17, 1, np.complex128),
+                                        (3 << 15, 2, np.complex64)])
+def test_gpu_pipelined_bits_equal_cpu_emulation(gpu_ready, emu, tmp_path, n, batch, dt):
+    """The persistent pipelined kernels (rows, four-step passes A and B; chosen through wisdom)
+    give the bits of the emulation's plain kernels."""
     import torch
 
     import catabrafft as C

@@ -235,14 +235,17 @@ inline int pass_table_rmax(const PassTable& pt) {
 // when from_global (to_global) is false the line's input is already in LDS (its output is left
 // in LDS).  dst_aliases_lds asks for a barrier between the reads and writes of the last pass
 // because GDST writes the line's own LDS.  The caller provides the barrier before the first pass
-// when it staged the input into LDS.  TWT is a twiddle table (TwGlobal or TwLds).
+// when it staged the input into LDS.  src_aliases_lds asks for a barrier between the reads and
+// writes of the first pass because GSRC reads the line's own LDS.  TWT is a twiddle table
+// (TwGlobal or TwLds).
 template <typename T, int E, int RM = 16, class TWT, class GSRC, class GDST>
 __device__ __forceinline__ void run_line_t(const PassTable& pt, const int N, const int t,
                                            const int Tt, const bool active, const TWT& twt,
                                            const GSRC& gsrc, const GDST& gdst,
                                            const LdsLine<T>& lds, const bool from_global,
                                            const bool to_global, const bool dst_aliases_lds,
-                                           const int ip0, const int s0) {
+                                           const int ip0, const int s0,
+                                           const bool src_aliases_lds = false) {
     int s = s0;
     for (int ip = ip0; ip < pt.npass; ++ip) {
         const int R = pt.radix[ip];
@@ -251,11 +254,11 @@ __device__ __forceinline__ void run_line_t(const PassTable& pt, const int N, con
         const bool sg = first && from_global;
         const bool dg = last && to_global;
         if (sg && dg) {
-#define CFFT_CALL_GG(RR) stockham_pass<T, RR, bfly_per_thread(E, RR)>(N, s, t, Tt, active, twa, gsrc, gdst, dst_aliases_lds)
+#define CFFT_CALL_GG(RR) stockham_pass<T, RR, bfly_per_thread(E, RR)>(N, s, t, Tt, active, twa, gsrc, gdst, dst_aliases_lds || src_aliases_lds)
             CFFT_RADIX_SWITCH(RM, R, CFFT_CALL_GG)
 #undef CFFT_CALL_GG
         } else if (sg) {
-#define CFFT_CALL_GL(RR) stockham_pass<T, RR, bfly_per_thread(E, RR)>(N, s, t, Tt, active, twa, gsrc, lds, false)
+#define CFFT_CALL_GL(RR) stockham_pass<T, RR, bfly_per_thread(E, RR)>(N, s, t, Tt, active, twa, gsrc, lds, src_aliases_lds)
             CFFT_RADIX_SWITCH(RM, R, CFFT_CALL_GL)
 #undef CFFT_CALL_GL
         } else if (dg) {

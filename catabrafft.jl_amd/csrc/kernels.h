@@ -67,7 +67,7 @@ __host__ __device__ constexpr size_t fsb_tw_offset(size_t elem, int C, int Ls) {
 // and one HBM write per transform.
 enum class ConvMode : int { Rader = 0, Bluestein = 1 };
 struct ConvSmallArgs {
-    int n, L, C, Tt, LsX, LsA, E;
+    int n, L, C, Tt, Ls, E;  // Ls: line stride of the LDS region the input and the work share
     int mode;              // ConvMode
     int64_t nlines, idist, odist;
     int sgn_in, sgn_out;

@@ -390,7 +390,10 @@ k_fs_b(const V_t<T>* in, GO* out, FourStepBArgs a, PassTable pt,
 
 // ------------------------------------------------------------------------------------------
 // conv body: Rader / Bluestein, whole transform in LDS.
-// LDS: X (input, then output; C lines of LsX) | A (convolution work, C lines of LsA) | dc[C]
+// LDS: C lines of stride Ls, shared by the input X (then the output) and the convolution work A
+// (element e of both at the same word) | dc[C].  x0 is read before the forward FFT_L overwrites
+// X; the first pass of that FFT (gather from X, writes to A) and the last pass of the second
+// (reads A, scatters into X) separate their reads from their writes with a barrier.
 // ------------------------------------------------------------------------------------------
 template <typename T>
 struct GatherSrc {   // a[e] from the staged input X
@@ -439,17 +442,13 @@ struct ScatterDst {  // inverse result c = conj(v)/L into the output X
 
 template <typename T, int E, class LINES>
 __device__ __forceinline__ void conv_body(unsigned char* smem, const LINES& lines, int nl, int n,
-                                          int L, int C, int Tt, int LsX, int LsA, int mode,
-                                          int sgn, double scale, const int* perm,
-                                          const int* iperm, const V_t<T>* chirp,
-                                          const V_t<T>* K, const PassTable& pt,
-                                          const V_t<T>* tw) {
-    T* base = reinterpret_cast<T*>(smem);
-    T* xre = base;
-    T* xim = xre + C * LsX;
-    T* are = xim + C * LsX;
-    T* aim = are + C * LsA;
-    V_t<T>* sdc = reinterpret_cast<V_t<T>*>(aim + C * LsA);
+                                          int L, int C, int Tt, int Ls, int mode, int sgn,
+                                          double scale, const int* perm, const int* iperm,
+                                          const V_t<T>* chirp, const V_t<T>* K,
+                                          const PassTable& pt, const V_t<T>* tw) {
+    T* xre = reinterpret_cast<T*>(smem);
+    T* xim = xre + C * Ls;
+    V_t<T>* sdc = reinterpret_cast<V_t<T>*>(xim + C * Ls);
     const T sg = (T)sgn;
     // 1. staged, coalesced load of the nl input lines (flat over lines x elements)
     batched_for<kStageLoadU>(
@@ -461,7 +460,7 @@ __device__ __forceinline__ void conv_body(unsigned char* smem, const LINES& line
         [&](int idx, V_t<T> v) {
             const int c = idx / n;
             v.y *= sg;
-            LdsLine<T>{xre + c * LsX, xim + c * LsX}.put(idx - c * n, v);
+            LdsLine<T>{xre + c * Ls, xim + c * Ls}.put(idx - c * n, v);
         });
     __syncthreads();
     const int tid = threadIdx.x;
@@ -469,17 +468,18 @@ __device__ __forceinline__ void conv_body(unsigned char* smem, const LINES& line
     const int t = tid - c * Tt;
     const bool active = (c < C) && (c < nl);
     const int cc = c < C ? c : 0;
-    const LdsLine<T> X{xre + cc * LsX, xim + cc * LsX};
-    const LdsLine<T> A{are + cc * LsA, aim + cc * LsA};
+    const LdsLine<T> X{xre + cc * Ls, xim + cc * Ls};
+    const LdsLine<T> A = X;
+    const V_t<T> x0 = X(0);
     // 2. forward FFT_L of the gathered / chirped sequence; pointwise product with K on the way out
     GatherSrc<T> gsrc{X, perm, chirp, n, mode};
     ConvEpiDst<T> edst{A, K, sdc + cc};
-    run_line<T, E>(pt, L, t, Tt, active, tw, gsrc, edst, A, true, true, true);
+    run_line_t<T, E>(pt, L, t, Tt, active, TwGlobal<V_t<T>>{tw}, gsrc, edst, A, true, true, true,
+                     0, 1, true);
     __syncthreads();
     // 3. second forward FFT_L of the conjugated product = conj of the inverse; scatter into X
-    const V_t<T> x0 = X(0);
     ScatterDst<T> sdst{X, iperm, chirp, x0, (T)(1.0 / L), n, mode};
-    run_line<T, E>(pt, L, t, Tt, active, tw, A, sdst, A, false, true);
+    run_line<T, E>(pt, L, t, Tt, active, tw, A, sdst, A, false, true, true);
     __syncthreads();
     if (mode == (int)ConvMode::Rader && active && t == 0) {
         const V_t<T> d = sdc[cc];
@@ -492,7 +492,7 @@ __device__ __forceinline__ void conv_body(unsigned char* smem, const LINES& line
         threadIdx.x, blockDim.x, nl * n,
         [&](int idx) {
             const int c2 = idx / n;
-            return LdsLine<T>{xre + c2 * LsX, xim + c2 * LsX}(idx - c2 * n);
+            return LdsLine<T>{xre + c2 * Ls, xim + c2 * Ls}(idx - c2 * n);
         },
         [&](int idx, V_t<T> v) {
             const int c2 = idx / n;
@@ -508,7 +508,7 @@ k_conv_small(const V_t<T>* __restrict__ in, V_t<T>* __restrict__ out, ConvSmallA
     const int64_t line0 = (int64_t)blockIdx.x * a.C;
     const int nl = (int)((a.nlines - line0) < a.C ? (a.nlines - line0) : a.C);
     UniformLines<T> lines{in + line0 * a.idist, out + line0 * a.odist, a.idist, a.odist, nullptr};
-    conv_body<T, E>(smem, lines, nl, a.n, a.L, a.C, a.Tt, a.LsX, a.LsA, a.mode, a.sgn_in,
+    conv_body<T, E>(smem, lines, nl, a.n, a.L, a.C, a.Tt, a.Ls, a.mode, a.sgn_in,
                     a.scale, a.perm, a.iperm, (const V_t<T>*)a.chirp, (const V_t<T>*)a.K, pt, tw);
 }
 
@@ -861,7 +861,7 @@ k_conv_multi(MultiArgs ma, const MItem* __restrict__ items, const MPlan* __restr
     double scale;
     item_direction(it, P->n, ma, sgn, scale);
     ItemLines<T> lines{ma.in_base, ma.out_base, &it};
-    conv_body<T, kMultiE>(smem, lines, it.count, P->n, P->L, P->C, P->line.Tt, P->LsX, P->LsA,
+    conv_body<T, kMultiE>(smem, lines, it.count, P->n, P->L, P->C, P->line.Tt, P->Ls,
                           P->mode, sgn, scale, P->perm, P->iperm, (const V_t<T>*)P->chirp,
                           (const V_t<T>*)P->K, P->line.pt, (const V_t<T>*)P->line.tw);
 }
@@ -961,8 +961,7 @@ hipError_t fsb_t(const void* in, void* out, const FourStepBArgs& a, const PassTa
 template <typename T, int E>
 hipError_t conv_t(const void* in, void* out, const ConvSmallArgs& a, const PassTable& pt,
                   const void* tw, hipStream_t st) {
-    const size_t lds =
-        2 * sizeof(T) * (size_t)a.C * (a.LsX + a.LsA) + sizeof(V_t<T>) * (size_t)a.C;
+    const size_t lds = 2 * sizeof(T) * (size_t)a.C * a.Ls + sizeof(V_t<T>) * (size_t)a.C;
     const int64_t blocks = (a.nlines + a.C - 1) / a.C;
     hipLaunchKernelGGL((k_conv_small<T, E>), dim3((unsigned)blocks), dim3(a.C * a.Tt), lds, st,
                        (const V_t<T>*)in, (V_t<T>*)out, a, pt, (const V_t<T>*)tw);

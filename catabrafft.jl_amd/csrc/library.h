@@ -71,7 +71,7 @@ struct Plan {
     // convolution (Rader / Bluestein)
     ConvMode mode = ConvMode::Rader;
     int64_t L = 0, g = 0;
-    int LsX = 0, LsA = 0;
+    int LsC = 0;  // fused convolution: LDS line stride (input and work lines share it)
     Prec cprec = Prec::F32;
     DevBuf perm, iperm, chirp, K, dc, w1, w2;
     std::unique_ptr<Plan> sub;

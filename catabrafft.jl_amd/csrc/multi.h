@@ -28,7 +28,7 @@ struct MPlan {
     MLine la, lb;
     const void* tw3;            // pass B's three-level twiddle table (see FourStepBArgs)
     // convolution (Rader / Bluestein)
-    int mode, L, LsX, LsA;
+    int mode, L, Ls;            // Ls: LDS line stride (input and convolution work share it)
     const int* perm;
     const int* iperm;
     const void* chirp;

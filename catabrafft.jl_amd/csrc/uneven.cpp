@@ -118,8 +118,7 @@ PlanEntry WorkList::add_plan(const Plan& p) {
             m.line = make_mline(p.line);
             m.mode = (int)p.mode;
             m.L = (int)p.L;
-            m.LsX = p.LsX;
-            m.LsA = p.LsA;
+            m.Ls = p.LsC;
             m.perm = (const int*)p.perm.p;
             m.iperm = (const int*)p.iperm.p;
             m.chirp = p.chirp.p;
@@ -223,7 +222,7 @@ void WorkList::add_transforms(const PlanEntry& e, const std::vector<XformRef>& x
         }
         case Kind::ConvSmall: {
             const int block = p.C * p.line.Tt;
-            const size_t lds = 2 * (esz(pr) / 2) * (size_t)p.C * (p.LsX + p.LsA) + esz(pr) * p.C;
+            const size_t lds = 2 * (esz(pr) / 2) * (size_t)p.C * p.LsC + esz(pr) * p.C;
             add_lines(1, MultiKernel::ConvSmall, p, e.mplan, pr, p.C, block, lds, xs,
                       4.0 * p.L * std::log2((double)p.L), false, 0);
             break;

@@ -27,7 +27,7 @@ static std::vector<PlanOptions> candidates(Ctx* ctx, const Plan& p, int flags) {
     std::vector<PlanOptions> out;
     const bool enchant = flags >= CFFT_ENCHANT;
     const std::vector<int> Es = enchant ? std::vector<int>{8, 16} : std::vector<int>{16};
-    const std::vector<int> tts = {128, 256, 512};
+    const std::vector<int> tts = {64, 128, 256, 512};
     PlanOptions base;
     base.explicit_opts = true;
     if (p.n == 1) {

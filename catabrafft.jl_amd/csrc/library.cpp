@@ -572,10 +572,9 @@ static int plan_conv(Ctx* ctx, Plan& p, const PlanOptions& o) {
         const size_t per = 2 * (size_t)(esize(p.prec) / 2) * p.LsC + esize(p.prec);
         if (per <= ctx->lds_max) {
             small = true;
-            // small workgroups (64 target threads) by default: more of them share a CU, which
-            // measured 13-14 % faster than 256 for p = 17 and 257 on MI355X
-            const int tt = o.explicit_opts ? o.target_threads : 64;
-            int c = std::max(1, tt / p.line.Tt);
+            // small workgroups by default (PlanOptions::target_threads = 64): more of them share a
+            // CU, which measured 13-14 % faster than 256 for p = 17 and 257 on MI355X
+            int c = std::max(1, o.target_threads / p.line.Tt);
             c = std::min<int64_t>(c, std::max<size_t>(1, std::max(ctx->lds_rows_budget, per) / per));
             c = std::min<int64_t>(c, std::max<int64_t>(1, p.batch));
             c = std::min(c, std::max(1, 1024 / p.line.Tt));

@@ -102,7 +102,8 @@ struct ExecShape {
 // Planner knobs; the autotuner enumerates them, NO_FLAG uses the defaults.
 struct PlanOptions {
     int E = 0;                 // per-thread complex capacity (8 or 16), 0 = default
-    int target_threads = 256;  // threads per workgroup the rows kernels aim for
+    int target_threads = 64;   // threads per workgroup the rows kernels aim for (short lines
+                               // share a workgroup up to this; small workgroups measured fastest)
     int64_t n1 = 0;            // four-step split (0 = balanced / direct when it fits)
     int ca = 0, cb = 0;        // four-step column tiles (0 = default)
     int64_t chunk = -1;        // four-step transforms per round (-1 = default, 0 = whole batch)

@@ -291,3 +291,22 @@ def test_gpu_f16_bits_equal_cpu_emulation(gpu_ready):
         torch.cuda.synchronize()
         assert np.array_equal(yd.cpu().numpy().view(np.uint16), ye.view(np.uint16)), n
     lib.cfft_finalize(ctx)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [89, 1 << 10, 10 ** 4])
+def test_gpu_f16_host_arrays(gpu_ready, n):
+    """numpy ComplexF16 arrays (catabrafft.complex32) through fft / ifft: the host-pointer path
+    (cfft_exec_host) the Julia shim uses; the reference's round trip and the oracle."""
+    import catabrafft as C
+
+    x = C.to_complex32(seeded_complex(n, seed=2 * n))
+    y = C.fft(x)
+    z = C.ifft(y)
+    assert y.dtype == C.complex32 and z.dtype == C.complex32
+    xw = C.from_complex32(x)
+    assert roundtrip_ok(xw, C.from_complex32(z))
+    ref = O.fft(xw.astype(np.complex128), "d", -1, False)
+    assert rel_l2(C.from_complex32(y), ref) < 2e-3
+    p = C.plan_fft(x)
+    assert np.array_equal((p * x).view(np.uint16), y.view(np.uint16))

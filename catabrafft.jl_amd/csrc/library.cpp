@@ -728,10 +728,9 @@ static int run_fft(const Plan& p, const void* in, void* out, int sgn, double sca
     return run_fft_s(p, in, out, sgn, scale, epi, st, sh);
 }
 
-// Runs the two passes over the batch in chunks of fs.chunk transforms (0 = whole batch).  Pass B
-// overwrites the middle in place with the output, so when a chunk's middle fits the Infinity
-// Cache its dirty lines are replaced before they are written back and HBM sees about one read
-// and one write per element.
+// Runs the two passes over the batch in chunks of fs.chunk transforms (0 = the whole batch, the
+// default: chunks small enough for the middle to stay in the Infinity Cache measured slower on
+// MI355X, their launch tails outweighing the saved HBM traffic).  Chunking stays a tuning knob.
 static int run_fourstep(const Plan& p, const void* in, void* mid, void* out, int64_t idist,
                         int64_t mdist, int64_t odist, int sgn, double scale, ConvEpilogue epi,
                         hipStream_t st, int64_t B) {

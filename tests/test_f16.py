@@ -310,3 +310,33 @@ def test_gpu_f16_host_arrays(gpu_ready, n):
     assert rel_l2(C.from_complex32(y), ref) < 2e-3
     p = C.plan_fft(x)
     assert np.array_equal((p * x).view(np.uint16), y.view(np.uint16))
+
+
+@pytest.mark.parametrize("n,batch,kind", [(2520, 3, "direct"), (1 << 16, 2, "fourstep"),
+                                          (17, 5, "rader-fused")])
+def test_emulated_f16_strided_rows_and_in_place(E, n, batch, kind):
+    """Rows at idist / odist > n, and in == out, against the C32 plan on the same layout."""
+    lib, ctx = E
+    idist, odist = n + 3, n + 5
+    x = seeded_complex(idist, batch=batch, seed=9)
+    xh = to_h(x)
+    x32 = from_h(xh).astype(np.complex64)
+    p16, p32 = ctypes.c_void_p(), ctypes.c_void_p()
+    assert lib.cfft_plan_1d(ctx, n, batch, idist, odist, C16, 0, ctypes.byref(p16)) == 0
+    assert lib.cfft_plan_1d(ctx, n, batch, idist, odist, C32, 0, ctypes.byref(p32)) == 0
+    assert kind in _desc(lib, p16)
+    yh = np.zeros((batch, odist, 2), np.float16)
+    y32 = np.zeros((batch, odist), np.complex64)
+    assert lib.cfft_exec(p16, xh.ctypes.data, yh.ctypes.data, -1, 0, None) == 0
+    assert lib.cfft_exec(p32, x32.ctypes.data, y32.ctypes.data, -1, 0, None) == 0
+    assert np.array_equal(yh[:, :n].view(np.uint16), to_h(y32[:, :n]).view(np.uint16))
+    assert not yh[:, n:].any()  # the gaps between output rows are not written
+    q16, q32 = ctypes.c_void_p(), ctypes.c_void_p()
+    assert lib.cfft_plan_1d(ctx, n, batch, idist, idist, C16, 0, ctypes.byref(q16)) == 0
+    assert lib.cfft_plan_1d(ctx, n, batch, idist, idist, C32, 0, ctypes.byref(q32)) == 0
+    zh, z32 = xh.copy(), x32.copy()
+    assert lib.cfft_exec(q16, zh.ctypes.data, zh.ctypes.data, 1, 1, None) == 0
+    assert lib.cfft_exec(q32, z32.ctypes.data, z32.ctypes.data, 1, 1, None) == 0
+    assert np.array_equal(zh[:, :n].view(np.uint16), to_h(z32[:, :n]).view(np.uint16))
+    for p in (p16, p32, q16, q32):
+        lib.cfft_plan_destroy(p)

@@ -255,6 +255,16 @@ __device__ __forceinline__ int xcd_block(int remap) {
     return (b & 7) * (G >> 3) + (b >> 3);
 }
 
+// Work-list launches order their items by decreasing cost, so one contiguous range per XCD would
+// give one XCD the most expensive items.  Instead each window of 64 blocks gives XCD x the eight
+// consecutive items [w + 8x, w + 8x + 8) (adjacent tiles of a transform share an L2); blocks past
+// the last full window keep their index.
+__device__ __forceinline__ int xcd_item() {
+    const int b = blockIdx.x;
+    if (b >= ((int)gridDim.x & ~63)) return b;
+    return (b & ~63) | ((b & 7) << 3) | ((b >> 3) & 7);
+}
+
 template <typename T, int E, typename GI = V_t<T>>
 __global__ void __launch_bounds__(1024)
 k_fs_a(const GI* __restrict__ in, V_t<T>* __restrict__ out, FourStepAArgs a, PassTable pt,
@@ -821,7 +831,7 @@ template <typename T>
 __global__ void __launch_bounds__(1024)
 k_fs_a_multi(MultiArgs ma, const MItem* __restrict__ items, const MPlan* __restrict__ plans) {
     CFFT_DYN_LDS(smem);
-    const MItem it = items[blockIdx.x];
+    const MItem it = items[xcd_item()];
     const MPlan* P = plans + it.plan;
     int sgn;
     double scale;
@@ -836,7 +846,7 @@ template <typename T>
 __global__ void __launch_bounds__(1024)
 k_fs_b_multi(MultiArgs ma, const MItem* __restrict__ items, const MPlan* __restrict__ plans) {
     CFFT_DYN_LDS(smem);
-    const MItem it = items[blockIdx.x];
+    const MItem it = items[xcd_item()];
     const MPlan* P = plans + it.plan;
     int sgn;
     double scale;

@@ -1403,7 +1403,9 @@ int cfft_exec_host(cfft_plan plan, const void* in, void* out, int direction, int
     Ctx* c = p.ctx;
     CFFT_HIP(hipSetDevice(c->device));
     const int64_t es = io_esize(p);
-    size_t ib, ob;
+    // input and output spans, and the output rows the transform writes (rows of `row` bytes,
+    // `pitch` bytes apart): the gaps between output rows keep the caller's contents
+    size_t ib, ob, rows = 1, row = 0, pitch = 0;
     if (p.kind == Kind::Uneven) {
         ib = ob = (size_t)p.uneven_span * es;
     } else if (p.kind == Kind::ND) {
@@ -1414,22 +1416,76 @@ int cfft_exec_host(cfft_plan plan, const void* in, void* out, int direction, int
         const size_t cb = (size_t)((p.batch - 1) * p.odist + p.n / 2 + 1) * es;
         ib = direction < 0 ? rb : cb;
         ob = direction < 0 ? cb : rb;
+        rows = (size_t)p.batch;
+        row = direction < 0 ? (size_t)(p.n / 2 + 1) * es : (size_t)p.n * (es / 2);
+        pitch = direction < 0 ? (size_t)p.odist * es : (size_t)p.idist * (es / 2);
     } else {
         if (p.batch == 0) return CFFT_OK;
         ib = (size_t)((p.batch - 1) * p.idist + p.n) * es;
         ob = (size_t)((p.batch - 1) * p.odist + p.n) * es;
+        rows = (size_t)p.batch;
+        row = (size_t)p.n * es;
+        pitch = (size_t)p.odist * es;
     }
+    const bool whole = rows == 1 || row == pitch || p.kind == Kind::Uneven || p.kind == Kind::ND;
+    auto copy_out = [&](const void* from) {  // host copy of the written rows
+        if (whole) {
+            std::memcpy(out, from, ob);
+            return;
+        }
+        for (size_t r = 0; r < rows; ++r)
+            std::memcpy((char*)out + r * pitch, (const char*)from + r * pitch, row);
+    };
     std::lock_guard<std::mutex> g(c->stage_mu);
     int rc;
+    // Small plans of one kernel that reads its input once and writes its output once (direct
+    // rows, fused convolutions) run on fine-grained pinned copies of the host buffers, read and
+    // written by the kernel over PCIe: no DMA transfers, whose latency dominates small transforms.
+    constexpr size_t kZeroCopyMax = (size_t)256 << 10;
+    if ((p.kind == Kind::Direct || p.kind == Kind::ConvSmall) && !p.real &&
+        std::max(ib, ob) <= kZeroCopyMax && c->zc_in.reserve(ib) == 0 &&
+        c->zc_out.reserve(ob) == 0) {
+        void *din = nullptr, *dout = nullptr;
+        CFFT_HIP(hipHostGetDevicePointer(&din, c->zc_in.p, 0));
+        CFFT_HIP(hipHostGetDevicePointer(&dout, c->zc_out.p, 0));
+        std::memcpy(c->zc_in.p, in, ib);
+        rc = plan_exec(p, din, dout, direction, normalize, c->stream);
+        if (rc) return rc;
+        CFFT_HIP(hipStreamSynchronize(c->stream));
+        copy_out(c->zc_out.p);
+        return CFFT_OK;
+    }
     if (c->stage_in.bytes < ib && (rc = c->stage_in.alloc(ib))) return rc;
     if (c->stage_out.bytes < ob && (rc = c->stage_out.alloc(ob))) return rc;
-    CFFT_HIP(hipMemcpyAsync(c->stage_in.p, in, ib, hipMemcpyHostToDevice, c->stream));
+    // Transfers up to kPinnedMax go through pinned copies (a host memcpy each way): the DMA then
+    // runs asynchronously instead of through the runtime's pageable staging, which dominates the
+    // latency of small transforms.
+    constexpr size_t kPinnedMax = (size_t)4 << 20;
+    const bool pinned = std::max(ib, ob) <= kPinnedMax && c->pin_in.reserve(ib) == 0 &&
+                        c->pin_out.reserve(ob) == 0;
+    const void* src = in;
+    if (pinned) {
+        std::memcpy(c->pin_in.p, in, ib);
+        src = c->pin_in.p;
+    }
+    CFFT_HIP(hipMemcpyAsync(c->stage_in.p, src, ib, hipMemcpyHostToDevice, c->stream));
     if (p.kind == Kind::Uneven)  // untouched gaps come back as they went in
-        CFFT_HIP(hipMemcpyAsync(c->stage_out.p, in, ob, hipMemcpyHostToDevice, c->stream));
+        CFFT_HIP(hipMemcpyAsync(c->stage_out.p, src, ob, hipMemcpyHostToDevice, c->stream));
     rc = plan_exec(p, c->stage_in.p, c->stage_out.p, direction, normalize, c->stream);
     if (rc) return rc;
-    CFFT_HIP(hipMemcpyAsync(out, c->stage_out.p, ob, hipMemcpyDeviceToHost, c->stream));
-    CFFT_HIP(hipStreamSynchronize(c->stream));
+    if (pinned) {
+        CFFT_HIP(hipMemcpyAsync(c->pin_out.p, c->stage_out.p, ob, hipMemcpyDeviceToHost,
+                                c->stream));
+        CFFT_HIP(hipStreamSynchronize(c->stream));
+        copy_out(c->pin_out.p);
+    } else {
+        if (whole)
+            CFFT_HIP(hipMemcpyAsync(out, c->stage_out.p, ob, hipMemcpyDeviceToHost, c->stream));
+        else
+            CFFT_HIP(hipMemcpy2DAsync(out, pitch, c->stage_out.p, pitch, row, rows,
+                                      hipMemcpyDeviceToHost, c->stream));
+        CFFT_HIP(hipStreamSynchronize(c->stream));
+    }
     return CFFT_OK;
 }
 

@@ -113,6 +113,35 @@ struct PlanOptions {
     bool alloc_work = true;    // large convolutions: allocate work buffers for `batch` lines
 };
 
+// Pinned host staging buffer (hipHostMalloc, grown on demand with headroom): asynchronous
+// table uploads, and small host-pointer transfers.
+struct HostStage {
+    void* p = nullptr;
+    size_t cap = 0;
+    unsigned flags = hipHostMallocDefault;  // hipHostMallocCoherent: fine-grained (zero-copy)
+    HostStage() = default;
+    explicit HostStage(unsigned f) : flags(f) {}
+    HostStage(const HostStage&) = delete;
+    HostStage& operator=(const HostStage&) = delete;
+    ~HostStage() {
+        if (p) (void)hipHostFree(p);
+    }
+    int reserve(size_t n) {
+        if (n <= cap) return 0;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        const size_t want = n + n / 2;
+        if (hipHostMalloc(&p, want, flags) != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            return -1;
+        }
+        cap = want;
+        return 0;
+    }
+};
+
 using PlanKey = std::tuple<int64_t, int64_t, int64_t, int64_t, int, int>;
 
 struct Ctx {
@@ -125,6 +154,9 @@ struct Ctx {
     std::map<PlanKey, std::shared_ptr<Plan>> cache;
     std::mutex stage_mu;
     DevBuf stage_in, stage_out;
+    HostStage pin_in, pin_out;  // pinned copies of small host-pointer transfers
+    // fine-grained pinned buffers that small single-kernel plans read and write directly
+    HostStage zc_in{hipHostMallocCoherent}, zc_out{hipHostMallocCoherent};
     std::map<std::string, PlanOptions> wisdom;  // tuned choices (autotune.cpp), guarded by mu
 };
 

@@ -18,32 +18,6 @@ struct PlanEntry {
 };
 
 // Where one transform of a plan entry reads and writes.
-// Pinned host staging buffer for asynchronous table uploads (grown on demand).
-struct HostStage {
-    void* p = nullptr;
-    size_t cap = 0;
-    HostStage() = default;
-    HostStage(const HostStage&) = delete;
-    HostStage& operator=(const HostStage&) = delete;
-    ~HostStage() {
-        if (p) (void)hipHostFree(p);
-    }
-    int reserve(size_t n) {
-        if (n <= cap) return 0;
-        if (p) (void)hipHostFree(p);
-        p = nullptr;
-        cap = 0;
-        const size_t want = n + n / 2;
-        if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) {
-            p = nullptr;
-            (void)hipGetLastError();
-            return -1;
-        }
-        cap = want;
-        return 0;
-    }
-};
-
 struct XformRef {
     uint64_t in = 0, out = 0;  // addresses or offsets, see in_sel / out_sel
     int in_sel = kAbs, out_sel = kAbs;

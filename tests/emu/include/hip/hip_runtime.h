@@ -135,7 +135,13 @@ hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t a, int d);
 hipError_t hipFuncSetAttribute(const void* f, hipFuncAttribute a, int v);
 hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int* n, const void* f, int block, size_t lds);
 #define hipHostMallocDefault 0
+#define hipHostMallocCoherent 0x40000000
 hipError_t hipHostMalloc(void** p, size_t n, unsigned flags);
+// host memory is device memory in the emulation
+inline hipError_t hipHostGetDevicePointer(void** d, void* h, unsigned) {
+    *d = h;
+    return hipSuccess;
+}
 hipError_t hipHostFree(void* p);
 hipError_t hipGetLastError();
 const char* hipGetErrorString(hipError_t e);

@@ -319,3 +319,20 @@ def test_emulated_fourstep_chunks(E, monkeypatch):
     z = np.zeros_like(x)
     assert lib.cfft_exec(q, xs.ctypes.data, z.ctypes.data, -1, 0, None) == 0
     assert np.array_equal(y, z)
+
+
+@pytest.mark.parametrize("n,batch,dt", [(60, 3, np.complex64), (1 << 16, 2, np.complex64),
+                                        (17, 4, np.complex128), (2520, 2, np.complex128)])
+def test_emulated_host_path_keeps_row_gaps(E, n, batch, dt):
+    """cfft_exec_host (zero-copy for small single-kernel plans, pinned staging otherwise) writes
+    the output rows only: the gaps between rows at odist > n keep the caller's contents."""
+    lib, ctx = E
+    idist, odist = n + 2, n + 7
+    x = seeded_complex(idist, batch=batch, seed=11, dtype=dt)
+    p = _plan(lib, ctx, n, batch, dt, idist, odist)
+    out = np.full((batch, odist), 7 + 7j, dtype=dt)
+    assert lib.cfft_exec_host(p, x.ctypes.data, out.ctypes.data, -1, 0) == 0
+    ref = O.fft(np.ascontiguousarray(x[:, :n]).astype(np.complex128), "d", -1, False)
+    assert rel_l2(out[:, :n], ref) < _tol(n, dt)
+    assert np.all(out[:, n:] == 7 + 7j)
+    lib.cfft_plan_destroy(p)

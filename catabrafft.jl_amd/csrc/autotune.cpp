@@ -106,8 +106,8 @@ static void copy_shape(const Plan& from, Plan& to) {
 
 // ------------------------------------------------------------------------------- wisdom
 // Tuned choices keyed by (n, batch, idist, odist, dtype, flags); kept per context and written /
-// read as text lines "n batch idist odist dtype flags : E target_threads n1 ca cb chunk pipe"
-// (files without the last field read pipe = default).
+// read as text lines "n batch idist odist dtype flags : E target_threads n1 ca cb chunk pipe
+// large" (files without the last fields read pipe = default, large = 0).
 static std::string wisdom_key(const Plan& p, int flags) {
     std::ostringstream k;
     k << p.n << " " << p.batch << " " << p.idist << " " << p.odist << " " << p.dtype << " " << flags;
@@ -121,8 +121,9 @@ int wisdom_export(Ctx* ctx, const char* path) {
     std::fprintf(f, "# catabrafft wisdom v1 (gfx950)\n");
     for (const auto& kv : ctx->wisdom) {
         const PlanOptions& o = kv.second;
-        std::fprintf(f, "%s : %d %d %lld %d %d %lld %d\n", kv.first.c_str(), o.E,
-                     o.target_threads, (long long)o.n1, o.ca, o.cb, (long long)o.chunk, o.pipe);
+        std::fprintf(f, "%s : %d %d %lld %d %d %lld %d %d\n", kv.first.c_str(), o.E,
+                     o.target_threads, (long long)o.n1, o.ca, o.cb, (long long)o.chunk, o.pipe,
+                     o.large);
     }
     std::fclose(f);
     return CFFT_OK;
@@ -143,11 +144,12 @@ int wisdom_import(Ctx* ctx, const char* path) {
         while (!key.empty() && key.back() == ' ') key.pop_back();
         PlanOptions o;
         long long n1 = 0, chunk = -1;
-        int pipe = -1;
-        const int got = std::sscanf(colon + 1, "%d %d %lld %d %d %lld %d", &o.E,
-                                    &o.target_threads, &n1, &o.ca, &o.cb, &chunk, &pipe);
-        o.pipe = got == 7 ? pipe : -1;
-        if (got != 6 && got != 7) {
+        int pipe = -1, large = 0;
+        const int got = std::sscanf(colon + 1, "%d %d %lld %d %d %lld %d %d", &o.E,
+                                    &o.target_threads, &n1, &o.ca, &o.cb, &chunk, &pipe, &large);
+        o.pipe = got >= 7 ? pipe : -1;
+        o.large = got == 8 ? large : 0;
+        if (got < 6) {
             std::fclose(f);
             return fail(CFFT_ERR_INVALID, "malformed wisdom line");
         }

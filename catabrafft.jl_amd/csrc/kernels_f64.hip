@@ -47,6 +47,10 @@ hipError_t real_rows_f64(const void* in, void* out, const RealArgs& a, const Pas
                          const void* tw, int E, hipStream_t st) {
     return CFFT_E(real_rows_t, double, E, in, out, a, pt, tw, st);
 }
+hipError_t transpose_f64(const void* src, void* dst, int64_t rows, int64_t cols,
+                          const void* tw3, int tbits, double sx, double sy, hipStream_t st) {
+    return transpose_t<double>(src, dst, rows, cols, tw3, tbits, sx, sy, st);
+}
 hipError_t real_aux_f64(const void* in, void* out, const RealAuxArgs& a, hipStream_t st) {
     return real_aux_t<double>(in, out, a, st);
 }

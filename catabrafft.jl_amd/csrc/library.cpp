@@ -393,6 +393,26 @@ static int pick_tile(Ctx* ctx, const LinePlan& lp, Prec pr, int want, size_t res
     return std::max(c, 1);
 }
 
+// w_n^e for e < n as (A[e & m] * B[(e >> b) & m]) * C[e >> 2b], b = ceil(log2(n) / 3): three
+// tables of about n^(1/3) entries, small enough for pass B's LDS.
+static void set_tw3_shape(FourStepPlan& fs, int64_t n) {
+    int tb = 0;
+    while (((int64_t)1 << (3 * tb)) < n) ++tb;
+    fs.tbits = tb;
+    fs.tnc = (int)(((n - 1) >> (2 * tb)) + 1);
+}
+static int upload_tw3(FourStepPlan& fs, int64_t n, Prec pr) {
+    const int tb = fs.tbits;
+    std::vector<cld> t3;
+    t3.reserve(tw3_entries(fs.tbits, fs.tnc));
+    for (int64_t i = 0; i < ((int64_t)1 << tb); ++i) t3.push_back(omega(i, n));
+    for (int64_t i = 0; i < ((int64_t)1 << tb); ++i) t3.push_back(omega(i << tb, n));
+    for (int64_t i = 0; i < fs.tnc; ++i) t3.push_back(omega(i << (2 * tb), n));
+    return upload(fs.tw3, t3, pr);
+}
+
+static int plan_fft_core(Ctx* ctx, Plan& p, const PlanOptions& o);
+
 static int plan_fourstep_factors(Ctx* ctx, FourStepPlan& fs, int64_t n, Prec pr, int E,
                                  int64_t n1_force, int ca, int cb, int64_t chunk) {
     fs.chunk = chunk;
@@ -420,12 +440,7 @@ static int plan_fourstep_factors(Ctx* ctx, FourStepPlan& fs, int64_t n, Prec pr,
     if (rc) return rc;
     rc = make_line(fs.b, fs.N2, pr, E, 1024);
     if (rc) return rc;
-    // w_n^e for e < n as (A[e & m] * B[(e >> b) & m]) * C[e >> 2b], b = ceil(log2(n) / 3):
-    // three tables of about n^(1/3) entries, small enough for pass B's LDS
-    int tb = 0;
-    while (((int64_t)1 << (3 * tb)) < n) ++tb;
-    fs.tbits = tb;
-    fs.tnc = (int)(((n - 1) >> (2 * tb)) + 1);
+    set_tw3_shape(fs, n);
     const size_t es = (size_t)esize(pr);
     const size_t table = (size_t)tw3_entries(fs.tbits, fs.tnc) * es;
     // Narrow tiles (4 columns; 8 in F32 pass B) let several workgroups share a CU, so one
@@ -444,15 +459,63 @@ static int plan_fourstep_factors(Ctx* ctx, FourStepPlan& fs, int64_t n, Prec pr,
         if (C < 1 || (C & (C - 1)) || C * lp.Tt > 1024 || lds > ctx->lds_max)
             return fail(CFFT_ERR_UNSUPPORTED, "four-step tile does not fit a workgroup");
     }
-    std::vector<cld> t3;
-    t3.reserve(tw3_entries(fs.tbits, fs.tnc));
-    for (int64_t i = 0; i < ((int64_t)1 << tb); ++i) t3.push_back(omega(i, n));
-    for (int64_t i = 0; i < ((int64_t)1 << tb); ++i) t3.push_back(omega(i << tb, n));
-    for (int64_t i = 0; i < fs.tnc; ++i) t3.push_back(omega(i << (2 * tb), n));
-    return upload(fs.tw3, t3, pr);
+    return upload_tw3(fs, n, pr);
 }
 
-static int plan_fft_core(Ctx* ctx, Plan& p, const PlanOptions& o);
+// Plans n = N1 * M too long for the four-step (its factors do not fit a workgroup), or forced
+// by o.large: the six-step path.
+//   1. pass A: the N1-point FFTs of the M columns of x (N1 x M row-major), column j2 written as
+//      row j2 of a middle M x N1 (k_fs_a);
+//   2. transpose to N1 x M, times w_n^{j2 k1} (k_transpose);
+//   3. the M-point FFTs of the N1 contiguous rows (sub-plan: direct or four-step, in place);
+//   4. transpose to M x N1: X[k1 + N1 k2], conjugated and scaled for backward transforms.
+// Five reads and writes per element instead of the four-step's two; the sizes it serves cannot
+// run otherwise.
+static int plan_large(Ctx* ctx, Plan& p, const PlanOptions& o) {
+    const int64_t n = p.n;
+    const int E = o.E ? o.E : (p.prec == Prec::F64 ? 8 : 16);
+    int64_t N1 = 0;
+    if (o.n1 > 0) {
+        N1 = o.n1;
+        if (n % N1 || !line_fits(ctx, N1, p.prec, E))
+            return fail(CFFT_ERR_UNSUPPORTED, "six-step split does not fit a workgroup");
+    } else {  // the largest factor that fits a workgroup, so that M is as short as possible
+        for (int64_t a = 2; a * a <= n; ++a) {
+            if (n % a) continue;
+            if (line_fits(ctx, a, p.prec, E)) N1 = std::max(N1, a);
+            if (line_fits(ctx, n / a, p.prec, E)) N1 = std::max(N1, n / a);
+        }
+    }
+    if (N1 < 2 || N1 == n) return fail(CFFT_ERR_UNSUPPORTED, "no six-step split for n=" + std::to_string(n));
+    const int64_t M = n / N1;
+    FourStepPlan& fs = p.fs;
+    fs.N1 = (int)N1;
+    fs.N2 = (int)M;
+    int rc = make_line(fs.a, N1, p.prec, E, 1024);
+    if (rc) return rc;
+    fs.CA = pick_tile(ctx, fs.a, p.prec, 4);
+    fs.a.Ls = fs_line_stride(ctx, p.prec, fs.a, fs.CA, 16);
+    if (2 * (size_t)(esize(p.prec) / 2) * fs.a.Ls * fs.CA > ctx->lds_max || fs.CA * fs.a.Tt > 1024)
+        return fail(CFFT_ERR_UNSUPPORTED, "six-step tile does not fit a workgroup");
+    set_tw3_shape(fs, n);
+    if ((rc = upload_tw3(fs, n, p.prec))) return rc;
+    std::unique_ptr<Plan> s(new Plan());
+    s->ctx = ctx;
+    s->n = M;
+    s->batch = N1;
+    s->idist = s->odist = M;
+    s->prec = p.prec;
+    s->dtype = p.prec == Prec::F32 ? CFFT_C32 : CFFT_C64;
+    s->flags = CFFT_NO_FLAG;
+    if ((rc = plan_fft_core(ctx, *s, PlanOptions{}))) return rc;
+    if (s->kind != Kind::Direct && s->kind != Kind::FourStep)
+        return fail(CFFT_ERR_UNSUPPORTED, "six-step rows of length " + std::to_string(M) +
+                                              " are not plannable");
+    p.sub = std::move(s);
+    p.kind = Kind::Large;
+    return CFFT_OK;
+}
+
 
 // Convolution data of a Rader (prime n) or Bluestein transform: permutations or chirp, and the
 // zero-padded kernel b whose spectrum the convolution multiplies by.
@@ -660,12 +723,21 @@ static int plan_fft_core(Ctx* ctx, Plan& p, const PlanOptions& o) {
             if (line_fits(ctx, p.n, p.prec, Ed))
                 return plan_direct(ctx, p, Ed, o.target_threads, pipe);
         }
+        if (o.large) return plan_large(ctx, p, o);
         // Four-step lines default to E = 8 in F64 (twice the threads per tile), 16 in F32.
         const int64_t chunk = o.chunk >= 0 ? o.chunk : default_chunk(p);
         int rc = CFFT_ERR_UNSUPPORTED;
         if (!o.E && p.prec == Prec::F64)
             rc = plan_fourstep_factors(ctx, p.fs, p.n, p.prec, 8, o.n1, o.ca, o.cb, chunk);
         if (rc) rc = plan_fourstep_factors(ctx, p.fs, p.n, p.prec, E, o.n1, o.ca, o.cb, chunk);
+        if (rc && o.n1 == 0) {  // too long for the four-step: six steps
+            p.fs.b.dtw.reset();
+            p.fs.tw3.reset();
+            p.fs.CB = 1;
+            p.fs.chunk = 0;
+            p.fs.pipe_a = p.fs.pipe_b = 0;
+            return plan_large(ctx, p, o);
+        }
         if (rc) return rc;
         p.kind = Kind::FourStep;
         if (o.pipe > 0) fs_pipe_setup(ctx, p);
@@ -700,6 +772,10 @@ static std::string describe(const Plan& p) {
             if (p.fs.chunk) os << " chunk=" << p.fs.chunk;
             if (p.fs.pipe_a || p.fs.pipe_b)
                 os << " pipe=" << (p.fs.pipe_a ? "A" : "") << (p.fs.pipe_b ? "B" : "");
+            break;
+        case Kind::Large:
+            os << "sixstep N1=" << p.fs.N1 << " " << rad(p.fs.a) << " CA=" << p.fs.CA
+               << " Ls=" << p.fs.a.Ls << " M=" << p.fs.N2 << " {" << describe(*p.sub) << "}";
             break;
         case Kind::ConvSmall:
             os << (p.mode == ConvMode::Rader ? "rader" : "bluestein") << "-fused L=" << p.L << " "
@@ -850,6 +926,49 @@ static int run_fft_s(const Plan& p, const void* in, void* out, int sgn, double s
             }
             return run_fourstep(p, in, out, out, p.idist, p.odist, p.odist, sgn, scale, epi, st, B);
         }
+        case Kind::Large: {  // six steps per transform (plan_large)
+            if (epi.K) return fail(CFFT_ERR_UNSUPPORTED, "six-step plans take no epilogue");
+            const FourStepPlan& fs = p.fs;
+            const int64_t n = p.n, M = fs.N2, N1 = fs.N1;
+            const size_t es = esize(p.prec);
+            std::lock_guard<std::mutex> g(p.scratch_mu);
+            if (p.scratch.bytes < (size_t)n * es) {
+                int rc = p.scratch.alloc((size_t)n * es);
+                if (rc) return rc;
+            }
+            for (int64_t b = 0; b < B; ++b) {
+                const char* inb = (const char*)in + (size_t)b * p.idist * es;
+                char* outb = (char*)out + (size_t)b * p.odist * es;
+                // in place: pass A into the scratch, then out holds the rows and the scratch the
+                // result, copied back at the end
+                const bool inplace = inb == outb;
+                char* mid = inplace ? (char*)p.scratch.p : outb;
+                char* rowsb = inplace ? outb : (char*)p.scratch.p;
+                FourStepAArgs a{};
+                a.N1 = (int)N1;
+                a.N2 = (int)M;
+                a.C = fs.CA;
+                a.Tt = fs.a.Tt;
+                a.Ls = fs.a.Ls;
+                a.E = fs.a.E;
+                a.batch = 1;
+                a.idist = n;
+                a.odist = n;
+                a.sgn_in = sgn;
+                a.xcd = 1;
+                CFFT_HIP(launch_fourstep_a(p.prec, inb, mid, a, fs.a.pt, fs.a.dtw.p, st));
+                CFFT_HIP(launch_transpose(p.prec, mid, rowsb, M, N1, fs.tw3.p, fs.tbits, 1.0, 1.0,
+                                          st));
+                int rc = run_fft(*p.sub, rowsb, rowsb, 1, 1.0, ConvEpilogue{nullptr, nullptr}, st);
+                if (rc) return rc;
+                char* res = inplace ? (char*)p.scratch.p : outb;
+                CFFT_HIP(launch_transpose(p.prec, rowsb, res, N1, M, nullptr, 0, scale,
+                                          scale * sgn, st));
+                if (inplace)
+                    CFFT_HIP(hipMemcpyAsync(outb, res, (size_t)n * es, hipMemcpyDeviceToDevice, st));
+            }
+            return CFFT_OK;
+        }
         case Kind::Cols: {  // fs.N1 adjacent lines of length fs.N2 (stride N1), batch blocks
             FourStepBArgs b{};
             b.N1 = p.fs.N1;
@@ -952,7 +1071,8 @@ static int real_exec(const Plan& p, const void* in, void* out, int direction, in
 
 int plan_exec_shaped(const Plan& p, const void* in, void* out, int direction, int normalize,
                      hipStream_t st, const ExecShape& sh) {
-    if (p.kind == Kind::Uneven || p.real || p.kind == Kind::ND || p.kind == Kind::Cols || p.half)
+    if (p.kind == Kind::Uneven || p.real || p.kind == Kind::ND || p.kind == Kind::Cols || p.half ||
+        p.kind == Kind::Large)
         return fail(CFFT_ERR_UNSUPPORTED, "shaped execution needs a 1-D float/double plan");
     if (sh.batch <= 0) return CFFT_OK;
     if (!in || !out) return fail(CFFT_ERR_INVALID, "null buffer");

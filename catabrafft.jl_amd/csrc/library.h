@@ -46,7 +46,7 @@ struct FourStepPlan {
     int64_t chunk = 0;  // transforms per pass-A/pass-B round (0 = the whole batch)
 };
 
-enum class Kind : int { Trivial, Direct, FourStep, ConvSmall, ConvLarge, Uneven, Cols, ND };
+enum class Kind : int { Trivial, Direct, FourStep, ConvSmall, ConvLarge, Uneven, Cols, ND, Large };
 
 struct Ctx;
 
@@ -74,7 +74,7 @@ struct Plan {
     int LsC = 0;  // fused convolution: LDS line stride (input and work lines share it)
     Prec cprec = Prec::F32;
     DevBuf perm, iperm, chirp, K, dc, w1, w2;
-    std::unique_ptr<Plan> sub;
+    std::unique_ptr<Plan> sub;  // large convolution: FFT_L; six-step: the M-point row FFTs
     // real-input transforms: 1 = fused (n even, n/2 fits a workgroup), 2 = even via a complex
     // sub-plan of n/2, 3 = odd via a complex sub-plan of n; idist / odist hold rdist (reals) and
     // cdist (complex elements)
@@ -108,6 +108,7 @@ struct PlanOptions {
     int ca = 0, cb = 0;        // four-step column tiles (0 = default)
     int64_t chunk = -1;        // four-step transforms per round (-1 = default, 0 = whole batch)
     int pipe = -1;             // direct: pipelined rows kernel (0 = off, 1 = on, -1 = default)
+    int large = 0;             // 1: the six-step path with N1 = n1 even where the four-step fits
     int force_conv = 0;
     bool explicit_opts = false;
     bool alloc_work = true;    // large convolutions: allocate work buffers for `batch` lines

@@ -167,6 +167,8 @@ static int srv_plan(Server* s, int64_t n, int dtype, Plan** out, size_t* new_byt
         PlanOptions o;
         o.alloc_work = false;  // a large convolution's work buffers come with each request
         if ((rc = plan_build(s->ctx, *p, o))) return rc;
+        if (p->kind == Kind::Large)  // one plan-owned scratch: rounds on several streams would race
+            return fail(CFFT_ERR_UNSUPPORTED, "the server takes lengths the four-step covers");
     }
     *new_bytes = plan_device_bytes(*p);
     *out = p.get();

@@ -336,3 +336,42 @@ def test_emulated_host_path_keeps_row_gaps(E, n, batch, dt):
     assert rel_l2(out[:, :n], ref) < _tol(n, dt)
     assert np.all(out[:, n:] == 7 + 7j)
     lib.cfft_plan_destroy(p)
+
+
+@pytest.mark.parametrize("n,n1,batch,dt", [(1 << 16, 256, 2, np.complex64),
+                                           (1 << 17, 128, 1, np.complex128),
+                                           (3 ** 10, 243, 2, np.complex64),
+                                           (2520 * 16, 2520, 1, np.complex128),
+                                           (1 << 14, 64, 3, np.complex64)])
+def test_emulated_sixstep_matches_oracle(E, tmp_path, n, n1, batch, dt):
+    """The six-step path (forced through wisdom at sizes the four-step also covers): pass A,
+    twiddled transpose, M-point row FFTs, transpose; forward, backward normalised, strided rows
+    and in place, against the oracle."""
+    lib, ctx = E
+    dcode = 0 if dt == np.complex64 else 1
+    idist, odist = n + 4, n + 6
+    path = tmp_path / "w.txt"
+    path.write_text(f"{n} {batch} {idist} {odist} {dcode} 1 : 0 64 {n1} 0 0 -1 -1 1\n"
+                    f"{n} {batch} {n} {n} {dcode} 1 : 0 64 {n1} 0 0 -1 -1 1\n")
+    c2 = ctypes.c_void_p()
+    assert lib.cfft_init(0, ctypes.byref(c2)) == 0
+    assert lib.cfft_wisdom_import(c2, str(path).encode()) == 0
+    p = _plan(lib, c2, n, batch, dt, idist, odist, flags=1)
+    d = _desc(lib, p)
+    assert d.startswith(f"n={n}") and "sixstep" in d and f"N1={n1}" in d, d
+    x = seeded_complex(idist, batch=batch, seed=n1, dtype=dt)
+    xr = np.ascontiguousarray(x[:, :n]).astype(np.complex128)
+    for direction, norm in ((-1, 0), (1, 1)):
+        y = np.full((batch, odist), 3 + 3j, dtype=dt)
+        assert lib.cfft_exec(p, x.ctypes.data, y.ctypes.data, direction, norm, None) == 0
+        ref = O.fft(xr, "d", direction, bool(norm))
+        assert rel_l2(y[:, :n], ref) < _tol(n, dt), (direction, norm)
+        assert np.all(y[:, n:] == 3 + 3j)
+    q = _plan(lib, c2, n, batch, dt, flags=1)
+    assert "sixstep" in _desc(lib, q)
+    z = np.ascontiguousarray(x[:, :n])
+    assert lib.cfft_exec(q, z.ctypes.data, z.ctypes.data, -1, 0, None) == 0
+    assert rel_l2(z, O.fft(xr, "d", -1, False)) < _tol(n, dt)
+    lib.cfft_plan_destroy(p)
+    lib.cfft_plan_destroy(q)
+    lib.cfft_finalize(c2)

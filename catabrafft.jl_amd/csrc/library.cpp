@@ -1102,8 +1102,38 @@ static int half_exec_staged(const Plan& p, const void* in, void* out, int sgn, d
     return CFFT_OK;
 }
 
+// Whether an execution uses buffers the plan owns (scratch middles, convolution work lines,
+// staging for ComplexF16, real-transform temporaries, uneven tables).
+static bool uses_work_buffers(const Plan& p, const void* in, const void* out) {
+    switch (p.kind) {
+        case Kind::FourStep: return in == out || p.half;
+        case Kind::Large:
+        case Kind::ConvLarge:
+        case Kind::Uneven:
+        case Kind::ND: return true;
+        default: return p.real >= 2 || (p.half && p.kind != Kind::Direct && p.kind != Kind::Trivial);
+    }
+}
+
+static int plan_exec_body(const Plan& p, const void* in, void* out, int direction, int normalize,
+                          hipStream_t st);
+
+// Executions of one plan on different streams may overlap on the device, except those that use
+// the plan's work buffers: they are ordered (each waits on the device for the previous one).
 int plan_exec(const Plan& p, const void* in, void* out, int direction, int normalize,
               hipStream_t st) {
+    if (!uses_work_buffers(p, in, out)) return plan_exec_body(p, in, out, direction, normalize, st);
+    std::lock_guard<std::mutex> g(p.exec_mu);
+    if (p.work_ev) CFFT_HIP(hipStreamWaitEvent(st, p.work_ev, 0));
+    const int rc = plan_exec_body(p, in, out, direction, normalize, st);
+    if (rc) return rc;
+    if (!p.work_ev) CFFT_HIP(hipEventCreateWithFlags(&p.work_ev, hipEventDisableTiming));
+    CFFT_HIP(hipEventRecord(p.work_ev, st));
+    return CFFT_OK;
+}
+
+static int plan_exec_body(const Plan& p, const void* in, void* out, int direction, int normalize,
+                          hipStream_t st) {
     if (p.kind == Kind::Uneven) return uneven_exec(p, in, out, direction, normalize, st);
     if (p.real) return real_exec(p, in, out, direction, normalize, st);
     if (p.batch == 0) return CFFT_OK;

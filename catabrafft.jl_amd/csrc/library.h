@@ -64,6 +64,16 @@ struct Plan {
     FourStepPlan fs;
     mutable std::mutex scratch_mu;
     mutable DevBuf scratch;
+    // Executions that use the plan's own work buffers are ordered on the device: each waits for
+    // work_ev (recorded by the previous one, on whatever stream) before it starts.
+    mutable std::mutex exec_mu;
+    mutable hipEvent_t work_ev = nullptr;
+    Plan() = default;
+    Plan(const Plan&) = delete;
+    Plan& operator=(const Plan&) = delete;
+    ~Plan() {
+        if (work_ev) (void)hipEventDestroy(work_ev);
+    }
     // ComplexF16 input and output (dtype CFFT_C16): the plan computes in float; direct and
     // four-step plans convert inside their kernels, other kinds through hbuf_in / hbuf_out
     bool half = false;

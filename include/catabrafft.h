@@ -7,7 +7,9 @@
  * complex plans, half (CFFT_C16, Julia ComplexF16), rows contiguous,
  * 0-based.  Forward uses e^{-2 pi i jk/n}; backward is unnormalised (CatabraFFT.bfft) and
  * normalize = 1 divides by n (CatabraFFT.ifft).  Plans are immutable once created and may be
- * executed concurrently on distinct streams.  Out-of-place execution never modifies the input;
+ * executed concurrently on distinct streams; executions that use the plan's own work buffers
+ * (in-place four-step, six-step, large convolutions, ComplexF16 staging, N-D, real and uneven
+ * plans) are ordered on the device in call order.  Out-of-place execution never modifies the input;
  * in == out is allowed.
  *
  * Reference interfaces these entry points replace (CatabraFFT.jl sources):

@@ -214,3 +214,48 @@ def test_plan_api(gpu_ready):
         p * seeded_complex(128, seed=1)
     C.empty_cache()
     assert rel_l2(C.fft(x), y) == 0.0
+
+
+@pytest.mark.parametrize("n,batch,dt,inplace,code", [
+    (65537, 3, np.complex64, False, None),       # large convolution: plan-owned work lines
+    (1 << 20, 3, np.complex128, True, None),     # four-step in place: plan-owned middle
+    (17 * 19 * 23, 5, np.complex64, False, 2),   # ComplexF16 staged through plan-owned buffers
+    (2520, 7, np.complex64, False, None),        # direct: no work buffers, runs concurrently
+])
+def test_one_plan_on_two_streams(gpu_ready, n, batch, dt, inplace, code):
+    """Executions of one plan enqueued on two streams at once give the results of sequential
+    executions, bit for bit (those that use the plan's own work buffers are ordered on the
+    device)."""
+    torch = gpu_ready
+    import catabrafft as C
+
+    if code == 2:
+        p = C.Plan(n, batch, 2)
+        mk = lambda s: torch.from_numpy(np.stack([seeded_complex(n, batch=batch, seed=s).real,
+                                                  seeded_complex(n, batch=batch, seed=s).imag],
+                                                 -1).astype(np.float16)).cuda()
+    else:
+        p = C.Plan(n, batch, dt)
+        mk = lambda s: torch.from_numpy(seeded_complex(n, batch=batch, seed=s, dtype=dt)).cuda()
+    xs = [mk(1), mk(2)]
+    refs = []
+    for x in xs:
+        y = x.clone() if inplace else torch.empty_like(x)
+        p.exec_ptr(x.data_ptr() if not inplace else y.data_ptr(), y.data_ptr(), -1, False,
+                   torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        refs.append(y.cpu())
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    torch.cuda.synchronize()
+    outs = []
+    for x, s in zip(xs, streams):
+        y = x.clone() if inplace else torch.empty_like(x)
+        torch.cuda.synchronize()
+        outs.append(y)
+    for x, y, s in zip(xs, outs, streams):
+        p.exec_ptr(y.data_ptr() if inplace else x.data_ptr(), y.data_ptr(), -1, False,
+                   s.cuda_stream)
+    torch.cuda.synchronize()
+    for y, r in zip(outs, refs):
+        assert torch.equal(y.cpu().view(torch.int16) if code == 2 else y.cpu(), 
+                           r.view(torch.int16) if code == 2 else r)

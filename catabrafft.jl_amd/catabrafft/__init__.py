@@ -147,10 +147,14 @@ class Plan:
         self.h = h
 
     def __del__(self):
+        # at interpreter shutdown the module globals may already be gone: then the process exit
+        # releases the plan
         h = getattr(self, "h", None)
-        if h is not None and h.value and _lib._lib is not None:
-            _lib._lib.cfft_plan_destroy(h)
-            self.h = None
+        lib_mod = globals().get("_lib")
+        if h is None or not h.value or lib_mod is None or getattr(lib_mod, "_lib", None) is None:
+            return
+        lib_mod._lib.cfft_plan_destroy(h)
+        self.h = None
 
     # device pointers
     def exec_ptr(self, in_ptr: int, out_ptr: int, direction: int = FORWARD,

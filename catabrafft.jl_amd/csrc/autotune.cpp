@@ -56,7 +56,7 @@ static std::vector<PlanOptions> candidates(Ctx* ctx, const Plan& p, int flags) {
             if (r <= (enchant ? 2.1 : 0.8)) splits.push_back(a);
         }
         const std::vector<int> tiles =
-            enchant ? std::vector<int>{4, 8, 16, 32} : std::vector<int>{8, 16};
+            enchant ? std::vector<int>{2, 4, 8, 16, 32} : std::vector<int>{4, 8, 16};
         // chunks: whole batch, and transforms per round whose middle is ~32 / ~96 MB
         std::vector<int64_t> chunks{0};
         const int64_t tb = p.n * (p.prec == Prec::F32 ? 8 : 16);

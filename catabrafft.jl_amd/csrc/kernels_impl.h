@@ -279,7 +279,9 @@ k_fs_a(const GI* __restrict__ in, V_t<T>* __restrict__ out, FourStepAArgs a, Pas
 }
 
 // ------------------------------------------------------------------------------------------
-// Four-step pass B body: lines k10 .. k10+C-1, staged load and staged (transposing) store.
+// Four-step pass B body: lines k10 .. k10+C-1.  The first pass reads the middle from global
+// memory and applies the twiddle in registers; the last pass writes the output directly, or
+// through a staged store when a convolution epilogue (K) is fused.
 //   out[k1 + N1*k2] = DFT_N2(w_n^{j2 k1} in[j2*N1 + k1])[k2];  in == out allowed (same index
 // set).  w_n^e = (A[e & m] * B[(e >> tbits) & m]) * C[e >> 2 tbits] from the three-level table
 // tw3 (nullptr: no twiddle), which is copied into LDS after the lines.

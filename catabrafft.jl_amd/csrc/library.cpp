@@ -679,8 +679,6 @@ static int plan_conv(Ctx* ctx, Plan& p, const PlanOptions& o) {
     return CFFT_OK;
 }
 
-// Default four-step chunk: transforms per chunk whose middle stays within ~96 MB of the 256 MB
-// Infinity Cache (the whole batch when it is small already).
 // Transforms per four-step round by default: the whole batch (measured fastest on MI355X for
 // 2^20 x 64 in F32 and F64 once pass B's twiddles come from LDS); smaller rounds, whose middle
 // stays in the Infinity Cache, are tuning candidates.  CFFT_FOURSTEP_CHUNK overrides it.

@@ -299,12 +299,11 @@ static bool line_fits(Ctx* ctx, int64_t N, Prec pr, int E) {
 // pass's reads and the last pass's writes, which go to global memory -- and charges each
 // instruction the bank conflicts of 32 four-byte banks per half wave (weight 4) plus those of 64
 // banks over the whole wave (tie-break).  Results are cached per (precision, N, E, C, LDS room).
-static int bank_cost(int* words, int n, int banks) {
-    // words: the 4-byte LDS words of one group of lanes; cost = most distinct words on one bank
-    std::sort(words, words + n);
+static int bank_cost(const int* words, int n, int banks) {
+    // words: the 4-byte LDS words of one group of lanes (distinct: a pass reads or writes each
+    // element once); cost = most words on one bank (banks: a power of two)
     int cnt[64] = {0}, worst = 0;
-    for (int i = 0; i < n; ++i)
-        if (i == 0 || words[i] != words[i - 1]) worst = std::max(worst, ++cnt[words[i] % banks]);
+    for (int i = 0; i < n; ++i) worst = std::max(worst, ++cnt[words[i] & (banks - 1)]);
     return worst;
 }
 
@@ -349,14 +348,16 @@ static int fs_line_stride(Ctx* ctx, Prec pr, const LinePlan& lp, int C, size_t r
             }
             cost += bank_cost(acc, n, 64);
         };
-        for (int w = 0; w < waves; ++w) {
+        // a candidate whose partial cost reaches the best so far cannot win: stop early
+        auto over = [&] { return best_cost >= 0 && cost >= best_cost; };
+        for (int w = 0; w < waves && !over(); ++w) {
             int s = 1;
-            for (int ip = 0; ip < lp.pt.npass; ++ip) {
+            for (int ip = 0; ip < lp.pt.npass && !over(); ++ip) {
                 const int R = lp.pt.radix[ip], nb = N / R, MB = std::max(1, E / R);
                 const bool rd = ip > 0, wr = ip + 1 < lp.pt.npass;
-                for (int ib = 0; ib < MB; ++ib)
-                    for (int j = 0; j < R; ++j)
-                        for (int pass_wr = 0; pass_wr < 2; ++pass_wr) {
+                for (int ib = 0; ib < MB && !over(); ++ib)
+                    for (int j = 0; j < R && !over(); ++j)
+                        for (int pass_wr = 0; pass_wr < 2 && !over(); ++pass_wr) {
                             if (pass_wr ? !wr : !rd) continue;
                             for (int L = 0; L < 64; ++L) {
                                 const int tid = 64 * w + L, c = tid % C;
@@ -564,6 +565,27 @@ int conv_data(int64_t n, ConvData& d) {
 
 // K = FFT_L(b), rounded once to cprec.  Small L: long double on the host; larger L: this
 // library's own FP64 FFT on the device (FP64 is kept for kernel generation only).
+// A default (NO_FLAG) plan of `batch` contiguous transforms of length n from the context's plan
+// cache, created on first use.
+static int cached_plan_1d(Ctx* c, int64_t n, int64_t batch, int dtype, std::shared_ptr<Plan>* out) {
+    const PlanKey key{n, batch, n, n, dtype, CFFT_NO_FLAG};
+    {
+        std::lock_guard<std::mutex> g(c->mu);
+        auto it = c->cache.find(key);
+        if (it != c->cache.end()) {
+            *out = it->second;
+            return CFFT_OK;
+        }
+    }
+    std::shared_ptr<Plan> p;
+    int rc = plan_create(c, n, batch, n, n, dtype, CFFT_NO_FLAG, PlanOptions{}, &p);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> g(c->mu);
+    auto ins = c->cache.emplace(key, p);  // another thread may have made it meanwhile
+    *out = ins.first->second;
+    return CFFT_OK;
+}
+
 int conv_spectrum(Ctx* ctx, const std::vector<cld>& b, Prec cprec, DevBuf& K) {
     const int64_t L = (int64_t)b.size();
     if (L <= 4096) {
@@ -571,20 +593,14 @@ int conv_spectrum(Ctx* ctx, const std::vector<cld>& b, Prec cprec, DevBuf& K) {
         if (!host_fft_ld(h)) return fail(CFFT_ERR_UNSUPPORTED, "kernel spectrum length not smooth");
         return upload(K, h, cprec);
     }
-    Plan t;
-    t.ctx = ctx;
-    t.n = L;
-    t.batch = 1;
-    t.idist = t.odist = L;
-    t.prec = Prec::F64;
-    t.dtype = CFFT_C64;
-    PlanOptions o;
-    int rc = plan_build(ctx, t, o);
+    // the FP64 plan comes from the context's cache: convolutions of nearby lengths share L
+    std::shared_ptr<Plan> t;
+    int rc = cached_plan_1d(ctx, L, 1, CFFT_C64, &t);
     if (rc) return rc;
     DevBuf in, out;
     if ((rc = upload(in, b, Prec::F64))) return rc;
     if ((rc = out.alloc((size_t)L * 16))) return rc;
-    if ((rc = plan_exec(t, in.p, out.p, CFFT_FORWARD, 0, ctx->stream))) return rc;
+    if ((rc = plan_exec(*t, in.p, out.p, CFFT_FORWARD, 0, ctx->stream))) return rc;
     CFFT_HIP(hipStreamSynchronize(ctx->stream));
     if (cprec == Prec::F64) {
         std::swap(K.p, out.p);
@@ -600,7 +616,6 @@ int conv_spectrum(Ctx* ctx, const std::vector<cld>& b, Prec cprec, DevBuf& K) {
     return CFFT_OK;
 }
 
-// Precision of a convolution's work: FP64 when the I/O is FP32 and n exceeds 2^16.
 // Precision of a convolution's work: a Rader convolution of F32 data with p > 2^16 runs in FP64
 // (the precision policy for long Rader convolutions); Bluestein convolves in the data precision
 // at every length (its kernel spectrum is FP64-generated and rounded once: relative L2 error

@@ -146,10 +146,12 @@ int rows_pipe_blocks_per_cu(Prec p, int E, int R0, int block, size_t lds);
 hipError_t launch_fourstep_a(Prec p, const void* in, void* out, const FourStepAArgs& a,
                              const PassTable& pt, const void* tw, hipStream_t st);
 // Transpose of a rows x cols matrix: dst[c rows + r] = src[r cols + c], times w_n^{r c} from the
-// three-level table tw3 (nullptr: no twiddle), then (re sx, im sy).  The steps between the
-// passes of transforms too long for the four-step (the six-step path).
+// three-level table tw3 (nullptr: no twiddle); then, at output index o, the convolution
+// epilogue when K is set (dc = the value at o = 0, then conj(v K[o])), then (re sx, im sy).
+// The steps between the passes of transforms too long for the four-step (the six-step path).
 hipError_t launch_transpose(Prec p, const void* src, void* dst, int64_t rows, int64_t cols,
-                            const void* tw3, int tbits, double sx, double sy, hipStream_t st);
+                            const void* tw3, int tbits, double sx, double sy, const void* K,
+                            void* dc, hipStream_t st);
 // Persistent pipelined four-step passes (a.grid workgroups of at most kPipeMaxThreads, lines of
 // at least two passes, no epilogue); twn = entries of the line's twiddle table, copied into LDS.
 // fs_pipe_lds gives the LDS bytes (pass 1 = B: lines, pass twiddles, then the three-level table

@@ -401,7 +401,8 @@ k_fs_b(const V_t<T>* in, GO* out, FourStepBArgs a, PassTable pt,
 }
 
 // ------------------------------------------------------------------------------------------
-// Tiled transpose with an optional four-step twiddle and scale (the six-step path).  A 32 x 32
+// Tiled transpose with an optional four-step twiddle, convolution epilogue and scale (the
+// six-step path).  A 32 x 32
 // tile goes through LDS (one pad word per row), so both the reads (along a source row) and the
 // writes (along a destination row) are coalesced.  Grid: x over column tiles, y over row tiles
 // (y wraps when there are more than 65535).
@@ -411,7 +412,8 @@ constexpr int kTrTile = 32;
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_transpose(const V_t<T>* __restrict__ src, V_t<T>* __restrict__ dst, int64_t rows, int64_t cols,
-            const V_t<T>* __restrict__ tw3, int tbits, T sx, T sy) {
+            const V_t<T>* __restrict__ tw3, int tbits, T sx, T sy, const V_t<T>* __restrict__ K,
+            V_t<T>* dc) {
     CFFT_DYN_LDS(smem);
     V_t<T>(*tile)[kTrTile + 1] = reinterpret_cast<V_t<T>(*)[kTrTile + 1]>(smem);
     const int tx = threadIdx.x & (kTrTile - 1), ty = threadIdx.x >> 5;  // 32 x 8 threads
@@ -425,14 +427,22 @@ k_transpose(const V_t<T>* __restrict__ src, V_t<T>* __restrict__ dst, int64_t ro
             if (r < rows && c < cols) {
                 V_t<T> v = src[r * cols + c];
                 if (tw3) v = cmul(v, tw3_get<T>(tw3, tbits, (int)(r * c)));
-                tile[ty + k][tx] = V_t<T>{v.x * sx, v.y * sy};
+                tile[ty + k][tx] = v;
             }
         }
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < kTrTile; k += 8) {
             const int64_t c = c0 + ty + k, r = r0 + tx;
-            if (r < rows && c < cols) dst[c * rows + r] = tile[tx][ty + k];
+            if (r < rows && c < cols) {
+                V_t<T> v = tile[tx][ty + k];
+                const int64_t o = c * rows + r;
+                if (K) {  // a convolution's epilogue, as in GRowDst
+                    if (dc && o == 0) *dc = v;
+                    v = cconj(cmul(v, K[o]));
+                }
+                dst[o] = V_t<T>{v.x * sx, v.y * sy};
+            }
         }
         __syncthreads();
     }
@@ -440,14 +450,15 @@ k_transpose(const V_t<T>* __restrict__ src, V_t<T>* __restrict__ dst, int64_t ro
 
 template <typename T>
 hipError_t transpose_t(const void* src, void* dst, int64_t rows, int64_t cols, const void* tw3,
-                       int tbits, double sx, double sy, hipStream_t st) {
+                       int tbits, double sx, double sy, const void* K, void* dc, hipStream_t st) {
     const int64_t ct = (cols + kTrTile - 1) / kTrTile, rt = (rows + kTrTile - 1) / kTrTile;
     if (ct <= 0 || rt <= 0) return hipSuccess;
     if (ct > 0x7fffffff) return hipErrorInvalidValue;
     const dim3 grid((unsigned)ct, (unsigned)(rt < 65535 ? rt : 65535));
     const size_t lds = sizeof(V_t<T>) * kTrTile * (kTrTile + 1);
     hipLaunchKernelGGL(k_transpose<T>, grid, dim3(256), lds, st, (const V_t<T>*)src, (V_t<T>*)dst,
-                       rows, cols, (const V_t<T>*)tw3, tbits, (T)sx, (T)sy);
+                       rows, cols, (const V_t<T>*)tw3, tbits, (T)sx, (T)sy, (const V_t<T>*)K,
+                       (V_t<T>*)dc);
     return hipGetLastError();
 }
 

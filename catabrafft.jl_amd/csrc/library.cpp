@@ -684,7 +684,7 @@ static int plan_conv(Ctx* ctx, Plan& p, const PlanOptions& o) {
     so.n1 = 0;
     rc = plan_fft_core(ctx, s, so);
     if (rc) return rc;
-    if (s.kind != Kind::Direct && s.kind != Kind::FourStep)
+    if (s.kind != Kind::Direct && s.kind != Kind::FourStep && s.kind != Kind::Large)
         return fail(CFFT_ERR_UNSUPPORTED, "convolution length not plannable");
     if (!o.alloc_work) return CFFT_OK;  // the caller supplies per-transform work buffers
     const size_t wbytes = (size_t)p.batch * p.L * esize(p.cprec);
@@ -940,7 +940,6 @@ static int run_fft_s(const Plan& p, const void* in, void* out, int sgn, double s
             return run_fourstep(p, in, out, out, p.idist, p.odist, p.odist, sgn, scale, epi, st, B);
         }
         case Kind::Large: {  // six steps per transform (plan_large)
-            if (epi.K) return fail(CFFT_ERR_UNSUPPORTED, "six-step plans take no epilogue");
             const FourStepPlan& fs = p.fs;
             const int64_t n = p.n, M = fs.N2, N1 = fs.N1;
             const size_t es = esize(p.prec);
@@ -971,12 +970,14 @@ static int run_fft_s(const Plan& p, const void* in, void* out, int sgn, double s
                 a.xcd = 1;
                 CFFT_HIP(launch_fourstep_a(p.prec, inb, mid, a, fs.a.pt, fs.a.dtw.p, st));
                 CFFT_HIP(launch_transpose(p.prec, mid, rowsb, M, N1, fs.tw3.p, fs.tbits, 1.0, 1.0,
-                                          st));
+                                          nullptr, nullptr, st));
                 int rc = run_fft(*p.sub, rowsb, rowsb, 1, 1.0, ConvEpilogue{nullptr, nullptr}, st);
                 if (rc) return rc;
                 char* res = inplace ? (char*)p.scratch.p : outb;
+                // a convolution's epilogue (dc: two slots per line) goes into the last step
+                void* dcb = epi.dc ? (char*)epi.dc + (size_t)b * 2 * es : nullptr;
                 CFFT_HIP(launch_transpose(p.prec, rowsb, res, N1, M, nullptr, 0, scale,
-                                          scale * sgn, st));
+                                          scale * sgn, epi.K, dcb, st));
                 if (inplace)
                     CFFT_HIP(hipMemcpyAsync(outb, res, (size_t)n * es, hipMemcpyDeviceToDevice, st));
             }

@@ -167,7 +167,9 @@ static int srv_plan(Server* s, int64_t n, int dtype, Plan** out, size_t* new_byt
         PlanOptions o;
         o.alloc_work = false;  // a large convolution's work buffers come with each request
         if ((rc = plan_build(s->ctx, *p, o))) return rc;
-        if (p->kind == Kind::Large)  // one plan-owned scratch: rounds on several streams would race
+        // six-step plans (or convolutions over them) own one scratch: rounds on several streams
+        // would race
+        if (p->kind == Kind::Large || (p->sub && p->sub->kind == Kind::Large))
             return fail(CFFT_ERR_UNSUPPORTED, "the server takes lengths the four-step covers");
     }
     *new_bytes = plan_device_bytes(*p);

@@ -515,7 +515,7 @@ int uneven_create(Ctx* ctx, int64_t count, const int64_t* lengths, const int64_t
         } else {
             PlanOptions o;
             if ((rc = plan_build(ctx, *p, o))) return rc;
-            if (p->kind == Kind::Large)
+            if (p->kind == Kind::Large || (p->sub && p->sub->kind == Kind::Large))
                 return fail(CFFT_ERR_UNSUPPORTED, "uneven batches take lengths the four-step "
                                                   "covers (n=" + std::to_string(n) + " needs six steps)");
         }

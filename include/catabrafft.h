@@ -82,8 +82,9 @@ CFFT_API int cfft_finalize(cfft_ctx ctx);
 
 /* A plan for `batch` transforms of length n: row b of the input starts at element b*idist,
  * row b of the output at element b*odist (idist, odist >= n).  dtype CFFT_C32 / CFFT_C64 /
- * CFFT_C16.  n up to 2^30: lengths beyond the four-step's reach (16384^2 in C32, 8192^2 in
- * C64) run as six steps (about five reads and writes per element).  flags CFFT_NO_FLAG (default
+ * CFFT_C16.  n up to 2^30: lengths the four-step cannot split into two workgroup-sized
+ * factors (lines of at most 16384 points in C32, about 9800 in C64) run as six steps (about
+ * five reads and writes per element), as do the convolution FFTs of such lengths.  flags CFFT_NO_FLAG (default
  * heuristics) / CFFT_MEASURE (time the candidate kernel configurations on the device and keep
  * the fastest) / CFFT_ENCHANT (wider search). */
 CFFT_API int cfft_plan_1d(cfft_ctx ctx, int64_t n, int64_t batch, int64_t idist, int64_t odist,

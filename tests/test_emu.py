@@ -375,3 +375,29 @@ def test_emulated_sixstep_matches_oracle(E, tmp_path, n, n1, batch, dt):
     lib.cfft_plan_destroy(p)
     lib.cfft_plan_destroy(q)
     lib.cfft_finalize(c2)
+
+
+@pytest.mark.parametrize("n,batch,dt,kind", [(10007, 2, np.complex64, "rader-large"),
+                                             (3 * 10007, 1, np.complex128, "bluestein-large"),
+                                             (65537, 1, np.complex64, "rader-large")])
+def test_emulated_convolution_over_sixstep(E, tmp_path, n, batch, dt, kind):
+    """A large convolution whose length-L FFT runs as six steps (forced through wisdom): the
+    convolution epilogue (K, the A^[0] slots) is applied by the last transpose."""
+    lib, ctx = E
+    dcode = 0 if dt == np.complex64 else 1
+    path = tmp_path / "w.txt"
+    path.write_text(f"{n} {batch} {n} {n} {dcode} 1 : 0 64 0 0 0 -1 -1 1\n")
+    c2 = ctypes.c_void_p()
+    assert lib.cfft_init(0, ctypes.byref(c2)) == 0
+    assert lib.cfft_wisdom_import(c2, str(path).encode()) == 0
+    p = _plan(lib, c2, n, batch, dt, flags=1)
+    d = _desc(lib, p)
+    assert kind in d and "sixstep" in d, d
+    x = seeded_complex(n, batch=batch, seed=3, dtype=dt)
+    for direction, norm in ((-1, 0), (1, 1)):
+        y = np.zeros_like(x)
+        assert lib.cfft_exec(p, x.ctypes.data, y.ctypes.data, direction, norm, None) == 0
+        ref = O.fft(x.astype(np.complex128), "d", direction, bool(norm))
+        assert rel_l2(y, ref) < (2e-6 if dt == np.complex64 else 1e-13), (direction, norm)
+    lib.cfft_plan_destroy(p)
+    lib.cfft_finalize(c2)

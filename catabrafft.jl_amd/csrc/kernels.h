@@ -148,10 +148,12 @@ hipError_t launch_fourstep_a(Prec p, const void* in, void* out, const FourStepAA
 // Transpose of a rows x cols matrix: dst[c rows + r] = src[r cols + c], times w_n^{r c} from the
 // three-level table tw3 (nullptr: no twiddle); then, at output index o, the convolution
 // epilogue when K is set (dc = the value at o = 0, then conj(v K[o])), then (re sx, im sy).
-// The steps between the passes of transforms too long for the four-step (the six-step path).
+// The steps between the passes of transforms too long for the four-step (the six-step path),
+// and of N-D axes that one workgroup cannot take as strided lines.  batch > 1: that many
+// matrices back to back (no twiddle or epilogue then).
 hipError_t launch_transpose(Prec p, const void* src, void* dst, int64_t rows, int64_t cols,
-                            const void* tw3, int tbits, double sx, double sy, const void* K,
-                            void* dc, hipStream_t st);
+                            int64_t batch, const void* tw3, int tbits, double sx, double sy,
+                            const void* K, void* dc, hipStream_t st);
 // Persistent pipelined four-step passes (a.grid workgroups of at most kPipeMaxThreads, lines of
 // at least two passes, no epilogue); twn = entries of the line's twiddle table, copied into LDS.
 // fs_pipe_lds gives the LDS bytes (pass 1 = B: lines, pass twiddles, then the three-level table

@@ -92,10 +92,11 @@ hipError_t launch_real_rows(Prec p, const void* in, void* out, const RealArgs& a
                           : real_rows_f64(in, out, a, pt, tw, E, st);
 }
 hipError_t launch_transpose(Prec p, const void* src, void* dst, int64_t rows, int64_t cols,
-                            const void* tw3, int tbits, double sx, double sy, const void* K,
-                            void* dc, hipStream_t st) {
-    return p == Prec::F32 ? transpose_f32(src, dst, rows, cols, tw3, tbits, sx, sy, K, dc, st)
-                          : transpose_f64(src, dst, rows, cols, tw3, tbits, sx, sy, K, dc, st);
+                            int64_t batch, const void* tw3, int tbits, double sx, double sy,
+                            const void* K, void* dc, hipStream_t st) {
+    return p == Prec::F32
+               ? transpose_f32(src, dst, rows, cols, batch, tw3, tbits, sx, sy, K, dc, st)
+               : transpose_f64(src, dst, rows, cols, batch, tw3, tbits, sx, sy, K, dc, st);
 }
 hipError_t launch_real_aux(Prec p, const void* in, void* out, const RealAuxArgs& a,
                            hipStream_t st) {

@@ -38,9 +38,9 @@ hipError_t real_rows_f32(const void* in, void* out, const RealArgs& a, const Pas
     return CFFT_E(real_rows_t, float, E, in, out, a, pt, tw, st);
 }
 hipError_t transpose_f32(const void* src, void* dst, int64_t rows, int64_t cols,
-                          const void* tw3, int tbits, double sx, double sy, const void* K,
-                          void* dc, hipStream_t st) {
-    return transpose_t<float>(src, dst, rows, cols, tw3, tbits, sx, sy, K, dc, st);
+                          int64_t batch, const void* tw3, int tbits, double sx, double sy,
+                          const void* K, void* dc, hipStream_t st) {
+    return transpose_t<float>(src, dst, rows, cols, batch, tw3, tbits, sx, sy, K, dc, st);
 }
 hipError_t real_aux_f32(const void* in, void* out, const RealAuxArgs& a, hipStream_t st) {
     return real_aux_t<float>(in, out, a, st);

@@ -405,7 +405,7 @@ k_fs_b(const V_t<T>* in, GO* out, FourStepBArgs a, PassTable pt,
 // six-step path).  A 32 x 32
 // tile goes through LDS (one pad word per row), so both the reads (along a source row) and the
 // writes (along a destination row) are coalesced.  Grid: x over column tiles, y over row tiles
-// (y wraps when there are more than 65535).
+// (y wraps when there are more than 65535), z over the matrices of a batch.
 // ------------------------------------------------------------------------------------------
 constexpr int kTrTile = 32;
 
@@ -415,6 +415,8 @@ k_transpose(const V_t<T>* __restrict__ src, V_t<T>* __restrict__ dst, int64_t ro
             const V_t<T>* __restrict__ tw3, int tbits, T sx, T sy, const V_t<T>* __restrict__ K,
             V_t<T>* dc) {
     CFFT_DYN_LDS(smem);
+    src += (int64_t)blockIdx.z * rows * cols;  // matrix z of a batch (no twiddle or epilogue)
+    dst += (int64_t)blockIdx.z * rows * cols;
     V_t<T>(*tile)[kTrTile + 1] = reinterpret_cast<V_t<T>(*)[kTrTile + 1]>(smem);
     const int tx = threadIdx.x & (kTrTile - 1), ty = threadIdx.x >> 5;  // 32 x 8 threads
     const int64_t c0 = (int64_t)blockIdx.x * kTrTile;
@@ -449,17 +451,25 @@ k_transpose(const V_t<T>* __restrict__ src, V_t<T>* __restrict__ dst, int64_t ro
 }
 
 template <typename T>
-hipError_t transpose_t(const void* src, void* dst, int64_t rows, int64_t cols, const void* tw3,
-                       int tbits, double sx, double sy, const void* K, void* dc, hipStream_t st) {
+hipError_t transpose_t(const void* src, void* dst, int64_t rows, int64_t cols, int64_t batch,
+                       const void* tw3, int tbits, double sx, double sy, const void* K, void* dc,
+                       hipStream_t st) {
     const int64_t ct = (cols + kTrTile - 1) / kTrTile, rt = (rows + kTrTile - 1) / kTrTile;
-    if (ct <= 0 || rt <= 0) return hipSuccess;
-    if (ct > 0x7fffffff) return hipErrorInvalidValue;
-    const dim3 grid((unsigned)ct, (unsigned)(rt < 65535 ? rt : 65535));
+    if (ct <= 0 || rt <= 0 || batch <= 0) return hipSuccess;
+    if (ct > 0x7fffffff || (batch > 1 && (tw3 || K))) return hipErrorInvalidValue;
     const size_t lds = sizeof(V_t<T>) * kTrTile * (kTrTile + 1);
-    hipLaunchKernelGGL(k_transpose<T>, grid, dim3(256), lds, st, (const V_t<T>*)src, (V_t<T>*)dst,
-                       rows, cols, (const V_t<T>*)tw3, tbits, (T)sx, (T)sy, (const V_t<T>*)K,
-                       (V_t<T>*)dc);
-    return hipGetLastError();
+    const int64_t mat = rows * cols;
+    for (int64_t z0 = 0; z0 < batch; z0 += 65535) {
+        const int64_t nz = batch - z0 < 65535 ? batch - z0 : 65535;
+        const dim3 grid((unsigned)ct, (unsigned)(rt < 65535 ? rt : 65535), (unsigned)nz);
+        hipLaunchKernelGGL(k_transpose<T>, grid, dim3(256), lds, st,
+                           (const V_t<T>*)src + z0 * mat, (V_t<T>*)dst + z0 * mat, rows, cols,
+                           (const V_t<T>*)tw3, tbits, (T)sx, (T)sy, (const V_t<T>*)K,
+                           (V_t<T>*)dc);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 // ------------------------------------------------------------------------------------------

@@ -24,11 +24,11 @@ hipError_t real_rows_f32(const void* in, void* out, const RealArgs& a, const Pas
 hipError_t real_rows_f64(const void* in, void* out, const RealArgs& a, const PassTable& pt,
                          const void* tw, int E, hipStream_t st);
 hipError_t real_aux_f32(const void* in, void* out, const RealAuxArgs& a, hipStream_t st);
-hipError_t transpose_f32(const void* src, void* dst, int64_t rows, int64_t cols, const void* tw3,
-                         int tbits, double sx, double sy, const void* K, void* dc,
+hipError_t transpose_f32(const void* src, void* dst, int64_t rows, int64_t cols, int64_t batch,
+                         const void* tw3, int tbits, double sx, double sy, const void* K, void* dc,
                          hipStream_t st);
-hipError_t transpose_f64(const void* src, void* dst, int64_t rows, int64_t cols, const void* tw3,
-                         int tbits, double sx, double sy, const void* K, void* dc,
+hipError_t transpose_f64(const void* src, void* dst, int64_t rows, int64_t cols, int64_t batch,
+                         const void* tw3, int tbits, double sx, double sy, const void* K, void* dc,
                          hipStream_t st);
 hipError_t real_aux_f64(const void* in, void* out, const RealAuxArgs& a, hipStream_t st);
 void init_kernels_f32();

@@ -969,18 +969,36 @@ static int run_fft_s(const Plan& p, const void* in, void* out, int sgn, double s
                 a.sgn_in = sgn;
                 a.xcd = 1;
                 CFFT_HIP(launch_fourstep_a(p.prec, inb, mid, a, fs.a.pt, fs.a.dtw.p, st));
-                CFFT_HIP(launch_transpose(p.prec, mid, rowsb, M, N1, fs.tw3.p, fs.tbits, 1.0, 1.0,
-                                          nullptr, nullptr, st));
+                CFFT_HIP(launch_transpose(p.prec, mid, rowsb, M, N1, 1, fs.tw3.p, fs.tbits, 1.0,
+                                          1.0, nullptr, nullptr, st));
                 int rc = run_fft(*p.sub, rowsb, rowsb, 1, 1.0, ConvEpilogue{nullptr, nullptr}, st);
                 if (rc) return rc;
                 char* res = inplace ? (char*)p.scratch.p : outb;
                 // a convolution's epilogue (dc: two slots per line) goes into the last step
                 void* dcb = epi.dc ? (char*)epi.dc + (size_t)b * 2 * es : nullptr;
-                CFFT_HIP(launch_transpose(p.prec, rowsb, res, N1, M, nullptr, 0, scale,
+                CFFT_HIP(launch_transpose(p.prec, rowsb, res, N1, M, 1, nullptr, 0, scale,
                                           scale * sgn, epi.K, dcb, st));
                 if (inplace)
                     CFFT_HIP(hipMemcpyAsync(outb, res, (size_t)n * es, hipMemcpyDeviceToDevice, st));
             }
+            return CFFT_OK;
+        }
+        case Kind::ColsT: {  // B blocks of n x inner: transpose, row FFTs, transpose back
+            const int64_t n = p.n, inner = p.idist / p.n;
+            const size_t es = esize(p.prec), bytes = (size_t)B * n * inner * es;
+            std::lock_guard<std::mutex> g(p.scratch_mu);
+            if (p.scratch.bytes < bytes) {
+                int rc = p.scratch.alloc(bytes);
+                if (rc) return rc;
+            }
+            CFFT_HIP(launch_transpose(p.prec, in, p.scratch.p, n, inner, B, nullptr, 0, 1.0, 1.0,
+                                      nullptr, nullptr, st));
+            ExecShape rs;
+            rs.batch = B * inner;
+            int rc = run_fft_s(*p.sub, p.scratch.p, p.scratch.p, sgn, scale, epi, st, rs);
+            if (rc) return rc;
+            CFFT_HIP(launch_transpose(p.prec, p.scratch.p, out, inner, n, B, nullptr, 0, 1.0, 1.0,
+                                      nullptr, nullptr, st));
             return CFFT_OK;
         }
         case Kind::Cols: {  // fs.N1 adjacent lines of length fs.N2 (stride N1), batch blocks
@@ -1210,11 +1228,32 @@ int plan_nd_create(Ctx* ctx, int rank, const int64_t* dims, int64_t axes_mask, i
             PlanOptions o;
             if ((rc = plan_build(ctx, *s, o))) return rc;
             os << " | axis " << a << ": " << describe(*s);
+        } else if (!line_fits(ctx, dims[a], p->prec, 16) || inner > (int64_t)1 << 30) {
+            // a strided axis that one workgroup cannot take (not smooth, or too long): each
+            // outer block (dims[a] x inner) is transposed so that the axis is contiguous, one
+            // batched 1-D plan of any kind runs over the outer * inner rows, and the rows are
+            // transposed back
+            const int64_t N = dims[a];
+            s->kind = Kind::ColsT;
+            s->n = N;
+            s->batch = outer;
+            s->idist = s->odist = inner * N;
+            s->fs.N1 = (int)std::min<int64_t>(inner, INT32_MAX);
+            s->sub.reset(new Plan());
+            Plan& r = *s->sub;
+            r.ctx = ctx;
+            r.n = N;
+            r.batch = outer * inner;
+            r.idist = r.odist = N;
+            r.prec = p->prec;
+            r.dtype = sdtype;
+            r.flags = flags;
+            PlanOptions o;
+            if ((rc = plan_build(ctx, r, o))) return rc;
+            os << " | axis " << a << ": transposed lines N=" << N << " x " << inner << " {"
+               << describe(r) << "}";
         } else {
             const int64_t N2 = dims[a];
-            if (!line_fits(ctx, N2, p->prec, 16) || inner > (int64_t)1 << 30)
-                return fail(CFFT_ERR_UNSUPPORTED, "strided axis of length " + std::to_string(N2) +
-                                                      " must be smooth and fit one workgroup");
             s->kind = Kind::Cols;
             s->n = N2;
             s->batch = outer;

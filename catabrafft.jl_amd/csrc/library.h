@@ -46,7 +46,7 @@ struct FourStepPlan {
     int64_t chunk = 0;  // transforms per pass-A/pass-B round (0 = the whole batch)
 };
 
-enum class Kind : int { Trivial, Direct, FourStep, ConvSmall, ConvLarge, Uneven, Cols, ND, Large };
+enum class Kind : int { Trivial, Direct, FourStep, ConvSmall, ConvLarge, Uneven, Cols, ND, Large, ColsT };
 
 struct Ctx;
 

@@ -93,8 +93,9 @@ CFFT_API int cfft_plan_1d(cfft_ctx ctx, int64_t n, int64_t batch, int64_t idist,
 /* Multi-dimensional transform of one row-major array dims[0] x ... x dims[rank-1] (the last
  * axis contiguous) over the axes whose bit is set in axes_mask (bit a = axis a); normalize
  * divides by the product of the transformed lengths.  Replaces plan_fft's `region` argument
- * (CatabraFFT.jl:169, accepted and ignored there).  Every strided axis must be smooth and fit
- * one workgroup (<= 16384 points in C32, <= 8192 in C64). */
+ * (CatabraFFT.jl:169, accepted and ignored there).  Any axis length: strided axes that one
+ * workgroup cannot take as strided lines (not smooth, or longer than a workgroup's line) are
+ * transposed, transformed as contiguous rows and transposed back. */
 CFFT_API int cfft_plan_nd(cfft_ctx ctx, int rank, const int64_t* dims, int64_t axes_mask,
                           int dtype, int flags, cfft_plan* plan);
 

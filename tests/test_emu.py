@@ -401,3 +401,43 @@ def test_emulated_convolution_over_sixstep(E, tmp_path, n, batch, dt, kind):
         assert rel_l2(y, ref) < (2e-6 if dt == np.complex64 else 1e-13), (direction, norm)
     lib.cfft_plan_destroy(p)
     lib.cfft_finalize(c2)
+
+
+def _oracle_nd(x, axes, direction, normalize):
+    y = x.astype(np.complex128)
+    for a in axes:
+        y = np.moveaxis(y, a, -1)
+        shp = y.shape
+        y = O.fft(np.ascontiguousarray(y.reshape(-1, shp[-1])), "d", direction,
+                  normalize).reshape(shp)
+        y = np.moveaxis(y, -1, a)
+    return y
+
+
+@pytest.mark.parametrize("shape,axes,dt", [((17, 100), (0, 1), np.complex64),
+                                           ((13, 17, 40), (0, 1, 2), np.complex128),
+                                           ((97, 6, 5), (0,), np.complex64),
+                                           ((20000, 3), (0, 1), np.complex64),
+                                           ((4, 12000, 2), (1,), np.complex128)])
+def test_emulated_nd_transposed_axes(E, shape, axes, dt):
+    """N-D axes one workgroup cannot take as strided lines (not smooth: 17, 97; too long: 20000
+    in F32, 12000 in F64) run as transpose, batched 1-D plan, transpose; against the oracle."""
+    lib, ctx = E
+    rank = len(shape)
+    dims = (ctypes.c_int64 * rank)(*shape)
+    mask = sum(1 << (a % rank) for a in axes)
+    p = ctypes.c_void_p()
+    assert lib.cfft_plan_nd(ctx, rank, dims, mask, 0 if dt == np.complex64 else 1, 0,
+                            ctypes.byref(p)) == 0, lib.cfft_last_error()
+    assert "transposed" in _desc(lib, p), _desc(lib, p)
+    x = seeded_complex(int(np.prod(shape)), seed=rank, dtype=dt).reshape(shape)
+    for direction, norm in ((-1, 0), (1, 1)):
+        y = np.zeros_like(x)
+        assert lib.cfft_exec(p, x.ctypes.data, y.ctypes.data, direction, norm, None) == 0
+        ref = _oracle_nd(x, [a % rank for a in axes], direction, bool(norm))
+        n_all = max(int(np.prod([shape[a % rank] for a in axes])), 2)
+        assert rel_l2(y, ref) < _tol(n_all, dt), (direction, norm)
+    z = x.copy()
+    assert lib.cfft_exec(p, z.ctypes.data, z.ctypes.data, -1, 0, None) == 0
+    assert rel_l2(z, _oracle_nd(x, [a % rank for a in axes], -1, False)) < _tol(max(n_all, 2), dt)
+    lib.cfft_plan_destroy(p)

@@ -34,3 +34,22 @@ def test_nd_matches_oracle(gpu_ready, shape, axes, dt):
     assert rel_l2(y, oracle_nd(x, axes)) < tol
     back = C.ifftn(y, axes)
     assert rel_l2(back, x) < 2 * tol
+
+
+@pytest.mark.parametrize("shape,axes", [((17, 1000), (0, 1)), ((1009, 64), (0,)),
+                                        ((20000, 8), (0, 1)), ((3, 40000, 2), (1,)),
+                                        ((5, 17, 19, 8), (0, 1, 2, 3))])
+@pytest.mark.parametrize("dt", [np.complex64, np.complex128])
+def test_nd_transposed_axes_match_oracle(gpu_ready, shape, axes, dt):
+    """Strided axes one workgroup cannot take (not smooth, or longer than a workgroup's line)
+    run as transpose, batched 1-D plan of any kind, transpose back."""
+    import catabrafft as C
+
+    x = seeded_complex(int(np.prod(shape)), seed=13, dtype=dt).reshape(shape)
+    tol = 2e-6 if dt == np.complex64 else 64 * np.finfo(float).eps
+    p = C.NDPlan(shape, axes, dt)
+    assert "transposed" in p.describe(), p.describe()
+    y = C.fftn(x, axes)
+    assert rel_l2(y, oracle_nd(x, axes)) < tol
+    back = C.ifftn(y, axes)
+    assert rel_l2(back, x) < 2 * tol

@@ -49,20 +49,66 @@ __device__ __forceinline__ void store_elem(ch16& g, V_t<T> v) {
     g = ch16{(CFFT_F16)(float)v.x, (CFFT_F16)(float)v.y};
 }
 
-// Global-memory line input of storage type G; sg = -1 conjugates.
-template <typename T, typename G = V_t<T>>
+// Streaming (non-temporal) accesses, for data touched once in whole lines: the line inputs and
+// outputs of single-plan rows kernels and the four-step middle written by pass A.  They keep
+// such streams from evicting what other workgroups still read from the caches.  Column accesses
+// (runs of C elements per row, whose 128-byte lines the neighbouring tiles' workgroups read
+// through L2) stay ordinary: streaming loads there made the 2^20 F32 four-step 28 % slower.  So do
+// work-list launches, whose convolution and four-step intermediates are read again by the next
+// launch while still cached, and ComplexF16 lines.
+#ifndef CFFT_PLAIN_STREAMING
+template <typename T> struct StreamVec;
+template <> struct StreamVec<float> { typedef float type __attribute__((ext_vector_type(2))); };
+template <> struct StreamVec<double> { typedef double type __attribute__((ext_vector_type(2))); };
+template <typename T>
+__device__ __forceinline__ V_t<T> load_stream(const V_t<T>& g) {
+    typedef typename StreamVec<T>::type W;
+    const W w = __builtin_nontemporal_load(reinterpret_cast<const W*>(&g));
+    return V_t<T>{w.x, w.y};
+}
+template <typename T>
+__device__ __forceinline__ void store_stream(V_t<T>& g, V_t<T> v) {
+    typedef typename StreamVec<T>::type W;
+    const W w = {v.x, v.y};
+    __builtin_nontemporal_store(w, reinterpret_cast<W*>(&g));
+}
+#else
+template <typename T>
+__device__ __forceinline__ V_t<T> load_stream(const V_t<T>& g) { return g; }
+template <typename T>
+__device__ __forceinline__ void store_stream(V_t<T>& g, V_t<T> v) { g = v; }
+#endif
+template <typename T>
+__device__ __forceinline__ V_t<T> load_stream(const ch16& g) { return load_elem<T>(g); }
+template <typename T>
+__device__ __forceinline__ void store_stream(ch16& g, V_t<T> v) { store_elem<T>(g, v); }
+// S selects streaming (true) or ordinary (false) accesses.
+template <typename T, bool S, typename G>
+__device__ __forceinline__ V_t<T> load_line_elem(const G& g) {
+    if constexpr (S) return load_stream<T>(g);
+    else return load_elem<T>(g);
+}
+template <typename T, bool S, typename G>
+__device__ __forceinline__ void store_line_elem(G& g, V_t<T> v) {
+    if constexpr (S) store_stream<T>(g, v);
+    else store_elem<T>(g, v);
+}
+
+// Global-memory line input of storage type G, read once (S: streaming); sg = -1 conjugates.
+template <typename T, typename G = V_t<T>, bool S = true>
 struct GSrc {
     const G* p;
     T sg;
     __device__ __forceinline__ V_t<T> operator()(int e) const {
-        V_t<T> v = load_elem<T>(p[e]);
+        V_t<T> v = load_line_elem<T, S>(p[e]);
         v.y *= sg;
         return v;
     }
 };
 
-// Global-memory line output with scale, conjugation and the optional convolution epilogue.
-template <typename T, typename G = V_t<T>>
+// Global-memory line output (S: streaming) with scale, conjugation and the optional convolution
+// epilogue.
+template <typename T, typename G = V_t<T>, bool S = true>
 struct GRowDst {
     G* p;
     T sx, sy;
@@ -73,15 +119,15 @@ struct GRowDst {
             if (dc && e == 0) *dc = v;
             v = cconj(cmul(v, K[e]));
         }
-        store_elem<T>(p[e], V_t<T>{v.x * sx, v.y * sy});
+        store_line_elem<T, S>(p[e], V_t<T>{v.x * sx, v.y * sy});
     }
 };
 
-// Plain global-memory line output.
-template <typename T>
+// Plain global-memory line output (S: streaming).
+template <typename T, bool S = true>
 struct GPlainDst {
     V_t<T>* p;
-    __device__ __forceinline__ void put(int e, V_t<T> v) const { p[e] = v; }
+    __device__ __forceinline__ void put(int e, V_t<T> v) const { store_line_elem<T, S>(p[e], v); }
 };
 
 // Global-memory line with elements `stride` apart (a column of a row-major matrix); sg = -1
@@ -148,6 +194,7 @@ __device__ __forceinline__ LdsLine<T> lds_line(unsigned char* smem, int C, int L
 // ------------------------------------------------------------------------------------------
 template <typename T, typename GI = V_t<T>, typename GO = V_t<T>>
 struct UniformLines {  // line c at in0 + c*idist / out0 + c*odist, slots dc0 + 2c
+    static constexpr bool kStream = true;
     const GI* in0;
     GO* out0;
     int64_t idist, odist;
@@ -159,6 +206,7 @@ struct UniformLines {  // line c at in0 + c*idist / out0 + c*odist, slots dc0 + 
 
 template <typename T>
 struct ItemLines {  // an MItem's lines: uniform distances or a table of {in, out, dc}
+    static constexpr bool kStream = false;
     uint64_t ib, ob;
     const MItem* it;
     __device__ __forceinline__ const uint64_t* entry(int c) const {
@@ -197,8 +245,9 @@ __device__ __forceinline__ void rows_body(unsigned char* smem, const LINES& line
     const int li = active ? c : 0;
     using GI = std::remove_cv_t<std::remove_pointer_t<decltype(lines.in(0))>>;
     using GO = std::remove_pointer_t<decltype(lines.out(0))>;
-    GSrc<T, GI> gs{lines.in(li), (T)sgn};
-    GRowDst<T, GO> gd{lines.out(li), (T)scale, (T)(scale * sgn), K, K ? lines.dc(li) : nullptr};
+    constexpr bool S = LINES::kStream;
+    GSrc<T, GI, S> gs{lines.in(li), (T)sgn};
+    GRowDst<T, GO, S> gd{lines.out(li), (T)scale, (T)(scale * sgn), K, K ? lines.dc(li) : nullptr};
     if (pt.npass == 0) {
         if (active && t == 0) gd.put(0, gs(0));
         return;
@@ -230,7 +279,7 @@ k_rows(const G* __restrict__ in, G* __restrict__ out, RowsArgs a, PassTable pt,
 // Threads are mapped column first (lane = c + C t): a wave's accesses to a row of the matrix
 // cover C adjacent columns, so the first pass reads global memory directly in runs of C
 // elements, with no staging through LDS.
-template <typename T, int E, typename GI>
+template <typename T, int E, typename GI, bool S = true>
 __device__ __forceinline__ void fs_a_body(unsigned char* smem, const GI* src, V_t<T>* dst,
                                           int N1, int N2, int C, int Tt, int Ls, int j20,
                                           int sgn, const PassTable& pt, const V_t<T>* tw) {
@@ -243,7 +292,7 @@ __device__ __forceinline__ void fs_a_body(unsigned char* smem, const GI* src, V_
     const int j2c = active ? j2 : 0;
     const LdsLine<T> L = lds_line<T>(smem, C, Ls, c);
     GStrideSrc<T, GI> gs{src + j2c, N2, (T)sgn};
-    GPlainDst<T> d{dst + (int64_t)j2c * N1};
+    GPlainDst<T, S> d{dst + (int64_t)j2c * N1};
     run_line<T, E>(pt, N1, t, Tt, active, tw, gs, d, L, true, true);
 }
 
@@ -910,7 +959,7 @@ k_fs_a_multi(MultiArgs ma, const MItem* __restrict__ items, const MPlan* __restr
     int sgn;
     double scale;
     item_direction(it, P->n, ma, sgn, scale);
-    fs_a_body<T, kMultiE>(smem, mptr<const V_t<T>>(it.in, it.flags & kSelInMask, ma.in_base, ma.out_base),
+    fs_a_body<T, kMultiE, V_t<T>, false>(smem, mptr<const V_t<T>>(it.in, it.flags & kSelInMask, ma.in_base, ma.out_base),
                           mptr<V_t<T>>(it.out, (it.flags >> kSelOutShift) & 15, ma.in_base, ma.out_base),
                           P->N1, P->N2, P->CA, P->la.Tt, P->la.Ls, it.count * P->CA, sgn,
                           P->la.pt, (const V_t<T>*)P->la.tw);

@@ -18,6 +18,8 @@
 #define __shared__
 #define CFFT_DYN_LDS(name) unsigned char* name = emu::smem()
 #define CFFT_OPAQUE(x) asm volatile("" : "+r"(x))
+// Non-temporal (streaming) global accesses are a cache hint on the device; here they are plain.
+#define CFFT_PLAIN_STREAMING 1
 
 struct float2 {
     float x, y;

@@ -15,6 +15,8 @@
 #include <cstring>
 #include <mutex>
 #include <cmath>
+#include <memory>
+#include <set>
 #include <sstream>
 #include <vector>
 
@@ -129,11 +131,16 @@ int wisdom_export(Ctx* ctx, const char* path) {
     return CFFT_OK;
 }
 
+// Imported choices replace the context's cached plans of the same keys, so that later plan
+// requests follow them; plans already handed out stay valid (they are shared) and are released
+// after the lock.
 int wisdom_import(Ctx* ctx, const char* path) {
     FILE* f = std::fopen(path, "r");
     if (!f) return fail(CFFT_ERR_INVALID, std::string("cannot read ") + path);
     char line[512];
     int n = 0;
+    std::set<std::string> keys;
+    std::vector<std::shared_ptr<Plan>> dropped;
     std::lock_guard<std::mutex> g(ctx->mu);
     while (std::fgets(line, sizeof line, f)) {
         if (line[0] == '#') continue;
@@ -157,9 +164,22 @@ int wisdom_import(Ctx* ctx, const char* path) {
         o.chunk = chunk;
         o.explicit_opts = true;
         ctx->wisdom[key] = o;
+        keys.insert(key);
         ++n;
     }
     std::fclose(f);
+    for (auto it = ctx->cache.begin(); it != ctx->cache.end();) {
+        const PlanKey& k = it->first;
+        std::ostringstream s;
+        s << std::get<0>(k) << " " << std::get<1>(k) << " " << std::get<2>(k) << " "
+          << std::get<3>(k) << " " << std::get<4>(k) << " " << std::get<5>(k);
+        if (keys.count(s.str())) {
+            dropped.push_back(std::move(it->second));
+            it = ctx->cache.erase(it);
+        } else {
+            ++it;
+        }
+    }
     return n;
 }
 

@@ -136,7 +136,10 @@ CFFT_API int cfft_plan_destroy(cfft_plan plan);
 CFFT_API int cfft_empty_cache(cfft_ctx ctx);
 
 /* Wisdom: the choices MEASURE / ENCHANT planning made, as a text file, so a later process
- * reuses them without timing again (the plan serialisation drafted in spells.jl:180-374). */
+ * reuses them without timing again (the plan serialisation drafted in spells.jl:180-374).
+ * Import replaces the context's cached plans of the imported keys (n, batch, idist, odist,
+ * dtype, flags): later MEASURE / ENCHANT requests for them are built from the imported
+ * choices.  Plans already returned stay valid and unchanged. */
 CFFT_API int cfft_wisdom_export(cfft_ctx ctx, const char* path);
 CFFT_API int cfft_wisdom_import(cfft_ctx ctx, const char* path);
 

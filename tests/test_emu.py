@@ -203,6 +203,20 @@ def test_emulated_wisdom_round_trip(E, tmp_path):
     assert lib.cfft_exec(q, x.ctypes.data, y.ctypes.data, -1, 0, None) == 0
     assert rel_l2(y, O.fft(x.astype(np.complex128), "d")) < 1e-6
     assert lib.cfft_wisdom_import(ctx2, b"/nonexistent/wisdom") == -1
+    # an import replaces the cached plan of its key: the next request follows the new choice
+    forced = tmp_path / "forced.txt"
+    forced.write_text("2048 4 2048 2048 0 1 : 16 64 32 2 16 -1 0 0\n")
+    assert lib.cfft_wisdom_import(ctx2, str(forced).encode()) == 0
+    r = _plan(lib, ctx2, 2048, 4, np.complex64, flags=1)
+    d = _desc(lib, r)
+    assert "(wisdom)" in d and "N1=32" in d and "CA=2" in d and "CB=16" in d, d
+    y2 = np.zeros_like(x)
+    assert lib.cfft_exec(q, x.ctypes.data, y2.ctypes.data, -1, 0, None) == 0  # q still valid
+    assert np.array_equal(y, y2)
+    assert lib.cfft_exec(r, x.ctypes.data, y2.ctypes.data, -1, 0, None) == 0
+    assert rel_l2(y2, O.fft(x.astype(np.complex128), "d")) < 1e-6
+    for h in (p, q, r):
+        lib.cfft_plan_destroy(h)
     lib.cfft_finalize(ctx2)
 
 

@@ -1,0 +1,90 @@
+"""Tuned plans against the oracle: MEASURE / ENCHANT plans, and every family of configuration
+the autotuner can pick, forced through wisdom.
+
+The reference's MEASURE / ENCHANT planner (radix_plan.jl:23-68, radix_exec.jl:140-297) times
+candidate radix decompositions and keeps the fastest; each candidate computes the same DFT.
+Here the candidates are kernel configurations of one transform (threads per line, per-thread
+capacity E, the four-step split, its tile widths, chunking, the pipelined kernels, the six-step
+path), so whichever one the timing picks must match the oracle to the same tolerance as the
+default plan:  ComplexF64 4 * eps64 * log2(n), ComplexF32 1e-6 (relative L2).
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_l2, seeded_complex
+from test_gpu_parity import oracle_ref, run_dev, tol
+
+pytestmark = pytest.mark.gpu
+
+# wisdom fields after the key: E target_threads n1 ca cb chunk pipe large
+# (0 = the planner's default for n1 / ca / cb, chunk -1 = default)
+FORCED = [
+    # n, batch, dtype, fields, what the description must show
+    (1 << 16, 6, np.complex64, "16 64 256 2 32 -1 0 0", ["N1=256", "CA=2", "CB=32"]),
+    (1 << 16, 6, np.complex128, "8 64 128 8 4 2 0 0", ["N1=128", "CA=8", "CB=4"]),
+    (1 << 16, 5, np.complex64, "16 64 512 4 16 -1 1 0", ["N1=512", "pipe"]),
+    (1 << 16, 5, np.complex128, "16 64 0 0 0 -1 1 0", ["fourstep", "pipe"]),
+    (1 << 16, 3, np.complex64, "16 64 256 0 0 -1 0 1", ["sixstep"]),
+    (2520, 7, np.complex128, "8 128 40 16 16 -1 0 0", ["N1=40", "CA=16", "CB=16"]),
+    (2520, 7, np.complex64, "16 64 0 0 0 -1 1 0", ["direct", "pipe"]),
+    (1000, 9, np.complex64, "8 512 0 0 0 -1 0 0", ["direct"]),
+]
+
+
+def _check_plan(p, x, dt):
+    n = x.shape[-1]
+    for direction, norm in ((-1, False), (1, True)):
+        y = run_dev(lambda xd: p.exec_torch(xd, direction=direction, normalize=norm), x)
+        err = rel_l2(y, oracle_ref(x, direction, norm))
+        assert err <= tol(n, dt), (p.describe(), direction, err)
+
+
+@pytest.mark.parametrize("n,batch,dt,fields,marks", FORCED)
+def test_forced_configuration_matches_oracle(gpu_ready, tmp_path, n, batch, dt, fields, marks):
+    import catabrafft as C
+
+    code = 0 if dt == np.complex64 else 1
+    w = tmp_path / "wisdom.txt"
+    w.write_text(f"{n} {batch} {n} {n} {code} 1 : {fields}\n")
+    C.import_wisdom(str(w))
+    p = C.Plan(n, batch, dt, flags=C.MEASURE)
+    d = p.describe()
+    print(f"forced n={n} batch={batch} {np.dtype(dt)}: {d}", flush=True)
+    assert "(wisdom)" in d, d
+    for m in marks:
+        assert m in d, (m, d)
+    _check_plan(p, seeded_complex(n, batch, seed=n % 97 + batch, dtype=dt), dt)
+
+
+@pytest.mark.parametrize("n,batch,dt,flags", [(1000, 8, np.complex64, 2),
+                                              (2520, 16, np.complex128, 1),
+                                              (1 << 16, 4, np.complex64, 1),
+                                              (4099, 3, np.complex64, 2)])
+def test_tuned_plan_matches_oracle(gpu_ready, n, batch, dt, flags):
+    import catabrafft as C
+
+    p = C.Plan(n, batch, dt, flags=flags)
+    d = p.describe()
+    print(f"tuned n={n} batch={batch} {np.dtype(dt)} flags={flags}: {d}", flush=True)
+    assert "tuned(" in d or "(wisdom)" in d, d
+    _check_plan(p, seeded_complex(n, batch, seed=n % 89 + 3, dtype=dt), dt)
+
+
+def test_wisdom_round_trip_reproduces_the_tuned_bits(gpu_ready, tmp_path):
+    """A tuned choice exported and imported again plans the same kernels: same bits."""
+    import catabrafft as C
+
+    n, batch, dt = 720, 6, np.complex64
+    x = seeded_complex(n, batch, seed=11, dtype=dt)
+    p1 = C.Plan(n, batch, dt, flags=C.ENCHANT)
+    y1 = run_dev(lambda xd: p1.exec_torch(xd), x)
+    path = tmp_path / "wisdom.txt"
+    C.export_wisdom(str(path))
+    lines = [l for l in path.read_text().splitlines() if l.startswith(f"{n} {batch} {n} {n} 0 2 :")]
+    assert len(lines) == 1, path.read_text()
+    C.import_wisdom(str(path))
+    p2 = C.Plan(n, batch, dt, flags=C.ENCHANT)
+    assert "(wisdom)" in p2.describe(), p2.describe()
+    y2 = run_dev(lambda xd: p2.exec_torch(xd), x)
+    assert np.array_equal(y1.view(np.uint32), y2.view(np.uint32))
+    assert rel_l2(y1, oracle_ref(x)) <= tol(n, dt)

@@ -133,7 +133,8 @@ int wisdom_export(Ctx* ctx, const char* path) {
 
 // Imported choices replace the context's cached plans of the same keys, so that later plan
 // requests follow them; plans already handed out stay valid (they are shared) and are released
-// after the lock.
+// after the lock.  A malformed line ends the import with an error; the lines before it stay
+// imported.
 int wisdom_import(Ctx* ctx, const char* path) {
     FILE* f = std::fopen(path, "r");
     if (!f) return fail(CFFT_ERR_INVALID, std::string("cannot read ") + path);
@@ -141,6 +142,7 @@ int wisdom_import(Ctx* ctx, const char* path) {
     int n = 0;
     std::set<std::string> keys;
     std::vector<std::shared_ptr<Plan>> dropped;
+    bool malformed = false;
     std::lock_guard<std::mutex> g(ctx->mu);
     while (std::fgets(line, sizeof line, f)) {
         if (line[0] == '#') continue;
@@ -157,8 +159,8 @@ int wisdom_import(Ctx* ctx, const char* path) {
         o.pipe = got >= 7 ? pipe : -1;
         o.large = got == 8 ? large : 0;
         if (got < 6) {
-            std::fclose(f);
-            return fail(CFFT_ERR_INVALID, "malformed wisdom line");
+            malformed = true;
+            break;
         }
         o.n1 = n1;
         o.chunk = chunk;
@@ -180,6 +182,7 @@ int wisdom_import(Ctx* ctx, const char* path) {
             ++it;
         }
     }
+    if (malformed) return fail(CFFT_ERR_INVALID, "malformed wisdom line");
     return n;
 }
 

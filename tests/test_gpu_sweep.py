@@ -1,0 +1,59 @@
+"""Seeded sweep of lengths against the oracle.
+
+The size matrix of the reference's tests (test/runtests.jl:4-46) holds powers, a few primes and
+powers of ten.  This sweep adds lengths drawn at random, so that every planning branch meets
+lengths no fixed list names: 7-smooth lengths (direct, four-step), arbitrary integers (mostly
+with a large prime factor: Rader for primes, Bluestein for the rest, fused or large, FP32 or FP64
+convolution) and products of a smooth part with a prime.  Each case runs forward and normalised
+backward against the oracle at the parity tolerances: ComplexF64 4 * eps64 * log2(n),
+ComplexF32 1e-6 (relative L2).
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from test_gpu_parity import oracle_ref, run_dev, tol
+
+pytestmark = pytest.mark.gpu
+
+
+def _smooth(rng, lo, hi):
+    while True:
+        n = 1
+        while n < lo:
+            n *= int(rng.choice([2, 3, 5, 7]))
+        if n <= hi:
+            return n
+
+
+def _lengths():
+    rng = np.random.Generator(np.random.PCG64(2026))
+    out = []
+    for i in range(40):
+        lo = int(np.exp(rng.uniform(np.log(2), np.log(1 << 20))))
+        out.append(_smooth(rng, lo, 1 << 21))
+    for i in range(40):
+        out.append(int(np.exp(rng.uniform(np.log(2), np.log(1 << 20)))))
+    for i in range(20):
+        p = int(rng.choice([11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59, 61, 67, 71]))
+        out.append(p * _smooth(rng, 1, 1 << 14))
+    return out
+
+
+CASES = [(n, np.complex64 if i % 2 == 0 else np.complex128) for i, n in enumerate(_lengths())]
+
+
+@pytest.mark.parametrize("n,dt", CASES)
+def test_sweep_matches_oracle(gpu_ready, n, dt):
+    import catabrafft as C
+
+    rng = np.random.Generator(np.random.PCG64(n))
+    x = (rng.standard_normal((2, n)) + 1j * rng.standard_normal((2, n))).astype(dt)
+    p = C.Plan(n, 2, dt)
+    print(f"sweep n={n} {np.dtype(dt)}: {p.describe()}", flush=True)
+    y = run_dev(lambda xd: p.exec_torch(xd), x)
+    err = rel_l2(y, oracle_ref(x))
+    assert err <= tol(n, dt), (p.describe(), err)
+    yi = run_dev(lambda xd: p.exec_torch(xd, direction=1, normalize=True), x)
+    err = rel_l2(yi, oracle_ref(x, 1, True))
+    assert err <= tol(n, dt), (p.describe(), "backward", err)

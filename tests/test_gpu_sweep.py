@@ -57,3 +57,53 @@ def test_sweep_matches_oracle(gpu_ready, n, dt):
     yi = run_dev(lambda xd: p.exec_torch(xd, direction=1, normalize=True), x)
     err = rel_l2(yi, oracle_ref(x, 1, True))
     assert err <= tol(n, dt), (p.describe(), "backward", err)
+
+
+def _to_h(z):
+    return np.stack([z.real.astype(np.float16), z.imag.astype(np.float16)], axis=-1)
+
+
+def _from_h(h):
+    return h[..., 0].astype(np.float32) + 1j * h[..., 1].astype(np.float32)
+
+
+@pytest.mark.parametrize("n,dt", CASES[::4])
+def test_sweep_layouts_match_oracle(gpu_ready, n, dt):
+    """In place, padded row distances (only the n elements of each output row are written),
+    and ComplexF16 storage (relative L2 2e-3 against the oracle of the rounded input)."""
+    import torch
+
+    import catabrafft as C
+
+    rng = np.random.Generator(np.random.PCG64(n + 5))
+    x = (rng.standard_normal((2, n)) + 1j * rng.standard_normal((2, n))).astype(dt)
+    ref = oracle_ref(x)
+    s = torch.cuda.current_stream().cuda_stream
+    print(f"layouts n={n} {np.dtype(dt)}", flush=True)
+    # in place
+    p = C.Plan(n, 2, dt)
+    xd = torch.from_numpy(x).cuda()
+    p.exec_ptr(xd.data_ptr(), xd.data_ptr(), -1, False, s)
+    torch.cuda.synchronize()
+    assert rel_l2(xd.cpu().numpy(), ref) <= tol(n, dt), ("in place", p.describe())
+    # padded rows: idist = n + 3, odist = n + 5
+    q = C.Plan(n, 2, dt, idist=n + 3, odist=n + 5)
+    xp = np.zeros((2, n + 3), dtype=dt)
+    xp[:, :n] = x
+    xp[:, n:] = 9.0
+    yp = np.full((2, n + 5), 7.0 - 7.0j, dtype=dt)
+    xpd, ypd = torch.from_numpy(xp).cuda(), torch.from_numpy(yp).cuda()
+    q.exec_ptr(xpd.data_ptr(), ypd.data_ptr(), -1, False, s)
+    torch.cuda.synchronize()
+    yp = ypd.cpu().numpy()
+    assert rel_l2(yp[:, :n], ref) <= tol(n, dt), ("padded", q.describe())
+    assert np.all(yp[:, n:] == 7.0 - 7.0j), ("padding written", q.describe())
+    # ComplexF16 storage
+    xh = _to_h(x.astype(np.complex64))
+    h = C.Plan(n, 2, 2)
+    xhd = torch.from_numpy(xh).cuda()
+    yhd = torch.empty_like(xhd)
+    h.exec_ptr(xhd.data_ptr(), yhd.data_ptr(), -1, False, s)
+    torch.cuda.synchronize()
+    refh = oracle_ref(_from_h(xh).astype(np.complex128))
+    assert rel_l2(_from_h(yhd.cpu().numpy()), refh) < 2e-3, ("f16", h.describe())

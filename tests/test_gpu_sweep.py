@@ -107,3 +107,22 @@ def test_sweep_layouts_match_oracle(gpu_ready, n, dt):
     torch.cuda.synchronize()
     refh = oracle_ref(_from_h(xh).astype(np.complex128))
     assert rel_l2(_from_h(yhd.cpu().numpy()), refh) < 2e-3, ("f16", h.describe())
+
+
+@pytest.mark.parametrize("n,dt", [(n, dt) for n, dt in CASES[1::3] if n > 1])
+def test_sweep_real_matches_oracle(gpu_ready, n, dt):
+    """rfft against the oracle's complex transform of the real input; irfft and brfft / n back
+    to the input (even and odd lengths)."""
+    import catabrafft as C
+
+    rt = np.float32 if dt == np.complex64 else np.float64
+    rng = np.random.Generator(np.random.PCG64(n + 9))
+    x = rng.standard_normal((2, n)).astype(rt)
+    print(f"real n={n} {np.dtype(rt)}", flush=True)
+    X = C.rfft(x)
+    assert X.shape == (2, n // 2 + 1)
+    ref = oracle_ref(x.astype(np.complex128))[:, :n // 2 + 1]
+    t = tol(n, dt)
+    assert rel_l2(X, ref) <= t
+    assert rel_l2(C.irfft(X, n), x) <= 2 * t
+    assert rel_l2(C.brfft(X, n) / n, x) <= 2 * t

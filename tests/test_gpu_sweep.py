@@ -126,3 +126,40 @@ def test_sweep_real_matches_oracle(gpu_ready, n, dt):
     assert rel_l2(X, ref) <= t
     assert rel_l2(C.irfft(X, n), x) <= 2 * t
     assert rel_l2(C.brfft(X, n) / n, x) <= 2 * t
+
+
+def _nd_cases():
+    rng = np.random.Generator(np.random.PCG64(77))
+    out = []
+    while len(out) < 24:
+        rank = int(rng.integers(2, 4))
+        shape = []
+        for _ in range(rank):
+            kind = rng.integers(0, 3)
+            if kind == 0:
+                shape.append(_smooth(rng, int(rng.integers(2, 600)), 4096))
+            elif kind == 1:
+                shape.append(int(rng.integers(1, 700)))
+            else:
+                shape.append(int(np.exp(rng.uniform(np.log(2), np.log(40000)))))
+        if np.prod(shape) > (1 << 21):
+            continue
+        axes = tuple(a for a in range(rank) if rng.random() < 0.7) or (int(rng.integers(0, rank)),)
+        out.append((tuple(shape), axes, np.complex64 if len(out) % 2 == 0 else np.complex128))
+    return out
+
+
+@pytest.mark.parametrize("shape,axes,dt", _nd_cases())
+def test_sweep_nd_matches_oracle(gpu_ready, shape, axes, dt):
+    """fftn / ifftn over seeded shapes and axis sets against the oracle along each axis."""
+    import catabrafft as C
+    from test_gpu_nd import oracle_nd
+
+    rng = np.random.Generator(np.random.PCG64(int(np.prod(shape))))
+    x = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(dt)
+    total = int(np.prod([shape[a] for a in axes]))
+    t = 1e-6 if dt == np.complex64 else 4 * np.finfo(float).eps * max(1.0, np.log2(total))
+    print(f"nd shape={shape} axes={axes} {np.dtype(dt)}", flush=True)
+    y = C.fftn(x, axes)
+    assert rel_l2(y, oracle_nd(x, axes)) <= t
+    assert rel_l2(C.ifftn(y, axes), x) <= 2 * t

@@ -163,3 +163,32 @@ def test_sweep_nd_matches_oracle(gpu_ready, shape, axes, dt):
     y = C.fftn(x, axes)
     assert rel_l2(y, oracle_nd(x, axes)) <= t
     assert rel_l2(C.ifftn(y, axes), x) <= 2 * t
+
+
+@pytest.mark.parametrize("seed,dt", [(5, np.complex64), (6, np.complex128), (7, np.complex64)])
+def test_sweep_uneven_matches_oracle(gpu_ready, seed, dt):
+    """Seeded uneven batches (the cost-sorted work lists of every plan kind in one launch
+    sequence): each transform against the oracle, and bit-identical to its single plan."""
+    import torch
+
+    import catabrafft as C
+    from test_gpu_uneven_server import layout, mixed_lengths, oracle_rows
+
+    lengths = mixed_lengths(150, seed=seed)
+    offs, span = layout(lengths)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    x = (rng.standard_normal(span) + 1j * rng.standard_normal(span)).astype(dt)
+    p = C.UnevenPlan(lengths, offs, dt)
+    print(f"uneven seed={seed} {np.dtype(dt)}: {p.describe()}", flush=True)
+    xd = torch.from_numpy(x).cuda()
+    yd = torch.zeros_like(xd)
+    s = torch.cuda.current_stream().cuda_stream
+    p.exec_ptr(xd.data_ptr(), yd.data_ptr(), -1, False, s)
+    torch.cuda.synchronize()
+    y = yd.cpu().numpy()
+    ref = oracle_rows(x, lengths, offs, -1, False)
+    for n, o in zip(lengths, offs):
+        assert rel_l2(y[o:o + n], ref[o:o + n]) <= tol(n, dt), n
+    for n, o in list(zip(lengths, offs))[::11]:
+        alone = C.fft(xd[o:o + n].clone()).cpu().numpy()
+        assert np.array_equal(alone, y[o:o + n]), n

@@ -131,6 +131,16 @@ int wisdom_export(Ctx* ctx, const char* path) {
     return CFFT_OK;
 }
 
+// The knobs a wisdom line may carry: E of an instantiated kernel (0 = default, 8, 16), threads
+// per workgroup in 1..1024, tiles 0 (default) or a power of two up to 1024, a split n1 >= 0,
+// chunk >= -1, pipe in {-1, 0, 1} and large in {0, 1}.
+static bool wisdom_options_valid(int E, int tt, long long n1, int ca, int cb, long long chunk,
+                                 int pipe, int large) {
+    auto tile_ok = [](int c) { return c == 0 || (c > 0 && c <= 1024 && (c & (c - 1)) == 0); };
+    return (E == 0 || E == 8 || E == 16) && tt >= 1 && tt <= 1024 && n1 >= 0 && tile_ok(ca) &&
+           tile_ok(cb) && chunk >= -1 && pipe >= -1 && pipe <= 1 && (large == 0 || large == 1);
+}
+
 // Imported choices replace the context's cached plans of the same keys, so that later plan
 // requests follow them; plans already handed out stay valid (they are shared) and are released
 // after the lock.  A malformed line ends the import with an error; the lines before it stay
@@ -158,7 +168,8 @@ int wisdom_import(Ctx* ctx, const char* path) {
                                     &o.target_threads, &n1, &o.ca, &o.cb, &chunk, &pipe, &large);
         o.pipe = got >= 7 ? pipe : -1;
         o.large = got == 8 ? large : 0;
-        if (got < 6) {
+        if (got < 6 || !wisdom_options_valid(o.E, o.target_threads, n1, o.ca, o.cb, chunk,
+                                             o.pipe, o.large)) {
             malformed = true;
             break;
         }

@@ -203,7 +203,9 @@ int make_line(LinePlan& lp, int64_t N, Prec pr, int E, int max_threads) {
     std::vector<int> rad = line_radices(N);
     if (rad.empty() || (int)rad.size() > kMaxPasses)
         return fail(CFFT_ERR_UNSUPPORTED, "line length " + std::to_string(N) + " is not smooth");
-    if (E == 0) E = 16;
+    // The kernels are instantiated for E = 8 and E = 16 (a launch picks E <= 8 ? 8 : 16), so
+    // the thread count is computed for the instance that will run.
+    E = (E > 0 && E <= 8) ? 8 : 16;
     lp.E = E;
     lp.Tt = threads_for(rad, N, E);
     if (lp.Tt > max_threads)
@@ -1122,7 +1124,7 @@ static int half_exec_staged(const Plan& p, const void* in, void* out, int sgn, d
     const bool nd = p.kind == Kind::ND;
     const int64_t rows = nd ? 1 : p.batch, n = nd ? p.idist : p.n;
     const int64_t idist = p.idist, odist = p.odist;
-    std::lock_guard<std::mutex> g(p.scratch_mu);
+    std::lock_guard<std::mutex> g(p.hbuf_mu);
     const size_t ib = (size_t)rows * idist * 8, ob = (size_t)rows * odist * 8;
     int rc;
     if (p.hbuf_in.bytes < ib && (rc = p.hbuf_in.alloc(ib))) return rc;

@@ -76,7 +76,10 @@ struct Plan {
     }
     // ComplexF16 input and output (dtype CFFT_C16): the plan computes in float; direct and
     // four-step plans convert inside their kernels, other kinds through hbuf_in / hbuf_out
+    // (guarded by hbuf_mu, apart from scratch_mu: the float transform between the two
+    // conversions may itself take scratch_mu, e.g. a six-step plan)
     bool half = false;
+    mutable std::mutex hbuf_mu;
     mutable DevBuf hbuf_in, hbuf_out;
     // convolution (Rader / Bluestein)
     ConvMode mode = ConvMode::Rader;

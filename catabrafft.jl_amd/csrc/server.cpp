@@ -98,6 +98,9 @@ struct SrvRequest {
     double t_submit = 0, t_done = 0;
     size_t bytes = 0;
     int status = 1;  // 1 queued / running, 0 done, < 0 error
+    // an on-device request with in == out whose plan runs out of place (four-step): the result
+    // goes to the pooled dout and is copied into `out` on the device
+    bool via_dout = false;
     PoolBuf din, dout, work;
     std::string err;
 };
@@ -225,6 +228,16 @@ static int srv_launch_round(Server* s, std::vector<SrvRequest*>& admitted, int s
         const size_t bytes = (size_t)r->batch * r->n * es;
         const void* din = r->in;
         void* dout = r->out;
+        if (r->on_device && r->in == r->out && p->kind == Kind::FourStep) {
+            r->dout.p = s->pool.get(bytes, &r->dout.cls);
+            if (!r->dout.p) {
+                r->status = CFFT_ERR_NOMEM;
+                r->err = "device allocation failed";
+                continue;
+            }
+            r->via_dout = true;
+            dout = r->dout.p;
+        }
         if (!r->on_device) {
             r->din.p = s->pool.get(bytes, &r->din.cls);
             r->dout.p = s->pool.get(bytes, &r->dout.cls);
@@ -282,11 +295,16 @@ static int srv_launch_round(Server* s, std::vector<SrvRequest*>& admitted, int s
         }
         R->reqs.push_back(r);
     }
-    for (SrvRequest* r : admitted)  // requests that could not be staged leave now
-        if (r->status < 0) {
-            srv_release(s, r);
-            s->cv_done.notify_all();
-        }
+    // Requests that could not be staged leave now.  Copies of theirs may already be queued on
+    // the slot's stream: they finish before the buffers return to the pool.
+    bool staged_fail = false;
+    for (SrvRequest* r : admitted) staged_fail = staged_fail || r->status < 0;
+    if (staged_fail) {
+        (void)hipStreamSynchronize(st);
+        for (SrvRequest* r : admitted)
+            if (r->status < 0) srv_release(s, r);
+        s->cv_done.notify_all();
+    }
     SrvSlot& sl = s->slots[slot];
     sl.busy = true;  // claimed while the round is built without the lock
     lk.unlock();
@@ -297,14 +315,26 @@ static int srv_launch_round(Server* s, std::vector<SrvRequest*>& admitted, int s
     for (size_t i = 0; rc == CFFT_OK && i < R->launches.size(); ++i)
         if (launch_multi(R->launches[i], ma, st) != hipSuccess)
             rc = fail(CFFT_ERR_HIP, "work-list launch failed");
+    // A plan-launched request that fails is recorded on that request alone; the round goes on.
     for (size_t i = 0; rc == CFFT_OK && i < direct.size(); ++i) {
         const Direct& d = direct[i];
-        rc = plan_exec_shaped(*d.p, d.in, d.out, d.r->dir, d.r->norm, st, d.sh);
+        const int drc = plan_exec_shaped(*d.p, d.in, d.out, d.r->dir, d.r->norm, st, d.sh);
+        if (drc != CFFT_OK) {
+            d.r->status = drc;
+            d.r->err = cfft_last_error();
+        }
     }
-    for (SrvRequest* r : R->reqs)
-        if (rc == CFFT_OK && !r->on_device && r->status > 0)
-            (void)hipMemcpyAsync(r->out, r->dout.p, (size_t)r->batch * r->n * esz_of(r->dtype),
-                                 hipMemcpyDeviceToHost, st);
+    for (SrvRequest* r : R->reqs) {
+        if (rc != CFFT_OK || r->status <= 0) continue;
+        const size_t bytes = (size_t)r->batch * r->n * esz_of(r->dtype);
+        if (!r->on_device)
+            (void)hipMemcpyAsync(r->out, r->dout.p, bytes, hipMemcpyDeviceToHost, st);
+        else if (r->via_dout && hipMemcpyAsync(r->out, r->dout.p, bytes, hipMemcpyDeviceToDevice,
+                                               st) != hipSuccess) {
+            r->status = CFFT_ERR_HIP;
+            r->err = "device to device copy failed";
+        }
+    }
     if (rc == CFFT_OK && hipEventRecord(sl.done, st) != hipSuccess)
         rc = fail(CFFT_ERR_HIP, "event record failed");
     lk.lock();
@@ -328,8 +358,9 @@ static void srv_retire(Server* s) {
         const double t = now_s();
         for (SrvRequest* r : R->reqs) {
             srv_release(s, r);
-            if (r->status > 0) r->status = 0;
             r->t_done = t;
+            if (r->status < 0) continue;  // failed on its own: no latency or throughput sample
+            r->status = 0;
             s->lat.push_back(t - r->t_submit);
             s->last_done = t;
             s->transforms += r->batch;

@@ -158,7 +158,9 @@ CFFT_API int cfft_server_create(cfft_ctx ctx, int64_t hbm_cap_bytes, int nstream
  * buffers (the server stages them through device memory it admits under the cap);
  * on_device = 1: they are device buffers already resident in HBM (their bytes still count
  * toward the cap while the request is in flight).  Both stay owned by the caller and must live
- * until the ticket completes. */
+ * until the ticket completes.  in == out is allowed (device lengths that run as a four-step go
+ * through a pooled output buffer and a device copy).  A request that fails reports its own
+ * status; the other requests of its round are unaffected. */
 CFFT_API int cfft_server_submit(cfft_server srv, int64_t n, int64_t batch, int dtype,
                                 int direction, int normalize, const void* in, void* out,
                                 int on_device, int64_t* ticket);

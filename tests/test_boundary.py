@@ -79,14 +79,13 @@ def test_flags_match_the_reference():
     assert "#define CFFT_NO_FLAG 0" in hdr and "#define CFFT_ENCHANT 2" in hdr
 
 
-def test_generated_codelets_are_current():
+def test_generated_codelets_are_current(tmp_path):
     sys.path.insert(0, os.path.join(PKG, "gen"))
     import gen_codelets as G
 
     G.self_check()
     G.check_mul_diag()
-    tmp = os.path.join(ROOT, "_work", "codelets_check.h")
-    os.makedirs(os.path.dirname(tmp), exist_ok=True)
+    tmp = str(tmp_path / "codelets_check.h")
     subprocess.run([sys.executable, os.path.join(PKG, "gen", "gen_codelets.py"), tmp], check=True,
                    capture_output=True)
     assert open(tmp).read() == open(os.path.join(PKG, "csrc", "codelets.h")).read()

@@ -157,6 +157,29 @@ def test_emulated_server_rounds(E):
     assert lib.cfft_server_destroy(srv) == 0
 
 
+def test_emulated_server_in_place_requests(E):
+    """On-device requests with in == out: a four-step length (its plan runs out of place, so the
+    server stages its result through a pooled buffer) and a work-list length, in one round with
+    an ordinary request; each result matches the oracle."""
+    lib, ctx = E
+    srv = ctypes.c_void_p()
+    assert lib.cfft_server_create(ctx, 1 << 30, 1, ctypes.byref(srv)) == 0
+    cases = [(1 << 16, 2, True), (1024, 3, False), (2520, 2, True)]
+    reqs = []
+    for i, (n, b, inplace) in enumerate(cases):
+        x = seeded_complex(n, batch=b, seed=40 + i, dtype=np.complex64)
+        buf = x.copy()
+        y = buf if inplace else np.zeros_like(x)
+        t = ctypes.c_int64()
+        assert lib.cfft_server_submit(srv, n, b, 0, -1, 0, buf.ctypes.data, y.ctypes.data, 1,
+                                      ctypes.byref(t)) == 0
+        reqs.append((t.value, x, y, buf))
+    for t, x, y, _ in reqs:
+        assert lib.cfft_server_wait(srv, t) == 0, lib.cfft_last_error()
+        assert rel_l2(y, O.fft(x.astype(np.complex128), "d")) < 1e-6
+    assert lib.cfft_server_destroy(srv) == 0
+
+
 def test_emulated_measure_flags(E):
     lib, ctx = E
     x = seeded_complex(1024, batch=2, seed=2, dtype=np.complex64)
@@ -218,6 +241,35 @@ def test_emulated_wisdom_round_trip(E, tmp_path):
     for h in (p, q, r):
         lib.cfft_plan_destroy(h)
     lib.cfft_finalize(ctx2)
+
+
+@pytest.mark.parametrize("line", [
+    "4096 2 4096 4096 0 1 : 32 64 0 0 0 -1 0 0",    # E the kernels are not built for
+    "4096 2 4096 4096 0 1 : 16 0 0 0 0 -1 0 0",     # no threads
+    "4096 2 4096 4096 0 1 : 16 64 0 3 4 -1 0 0",    # tile not a power of two
+    "4096 2 4096 4096 0 1 : 16 64 -5 0 0 -1 0 0",   # negative split
+    "4096 2 4096 4096 0 1 : 16 64 0 0 0 -2 0 0",    # chunk below -1
+    "4096 2 4096 4096 0 1 : 16 64 0 0 0 -1 3 0",    # pipe out of range
+    "4096 2 4096 4096 0 1 : 16 64 0 0 0 -1 0 2"])   # large out of range
+def test_emulated_wisdom_rejects_out_of_range_knobs(E, tmp_path, line):
+    """A wisdom line whose knobs no kernel instance implements is refused (the import fails and
+    the key keeps its tuned or default plan), instead of building a plan that skips butterflies."""
+    lib, _ = E
+    path = tmp_path / "bad.txt"
+    path.write_text(line + "\n")
+    c2 = ctypes.c_void_p()
+    assert lib.cfft_init(0, ctypes.byref(c2)) == 0
+    assert lib.cfft_wisdom_import(c2, str(path).encode()) == -1
+    assert b"malformed" in lib.cfft_last_error()
+    path.write_text("4096 2 4096 4096 0 1 : 16 64 0 0 0 -1 0 0\n")
+    assert lib.cfft_wisdom_import(c2, str(path).encode()) == 0
+    p = _plan(lib, c2, 4096, 2, np.complex64, flags=1)
+    x = seeded_complex(4096, batch=2, seed=9, dtype=np.complex64)
+    y = np.zeros_like(x)
+    assert lib.cfft_exec(p, x.ctypes.data, y.ctypes.data, -1, 0, None) == 0
+    assert rel_l2(y, O.fft(x.astype(np.complex128), "d")) < 1e-6
+    lib.cfft_plan_destroy(p)
+    lib.cfft_finalize(c2)
 
 
 @pytest.mark.parametrize("n,batch,dt", [(2520, 11, np.complex64), (2520, 9, np.complex128),

@@ -340,3 +340,38 @@ def test_emulated_f16_strided_rows_and_in_place(E, n, batch, kind):
     assert np.array_equal(zh[:, :n].view(np.uint16), to_h(z32[:, :n]).view(np.uint16))
     for p in (p16, p32, q16, q32):
         lib.cfft_plan_destroy(p)
+
+
+@pytest.mark.timeout(120)
+def test_emulated_f16_sixstep_through_wisdom(E, tmp_path):
+    """A ComplexF16 plan forced onto the six-step path by wisdom: its staged float transform
+    takes the plan's scratch while the half-precision staging buffers are held.  The result is
+    bit-equal to the C32 six-step result rounded to float16 (and it returns at all)."""
+    lib, ctx = E
+    n, batch = 1 << 14, 2
+    path = tmp_path / "w.txt"
+    path.write_text(f"{n} {batch} {n} {n} {C16} 1 : 0 64 64 0 0 -1 -1 1\n"
+                    f"{n} {batch} {n} {n} {C32} 1 : 0 64 64 0 0 -1 -1 1\n")
+    c2 = ctypes.c_void_p()
+    assert lib.cfft_init(0, ctypes.byref(c2)) == 0
+    assert lib.cfft_wisdom_import(c2, str(path).encode()) == 0
+    ps = {}
+    for code in (C16, C32):
+        p = ctypes.c_void_p()
+        assert lib.cfft_plan_1d(c2, n, batch, n, n, code, 1, ctypes.byref(p)) == 0, \
+            lib.cfft_last_error()
+        assert "sixstep" in _desc(lib, p), _desc(lib, p)
+        ps[code] = p
+    xh = to_h(seeded_complex(n, batch=batch, seed=61))
+    x32 = from_h(xh).astype(np.complex64)
+    for direction, norm in ((-1, 0), (1, 1)):
+        y16 = np.zeros_like(xh)
+        y32 = np.zeros_like(x32)
+        assert lib.cfft_exec(ps[C16], xh.ctypes.data, y16.ctypes.data, direction, norm, None) == 0
+        assert lib.cfft_exec(ps[C32], x32.ctypes.data, y32.ctypes.data, direction, norm, None) == 0
+        assert np.array_equal(y16.view(np.uint16), to_h(y32).view(np.uint16)), (direction, norm)
+        ref = O.fft(x32.astype(np.complex128), "d", direction, bool(norm))
+        assert rel_l2(from_h(y16), ref) < 2e-3
+    for p in ps.values():
+        lib.cfft_plan_destroy(p)
+    lib.cfft_finalize(c2)

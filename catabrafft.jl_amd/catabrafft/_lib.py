@@ -67,6 +67,7 @@ def _sig(L):
     _set(L, "cfft_plan_1d_real", [vp, i64, i64, i64, i64, c_int, c_int, P(vp)])
     _set(L, "cfft_plan_many_uneven", [vp, i64, P(i64), P(i64), c_int, c_int, P(vp)])
     _set(L, "cfft_exec", [vp, vp, vp, c_int, c_int, vp])
+    _set(L, "cfft_exec_noalloc", [vp, vp, vp, c_int, c_int, vp])
     _set(L, "cfft_exec_host", [vp, vp, vp, c_int, c_int])
     _set(L, "cfft_plan_device_bytes", [vp, P(i64)])
     _set(L, "cfft_plan_describe", [vp, ctypes.c_char_p, i64])

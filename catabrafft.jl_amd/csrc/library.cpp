@@ -48,6 +48,53 @@ int fail(int code, const std::string& msg) {
             return fail(CFFT_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+// ------------------------------------------------------------------------------- no-alloc work area
+// The work area of cfft_exec_noalloc: a file-scope device array of kNoAllocScratchBytes, one copy
+// per GPU (each at its own device address), held until the process ends.  Its host-side name is
+// not a device address, so host code never passes `scratch` to a kernel or a copy: ctx_create
+// looks up the current device's address once (hipGetSymbolAddress) and every use goes through
+// that pointer.  Calls of cfft_exec_noalloc share their GPU's array.
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+bool g_scratch_looked_up[kMaxDevices] = {};
+void* g_scratch_ptr[kMaxDevices] = {};
+std::string g_scratch_err[kMaxDevices];
+}  // namespace
+
+// Looks up the work area's address on `device` (the current device) once per device.
+static void noalloc_scratch_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) return;
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (g_scratch_looked_up[device]) return;
+    g_scratch_looked_up[device] = true;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        (void)hipGetLastError();
+        g_scratch_err[device] = std::string("hipGetSymbolAddress(scratch) failed: ") +
+                                hipGetErrorString(e);
+        return;
+    }
+    g_scratch_ptr[device] = ptr;
+}
+
+// The work area's device address on `device` (nullptr, with the reason in *err, if unavailable).
+static void* noalloc_scratch_ptr(int device, std::string* err) {
+    if (device < 0 || device >= kMaxDevices) {
+        *err = "device index beyond the work-area table";
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_ptr[device])
+        *err = g_scratch_looked_up[device] ? g_scratch_err[device]
+                                            : std::string("no context was opened on this device");
+    return g_scratch_ptr[device];
+}
+
 // ------------------------------------------------------------------------------- device buffers
 DevBuf::~DevBuf() { reset(); }
 void DevBuf::reset() {
@@ -1179,6 +1226,37 @@ static int plan_exec_body(const Plan& p, const void* in, void* out, int directio
     return run_fft(p, in, out, sgn, scale, ConvEpilogue{nullptr, nullptr}, st);
 }
 
+// Execution without any allocation (cfft_exec_noalloc): plan kinds whose kernels need no work
+// buffer run as plan_exec runs them; a four-step whose middle needs a work buffer (in place, or
+// ComplexF16 data) takes it from the static work area instead of the plan's lazily allocated
+// scratch.  Other kinds hold work buffers of their own and are refused.  No event or buffer is
+// created here, and executions on one GPU share its work area (overlapping calls race).
+static int plan_exec_noalloc(const Plan& p, const void* in, void* out, int direction,
+                             int normalize, hipStream_t st) {
+    if (p.kind == Kind::Uneven || p.real || !(p.kind == Kind::Trivial || p.kind == Kind::Direct ||
+                                              p.kind == Kind::FourStep ||
+                                              (p.kind == Kind::ConvSmall && !p.half)))
+        return fail(CFFT_ERR_UNSUPPORTED,
+                    "cfft_exec_noalloc: this plan kind owns work buffers (only trivial, direct, "
+                    "four-step and fused-convolution 1-D plans run without allocating)");
+    if (p.batch == 0) return CFFT_OK;
+    if (!in || !out) return fail(CFFT_ERR_INVALID, "null buffer");
+    const int sgn = direction > 0 ? -1 : 1;
+    const double scale = normalize ? 1.0 / (double)p.n : 1.0;
+    const ConvEpilogue none{nullptr, nullptr};
+    if (p.kind != Kind::FourStep || (in != out && !p.half))
+        return run_fft(p, in, out, sgn, scale, none, st);
+    const size_t need = (size_t)p.batch * (size_t)p.n * (size_t)esize(p.prec);
+    if (need > kNoAllocScratchBytes)
+        return fail(CFFT_ERR_UNSUPPORTED,
+                    "cfft_exec_noalloc: the four-step middle needs " + std::to_string(need) +
+                        " bytes; the work area holds " + std::to_string(kNoAllocScratchBytes));
+    std::string err;
+    void* work = noalloc_scratch_ptr(p.ctx->device, &err);
+    if (!work) return fail(CFFT_ERR_HIP, "cfft_exec_noalloc: " + err);
+    return run_fourstep(p, in, work, out, p.idist, p.n, p.odist, sgn, scale, none, st, p.batch);
+}
+
 // ------------------------------------------------------------------------------- N-D
 // Multi-dimensional transforms over the axes in axes_mask (bit a = axis a, row-major dims, the
 // last axis contiguous): the role of plan_fft's `region` (CatabraFFT.jl:169, accepted there and
@@ -1491,6 +1569,7 @@ int ctx_create(int device, Ctx** out) {
     c->num_cu = v;
     CFFT_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     init_kernels();
+    noalloc_scratch_setup(device);
     *out = c.release();
     return CFFT_OK;
 }
@@ -1598,6 +1677,15 @@ int cfft_exec(cfft_plan plan, const void* in, void* out, int direction, int norm
         return fail(CFFT_ERR_INVALID, "direction must be -1 or +1");
     CFFT_HIP(hipSetDevice(plan->p->ctx->device));
     return plan_exec(*plan->p, in, out, direction, normalize, (hipStream_t)stream);
+}
+
+int cfft_exec_noalloc(cfft_plan plan, const void* in, void* out, int direction, int normalize,
+                      void* stream) {
+    if (!plan) return fail(CFFT_ERR_INVALID, "null plan");
+    if (direction != CFFT_FORWARD && direction != CFFT_BACKWARD)
+        return fail(CFFT_ERR_INVALID, "direction must be -1 or +1");
+    CFFT_HIP(hipSetDevice(plan->p->ctx->device));
+    return plan_exec_noalloc(*plan->p, in, out, direction, normalize, (hipStream_t)stream);
 }
 
 int cfft_exec_host(cfft_plan plan, const void* in, void* out, int direction, int normalize) {

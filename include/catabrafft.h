@@ -119,6 +119,18 @@ CFFT_API int cfft_plan_many_uneven(cfft_ctx ctx, int64_t count, const int64_t* l
 CFFT_API int cfft_exec(cfft_plan plan, const void* in, void* out, int direction, int normalize,
               void* stream);
 
+/* cfft_exec without any allocation.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  A four-step execution whose middle needs a work buffer (in == out, or
+ * ComplexF16 data) takes it from a static device work area of 64 MiB per GPU (batch * n * 8
+ * bytes in C32 and C16, * 16 in C64, must fit; larger is CFFT_ERR_UNSUPPORTED); direct,
+ * fused-convolution and trivial plans need none.  Plan kinds that own work buffers (six-step,
+ * large convolutions, N-D, real, uneven) are refused with CFFT_ERR_UNSUPPORTED.  Every call on a
+ * GPU shares that GPU's work area: calls that overlap on one GPU (other streams, other threads)
+ * race and must be ordered by the caller.  Results are bitwise those of cfft_exec. */
+CFFT_API int cfft_exec_noalloc(cfft_plan plan, const void* in, void* out, int direction,
+                               int normalize, void* stream);
+
 /* Executes on host pointers: stages through device buffers owned by the context and returns
  * when the result is in `out`.  This is what CatabraFFT.fft / ifft / bfft call. */
 CFFT_API int cfft_exec_host(cfft_plan plan, const void* in, void* out, int direction, int normalize);

@@ -507,3 +507,42 @@ def test_emulated_nd_transposed_axes(E, shape, axes, dt):
     assert lib.cfft_exec(p, z.ctypes.data, z.ctypes.data, -1, 0, None) == 0
     assert rel_l2(z, _oracle_nd(x, [a % rank for a in axes], -1, False)) < _tol(max(n_all, 2), dt)
     lib.cfft_plan_destroy(p)
+
+
+def test_emulated_exec_noalloc_matches_exec(E):
+    """cfft_exec_noalloc on the CPU build: in-place four-step plans take their middle from the
+    static work area (two calls in a row, the second smaller), other plans run as cfft_exec;
+    bitwise equal to cfft_exec.  Plans that own work buffers and middles beyond the work area are
+    refused."""
+    lib, ctx = E
+    for n, batch, dt, inplace in ((1 << 16, 2, np.complex64, True),
+                                  (1 << 16, 1, np.complex64, True),
+                                  (1 << 17, 1, np.complex128, True),
+                                  (2520, 3, np.complex64, False),
+                                  (17, 5, np.complex128, False)):
+        p = _plan(lib, ctx, n, batch, dt)
+        x = seeded_complex(n, batch=batch, seed=n + batch, dtype=dt)
+        for direction, norm in ((-1, 0), (1, 1)):
+            ref = x.copy()
+            got = x.copy()
+            if inplace:
+                assert lib.cfft_exec(p, ref.ctypes.data, ref.ctypes.data, direction, norm,
+                                     None) == 0
+                assert lib.cfft_exec_noalloc(p, got.ctypes.data, got.ctypes.data, direction, norm,
+                                             None) == 0, lib.cfft_last_error()
+            else:
+                assert lib.cfft_exec(p, x.ctypes.data, ref.ctypes.data, direction, norm, None) == 0
+                assert lib.cfft_exec_noalloc(p, x.ctypes.data, got.ctypes.data, direction, norm,
+                                             None) == 0, lib.cfft_last_error()
+            assert np.array_equal(got, ref), (n, direction)
+            want = O.fft(x.astype(np.complex128), "d", direction, bool(norm))
+            assert rel_l2(got, want) < _tol(n, dt)
+        lib.cfft_plan_destroy(p)
+    big = _plan(lib, ctx, 1 << 20, 9, np.complex64)  # 72 MiB middle
+    assert lib.cfft_exec_noalloc(big, 16, 16, -1, 0, None) == -4
+    assert b"work area" in lib.cfft_last_error()
+    rl = _plan(lib, ctx, 65537, 1, np.complex64)
+    assert "rader-large" in _desc(lib, rl)
+    assert lib.cfft_exec_noalloc(rl, 16, 32, -1, 0, None) == -4
+    for h in (big, rl):
+        lib.cfft_plan_destroy(h)

@@ -77,6 +77,15 @@ def build(force: bool = False, jobs: int = 8) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # the C one-transform-per-launch caller the serving benchmark compares against
+    loop = os.path.join(LIBDIR, "libcfft_one_per_launch.so")
+    cmd = ["gcc", "-O2", "-shared", "-fPIC", "-I", os.path.join(ROOT, "include"),
+           os.path.join(HERE, "bench", "one_per_launch.c"), "-L", LIBDIR, "-lcatabrafft",
+           "-Wl,-rpath,$ORIGIN", "-o", loop + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"build failed: {' '.join(cmd)}\n{r.stderr}")
+    os.replace(loop + ".tmp", loop)
     return LIB
 
 

@@ -2,9 +2,10 @@
 //
 // CatabraFFT.jl's MEASURE enumerates radix decompositions and times each candidate function once
 // with @elapsed (radix_plan.jl:23-68, radix_exec.jl:270-297); ENCHANT additionally A/B-tests
-// @simd ivdep variants (radix_exec.jl:140-259).  Here the candidates are kernel configurations
-// of the same transform: per-thread capacity E, threads per workgroup, the four-step split and
-// its column tiles.  Each candidate is built, checked to run, and timed with HIP events as the
+// @simd ivdep variants (radix_exec.jl:140-259).  Here the candidates are the radix sequence of
+// each line (orders and splits of its factors, radix_variants) and the kernel configurations of
+// the same transform: per-thread capacity E, threads per workgroup, the four-step split and its
+// column tiles.  Each candidate is built, checked to run, and timed with HIP events as the
 // median of several batches of launches on the real batch size; the fastest wins.  The choice is
 // a pure function of (n, batch, dtype, layout) and of the measured times, never of the data.
 #include <hip/hip_runtime.h>
@@ -13,6 +14,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <cmath>
 #include <memory>
@@ -24,6 +26,84 @@
 #include "library.h"
 
 namespace cfft {
+
+// Radix sequences of one line of length N that MEASURE / ENCHANT time, the default first (the
+// role of create_all_radix_plans, radix_plan.jl:23-68, which enumerates the non-increasing
+// decompositions of 2^k into 16/8/4/2 and times each).  The power-of-two part is split into
+// radices 16/8/4/2 in every way with at most one more pass than the fewest, the power-of-three part
+// into 9s and a 3 or into 3s; each multiset is tried largest radix first and smallest first
+// (ENCHANT also tries moving the first radix to the end).  Sequences with more passes than the
+// engine holds are dropped; at most `cap` are returned.
+static std::vector<std::vector<int>> radix_variants(int64_t N, bool enchant, size_t cap) {
+    std::vector<std::vector<int>> out;
+    const std::vector<int> def = line_radices(N);
+    if (def.empty()) return out;
+    out.push_back(def);
+    int cnt[14] = {0};
+    int64_t m = N;
+    for (int q : {2, 3, 5, 7, 11, 13})
+        while (m % q == 0) {
+            cnt[q]++;
+            m /= q;
+        }
+    // power-of-two multisets: parts in {4, 3, 2, 1} (exponents of 16, 8, 4, 2), at most one 1
+    std::vector<std::vector<int>> two;
+    const int a = cnt[2];
+    const int min_parts = (a + 3) / 4;
+    std::vector<int> part;
+    std::function<void(int, int)> rec = [&](int left, int maxp) {
+        if (left == 0) {
+            if ((int)part.size() <= min_parts + 1) {
+                std::vector<int> r;
+                for (int e : part) r.push_back(1 << e);
+                two.push_back(r);
+            }
+            return;
+        }
+        if ((int)part.size() > min_parts) return;
+        for (int e = std::min(maxp, left); e >= 1; --e) {
+            if (e == 1 && !part.empty() && part.back() == 1) continue;
+            part.push_back(e);
+            rec(left - e, e);
+            part.pop_back();
+        }
+    };
+    rec(a, 4);
+    if (two.empty()) two.push_back({});
+    std::vector<std::vector<int>> three{{}};
+    if (cnt[3] > 0) {
+        three.clear();
+        std::vector<int> t9;
+        for (int i = 0; i < cnt[3] / 2; ++i) t9.push_back(9);
+        if (cnt[3] % 2) t9.push_back(3);
+        three.push_back(t9);
+        if (cnt[3] > 1) three.push_back(std::vector<int>(cnt[3], 3));
+    }
+    std::vector<int> rest;
+    for (int q : {13, 11, 7, 5})
+        for (int i = 0; i < cnt[q]; ++i) rest.push_back(q);
+    auto add = [&](std::vector<int> r) {
+        if (r.empty() || (int)r.size() > kMaxPasses) return;
+        if (std::find(out.begin(), out.end(), r) == out.end()) out.push_back(r);
+    };
+    for (const auto& t2 : two)
+        for (const auto& t3 : three) {
+            std::vector<int> r = t2;
+            r.insert(r.end(), t3.begin(), t3.end());
+            r.insert(r.end(), rest.begin(), rest.end());
+            std::sort(r.begin(), r.end(), std::greater<int>());
+            add(r);
+            std::vector<int> up(r.rbegin(), r.rend());
+            add(up);
+            if (enchant && r.size() > 1) {
+                std::vector<int> rot(r.begin() + 1, r.end());
+                rot.push_back(r.front());
+                add(rot);
+            }
+        }
+    if (out.size() > cap) out.resize(cap);
+    return out;
+}
 
 static std::vector<PlanOptions> candidates(Ctx* ctx, const Plan& p, int flags) {
     std::vector<PlanOptions> out;
@@ -47,6 +127,25 @@ static std::vector<PlanOptions> candidates(Ctx* ctx, const Plan& p, int flags) {
                     o.pipe = pipe;
                     out.push_back(o);
                 }
+        // the line's radix sequence (direct lines that fit a workgroup), with the last pass
+        // through LDS or across wavefront lanes (candidates whose line does not allow the
+        // latter are dropped when built)
+        if (line_fits_lds(ctx, p.n, p.prec, 16)) {
+            const auto rads = radix_variants(p.n, enchant, enchant ? 12 : 6);
+            for (size_t i = 0; i < rads.size(); ++i)
+                for (int E : Es)
+                    for (int tt : {64, 256})
+                        for (int wave : {0, 1}) {
+                            if (i == 0 && wave == 0) continue;  // the shapes above
+                            PlanOptions o = base;
+                            o.E = E;
+                            o.target_threads = tt;
+                            o.pipe = 0;
+                            o.wave = wave;
+                            if (i > 0) o.rad_a = rads[i];
+                            out.push_back(o);
+                        }
+        }
         // four-step splits near sqrt(n) with a few tile widths
         std::vector<int64_t> splits;
         for (int64_t a = 2; a < p.n; ++a) {
@@ -82,14 +181,35 @@ static std::vector<PlanOptions> candidates(Ctx* ctx, const Plan& p, int flags) {
                                 o.pipe = pipe;
                                 out.push_back(o);
                             }
+        // the radix sequences of the two four-step lines, at the default tiles
+        for (int64_t a : splits) {
+            const auto ra = radix_variants(a, enchant, enchant ? 6 : 4);
+            const auto rb = radix_variants(p.n / a, enchant, enchant ? 6 : 4);
+            for (size_t i = 0; i < ra.size(); ++i)
+                for (size_t j = 0; j < rb.size(); ++j) {
+                    if (i == 0 && j == 0) continue;  // the default pair is already a candidate
+                    for (int E : Es) {
+                        PlanOptions o = base;
+                        o.E = E;
+                        o.n1 = a;
+                        o.chunk = 0;
+                        o.pipe = 0;
+                        o.rad_a = ra[i];
+                        o.rad_b = rb[j];
+                        out.push_back(o);
+                    }
+                }
+        }
     } else {
         for (int E : Es)
-            for (int tt : tts) {
-                PlanOptions o = base;
-                o.E = E;
-                o.target_threads = tt;
-                out.push_back(o);
-            }
+            for (int tt : tts)
+                for (int wave : {0, 1}) {
+                    PlanOptions o = base;
+                    o.E = E;
+                    o.target_threads = tt;
+                    o.wave = wave;
+                    out.push_back(o);
+                }
     }
     (void)ctx;
     return out;
@@ -109,11 +229,50 @@ static void copy_shape(const Plan& from, Plan& to) {
 // ------------------------------------------------------------------------------- wisdom
 // Tuned choices keyed by (n, batch, idist, odist, dtype, flags); kept per context and written /
 // read as text lines "n batch idist odist dtype flags : E target_threads n1 ca cb chunk pipe
-// large" (files without the last fields read pipe = default, large = 0).
+// large [ra=R,R,...] [rb=R,R,...] [wv=0|1]" (files without the last fields read pipe = default,
+// large = 0; ra / rb are the radix sequences of the direct or first line and of the second line,
+// absent for the default sequences; wv the wave tail of a direct or fused-convolution line; fc
+// the fused launches of a large convolution).
 static std::string wisdom_key(const Plan& p, int flags) {
     std::ostringstream k;
     k << p.n << " " << p.batch << " " << p.idist << " " << p.odist << " " << p.dtype << " " << flags;
     return k.str();
+}
+
+// " ra=16,16,4" (empty when the sequence is the default).
+static std::string radix_field(const char* tag, const std::vector<int>& r) {
+    if (r.empty()) return "";
+    std::string s = tag;
+    for (size_t i = 0; i < r.size(); ++i) s += (i ? "," : "") + std::to_string(r[i]);
+    return s;
+}
+
+// Parses "ra=..." / "rb=..." / "wv=..." tokens after the numeric fields; false on a malformed one.
+static bool parse_radix_fields(const char* text, PlanOptions& o) {
+    if (const char* wv = std::strstr(text, "wv=")) {
+        if (wv[3] != '0' && wv[3] != '1') return false;
+        o.wave = wv[3] - '0';
+    }
+    if (const char* fc = std::strstr(text, "fc=")) {
+        if (fc[3] != '0' && fc[3] != '1') return false;
+        o.fuse_conv = fc[3] - '0';
+    }
+    for (const char* tag : {"ra=", "rb="}) {
+        const char* at = std::strstr(text, tag);
+        if (!at) continue;
+        std::vector<int>& r = tag[1] == 'a' ? o.rad_a : o.rad_b;
+        const char* c = at + 3;
+        while (true) {
+            char* end = nullptr;
+            const long v = std::strtol(c, &end, 10);
+            if (end == c || v < 2 || v > 16) return false;
+            r.push_back((int)v);
+            if ((int)r.size() > kMaxPasses) return false;
+            if (*end != ',') break;
+            c = end + 1;
+        }
+    }
+    return true;
 }
 
 int wisdom_export(Ctx* ctx, const char* path) {
@@ -123,9 +282,12 @@ int wisdom_export(Ctx* ctx, const char* path) {
     std::fprintf(f, "# catabrafft wisdom v1 (gfx950)\n");
     for (const auto& kv : ctx->wisdom) {
         const PlanOptions& o = kv.second;
-        std::fprintf(f, "%s : %d %d %lld %d %d %lld %d %d\n", kv.first.c_str(), o.E,
+        std::fprintf(f, "%s : %d %d %lld %d %d %lld %d %d%s%s%s%s\n", kv.first.c_str(), o.E,
                      o.target_threads, (long long)o.n1, o.ca, o.cb, (long long)o.chunk, o.pipe,
-                     o.large);
+                     o.large, radix_field(" ra=", o.rad_a).c_str(),
+                     radix_field(" rb=", o.rad_b).c_str(),
+                     (o.wave >= 0 ? " wv=" + std::to_string(o.wave) : std::string()).c_str(),
+                     (o.fuse_conv >= 0 ? " fc=" + std::to_string(o.fuse_conv) : std::string()).c_str());
     }
     std::fclose(f);
     return CFFT_OK;
@@ -169,7 +331,8 @@ int wisdom_import(Ctx* ctx, const char* path) {
         o.pipe = got >= 7 ? pipe : -1;
         o.large = got == 8 ? large : 0;
         if (got < 6 || !wisdom_options_valid(o.E, o.target_threads, n1, o.ca, o.cb, chunk,
-                                             o.pipe, o.large)) {
+                                             o.pipe, o.large) ||
+            !parse_radix_fields(colon + 1, o)) {
             malformed = true;
             break;
         }
@@ -235,6 +398,7 @@ int autotune(Ctx* ctx, Plan& p, int flags, const PlanOptions& base) {
         copy_shape(p, t);
         t.batch = tb;
         if (plan_build(ctx, t, o) != CFFT_OK) continue;
+        if (o.wave > 0 && !t.line.pt.wave) continue;  // the line does not allow a wave tail
         bool ok = true;
         for (int w = 0; w < 2 && ok; ++w)
             ok = plan_exec(t, in.p, out.p, CFFT_FORWARD, 0, ctx->stream) == CFFT_OK;

@@ -40,6 +40,9 @@ constexpr int kMaxPasses = 20;
 // tw[twoff[i] + p (R-1) + (k-1)] = w_g^{p k}.
 struct PassTable {
     int npass;
+    // 1: the last pass runs across the lanes of a wavefront (wave_tail_pass); the host sets it
+    // only for lines that meet wave_tail_pass's conditions and kernels that map threads row first
+    int wave;
     int radix[kMaxPasses];
     int twoff[kMaxPasses];
 };
@@ -198,6 +201,147 @@ __device__ __forceinline__ void stockham_pass(const int N, const int s, const in
     }
 }
 
+// ---------------------------------------------------------------------------- lane exchanges
+// v of the lane whose index differs from this lane's by xor D, within its wavefront of 64 lanes,
+// in VALU instructions only (no LDS): DPP quad_perm for D = 1, 2, row_ror:8 for D = 8,
+// row_shl:4 / row_shr:4 and a select for D = 4, and gfx950's v_permlane16_swap / v_permlane32_swap
+// and a select for D = 16 / 32.  Every lane of the wavefront executes it (uniform control flow).
+// The CPU emulation harness of the tests supplies its own exchange (CFFT_EMU_LANES).
+#ifndef CFFT_EMU_LANES
+__device__ __forceinline__ int lane_id() {
+    return __builtin_amdgcn_mbcnt_hi(-1, __builtin_amdgcn_mbcnt_lo(-1, 0));
+}
+template <int D>
+__device__ __forceinline__ int lane_xor_i32(int v) {
+    if constexpr (D == 1) {
+        return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+    } else if constexpr (D == 2) {
+        return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+    } else if constexpr (D == 4) {
+        const int up = __builtin_amdgcn_update_dpp(0, v, 0x104, 0xF, 0xF, false);  // lane + 4
+        const int dn = __builtin_amdgcn_update_dpp(0, v, 0x114, 0xF, 0xF, false);  // lane - 4
+        return (lane_id() & 4) ? dn : up;
+    } else if constexpr (D == 8) {
+        return __builtin_amdgcn_update_dpp(0, v, 0x128, 0xF, 0xF, false);  // row_ror:8
+    } else if constexpr (D == 16) {
+        // swaps odd rows of the first operand with even rows of the second: with both = v, the
+        // first result holds rows {0,0,2,2} and the second rows {1,1,3,3}
+        const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+        return (lane_id() & 16) ? (int)r[0] : (int)r[1];
+    } else {
+        static_assert(D == 32, "lane distance");
+        // swaps the upper half of the first operand with the lower half of the second
+        const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+        return (lane_id() & 32) ? (int)r[0] : (int)r[1];
+    }
+}
+#else
+template <int D>
+__device__ __forceinline__ int lane_xor_i32(int v) { return cfft_emu_lane_xor(v, D); }
+#endif
+
+template <int D>
+__device__ __forceinline__ float lane_xor(float v) {
+    return __builtin_bit_cast(float, lane_xor_i32<D>(__builtin_bit_cast(int, v)));
+}
+template <int D>
+__device__ __forceinline__ double lane_xor(double v) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)lane_xor_i32<D>((int)(unsigned)u);
+    const unsigned hi = (unsigned)lane_xor_i32<D>((int)(unsigned)(u >> 32));
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+// d is uniform across the wavefront (a scalar branch).
+template <typename V>
+__device__ __forceinline__ V lane_xor_v(const V v, const int d) {
+    switch (d) {
+        case 1: return V{lane_xor<1>(v.x), lane_xor<1>(v.y)};
+        case 2: return V{lane_xor<2>(v.x), lane_xor<2>(v.y)};
+        case 4: return V{lane_xor<4>(v.x), lane_xor<4>(v.y)};
+        case 8: return V{lane_xor<8>(v.x), lane_xor<8>(v.y)};
+        case 16: return V{lane_xor<16>(v.x), lane_xor<16>(v.y)};
+        default: return V{lane_xor<32>(v.x), lane_xor<32>(v.y)};
+    }
+}
+
+__host__ __device__ constexpr int bit_reverse(int x, int bits) {
+    int r = 0;
+    for (int i = 0; i < bits; ++i) r |= ((x >> i) & 1) << (bits - 1 - i);
+    return r;
+}
+
+// The last two passes of a line whose last radix RL (a power of two, <= 16) spans lanes of one
+// wavefront.  The penultimate pass (radix R, stride s a power of two, one butterfly per thread,
+// b = t = q + s p) computes its butterfly and twiddles as stockham_pass does, but keeps the R
+// outputs in registers; the last pass -- for each output k, a DFT_RL over the digit p, whose
+// values sit in lanes s apart -- then runs as log2 RL radix-2 decimation-in-frequency stages
+// across lanes (lane_xor_v): at the stage of digit bit h the lane with that bit clear keeps
+// self + partner, the lane with it set (partner - self) w_{2h}^{p mod h}.  That replaces the
+// LDS write, barrier and LDS read between the two passes.  Lane p ends with output
+// k' = bitrev(p) of each DFT, which is out[q + s k + (N / RL) k'] of the line.  Every thread of
+// the workgroup runs the exchanges (threads without a butterfly carry zeros).  barrier: the
+// penultimate pass's reads and the last pass's writes may touch the same LDS.
+template <typename T, int R, class SRC, class DST, class TWA>
+__device__ __forceinline__ void wave_tail_pass(const int N, const int s, const int RL,
+                                               const int t_in, const bool active, const TWA& tw,
+                                               const SRC& src, const DST& dst,
+                                               const bool barrier) {
+    using V = typename Vec2<T>::type;
+    int t_ = t_in;
+    CFFT_OPAQUE(t_);
+    const int b = t_;
+    const int nb = N / R;
+    const bool live = active && b < nb;
+    const int bc = b < nb ? b : nb - 1;
+    const int p = bc / s;
+    const int q = bc - p * s;
+    constexpr int VW = (int)(sizeof(V) / 4);
+    constexpr int KP0 = TWA::kPrefetch ? tw_prefetch_vgprs(R * VW) / VW : 0;
+    constexpr int KP = (R - 1) < KP0 ? (R - 1) : KP0;
+    V w[KP > 0 ? KP : 1];
+#pragma unroll
+    for (int k = 0; k < KP; ++k) w[k] = tw.template get<R>(p, k + 1);
+    V v[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) v[j] = V{0, 0};
+    if (live) {
+#pragma unroll
+        for (int j = 0; j < R; ++j) v[j] = src(b + j * nb);
+        if constexpr (HasPost<SRC>::value) {
+#pragma unroll
+            for (int j = 0; j < R; ++j) v[j] = src.post(b + j * nb, v[j]);
+        }
+        Dft<R>::run(v);
+#pragma unroll
+        for (int k = 1; k <= KP; ++k) v[k] = cmul(v[k], w[k - 1]);
+#pragma unroll
+        for (int k = KP + 1; k < R; ++k) v[k] = cmul(v[k], tw.template get<R>(p, k));
+    }
+    for (int h = RL >> 1; h >= 1; h >>= 1) {
+        const bool upper = (p & h) != 0;
+        const T sg = upper ? T(-1) : T(1);
+        const V wl = lane_w16<V>(((p & (h - 1)) * (8 / h)) & 15);
+        const bool twiddled = upper && h > 1;
+        const int d = s * h;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const V o = lane_xor_v(v[k], d);
+            const V e = V{o.x + sg * v[k].x, o.y + sg * v[k].y};
+            const V f = cmul(e, wl);
+            v[k] = twiddled ? f : e;
+        }
+    }
+    if (barrier) __syncthreads();
+    if (live) {
+        int rl_bits = 0;
+        while ((1 << rl_bits) < RL) ++rl_bits;
+        const int kp = bit_reverse(p, rl_bits);
+        const int base = q + (N / RL) * kp;
+#pragma unroll
+        for (int k = 0; k < R; ++k) dst.put(base + s * k, v[k]);
+    }
+}
+
 // Dispatch of one pass on its radix.  The four source/destination combinations are
 // instantiated by the callers below.  Radices above RM_, a compile-time bound of the kernel, are
 // left out: a kernel built for lines of short radices holds fewer registers per lane, so more
@@ -223,6 +367,16 @@ __device__ __forceinline__ void stockham_pass(const int N, const int s, const in
         default: break;                    \
     }
 
+// Penultimate radices a wave tail takes (powers of two, within the kernel's bound RM_).
+#define CFFT_WAVE_SWITCH(RM_, R_, CALL) \
+    switch (R_) {                       \
+        CFFT_RADIX_CASE(RM_, 16, CALL)  \
+        CFFT_RADIX_CASE(RM_, 8, CALL)   \
+        CFFT_RADIX_CASE(RM_, 4, CALL)   \
+        CFFT_RADIX_CASE(RM_, 2, CALL)   \
+        default: break;                 \
+    }
+
 // The largest radix a kernel must hold for a pass table (16: every radix).
 inline int pass_table_rmax(const PassTable& pt) {
     int r = 2;
@@ -238,7 +392,7 @@ inline int pass_table_rmax(const PassTable& pt) {
 // when it staged the input into LDS.  src_aliases_lds asks for a barrier between the reads and
 // writes of the first pass because GSRC reads the line's own LDS.  TWT is a twiddle table
 // (TwGlobal or TwLds).
-template <typename T, int E, int RM = 16, class TWT, class GSRC, class GDST>
+template <typename T, int E, int RM = 16, bool WT = false, class TWT, class GSRC, class GDST>
 __device__ __forceinline__ void run_line_t(const PassTable& pt, const int N, const int t,
                                            const int Tt, const bool active, const TWT& twt,
                                            const GSRC& gsrc, const GDST& gdst,
@@ -252,6 +406,34 @@ __device__ __forceinline__ void run_line_t(const PassTable& pt, const int N, con
         const bool first = ip == ip0, last = ip == pt.npass - 1;
         const auto twa = twt.pass(pt.twoff[ip], N / (s * R));
         const bool sg = first && from_global;
+        if constexpr (WT) {
+          if (pt.wave && ip == pt.npass - 2) {
+            // the last two passes: the last one across the lanes of a wavefront
+            const int RL = pt.radix[pt.npass - 1];
+            const bool reads_lds = !sg || src_aliases_lds;
+            const bool writes_lds = !to_global || dst_aliases_lds;
+            const bool bar = reads_lds && writes_lds;
+            if (sg && to_global) {
+#define CFFT_CALL_WGG(RR) wave_tail_pass<T, RR>(N, s, RL, t, active, twa, gsrc, gdst, bar)
+                CFFT_WAVE_SWITCH(RM, R, CFFT_CALL_WGG)
+#undef CFFT_CALL_WGG
+            } else if (sg) {
+#define CFFT_CALL_WGL(RR) wave_tail_pass<T, RR>(N, s, RL, t, active, twa, gsrc, lds, bar)
+                CFFT_WAVE_SWITCH(RM, R, CFFT_CALL_WGL)
+#undef CFFT_CALL_WGL
+            } else if (to_global) {
+#define CFFT_CALL_WLG(RR) wave_tail_pass<T, RR>(N, s, RL, t, active, twa, lds, gdst, bar)
+                CFFT_WAVE_SWITCH(RM, R, CFFT_CALL_WLG)
+#undef CFFT_CALL_WLG
+            } else {
+#define CFFT_CALL_WLL(RR) wave_tail_pass<T, RR>(N, s, RL, t, active, twa, lds, lds, bar)
+                CFFT_WAVE_SWITCH(RM, R, CFFT_CALL_WLL)
+#undef CFFT_CALL_WLL
+            }
+            if (!to_global) __syncthreads();
+            return;
+          }
+        }
         const bool dg = last && to_global;
         if (sg && dg) {
 #define CFFT_CALL_GG(RR) stockham_pass<T, RR, bfly_per_thread(E, RR)>(N, s, t, Tt, active, twa, gsrc, gdst, dst_aliases_lds || src_aliases_lds)
@@ -276,8 +458,9 @@ __device__ __forceinline__ void run_line_t(const PassTable& pt, const int N, con
     }
 }
 
-// Every pass of one line with the plan's twiddle table in global memory.
-template <typename T, int E, int RM = 16, class GSRC, class GDST>
+// Every pass of one line with the plan's twiddle table in global memory.  WT: the kernel
+// instance holds the wave-tail path (pt.wave selects it at run time).
+template <typename T, int E, int RM = 16, bool WT = false, class GSRC, class GDST>
 __device__ __forceinline__ void run_line(const PassTable& pt, const int N, const int t,
                                          const int Tt, const bool active,
                                          const typename Vec2<T>::type* __restrict__ tw,
@@ -285,7 +468,7 @@ __device__ __forceinline__ void run_line(const PassTable& pt, const int N, const
                                          const LdsLine<T>& lds, const bool from_global,
                                          const bool to_global,
                                          const bool dst_aliases_lds = false) {
-    run_line_t<T, E, RM>(pt, N, t, Tt, active, TwGlobal<typename Vec2<T>::type>{tw}, gsrc, gdst, lds,
+    run_line_t<T, E, RM, WT>(pt, N, t, Tt, active, TwGlobal<typename Vec2<T>::type>{tw}, gsrc, gdst, lds,
                      from_global, to_global, dst_aliases_lds, 0, 1);
 }
 

@@ -97,6 +97,38 @@ struct ConvScatterArgs {
     const void* dc;        // per line slots {A^[0], x0} (Tc precision)
 };
 
+// Large convolution (Rader / Bluestein whose length-L FFTs run as a four-step L = N1 x N2) in
+// three launches instead of gather, two four-steps and scatter.  The second FFT uses the
+// transposed split (N2 x N1), so the lines of the first FFT's pass B are exactly the columns of
+// the second FFT's pass A and one workgroup does both:
+//   A  (k_convl_a):  the gathered (permuted / chirped, zero-padded) sequence read straight from
+//                    x as N2 columns of N1 points, FFT_N1 each -> m1[j2 N1 + k1];
+//   BA (k_convl_ba): lines k1 < N1: w_L^{j2 k1} m1[j2 N1 + k1], FFT_N2 -> X1[k1 + N1 k2], then
+//                    conj(X1 K) (and the slot A^[0] = X1[0]), FFT_N2 again -> m2[k1 N2 + k1'];
+//   C  (k_convl_c):  lines k1' < N2: w_L^{j2 k1'} m2[j2 N2 + k1'], FFT_N1 -> X2[k1' + N2 k2],
+//                    scattered into y as Rader (y[iperm[i]] = x0 + conj(X2[i]) / L) or Bluestein
+//                    (y[i] = chirp[i] conj(X2[i]) / L), scaled and conjugated for backward.
+// Ti = precision of x and y, Tc = precision of the convolution (and of m1, m2, K, chirp, dc).
+struct ConvLArgs {
+    int64_t n;             // transform length
+    int N1, N2;            // L = N1 * N2
+    int CA, CB, CC;        // lines (columns) per workgroup of the three kernels
+    int TtA, LsA, TtB, LsB, TtC, LsC;  // A and C run line a (N1 points), BA line b (N2)
+    int E;
+    int64_t nlines, idist, odist;  // rows of x and y
+    int mode, sgn;         // ConvMode; sgn = -1 conjugates the input and the output (backward)
+    double scale;          // output scale (1/n for ifft)
+    const int* perm;       // Rader: g^i mod n
+    const int* iperm;      // Rader: (g^-1)^i mod n
+    const void* chirp;     // Bluestein chirp (Tc)
+    const void* K;         // FFT_L of the convolution kernel (Tc)
+    void* m1;              // nlines x L work (Tc)
+    void* m2;              // nlines x L work (Tc)
+    void* dc;              // per line slots {A^[0], x0} (Tc)
+    const void* tw3;       // w_L^e, three-level table (Tc)
+    int tbits, tnc;
+};
+
 // Real-input transforms (n = 2m even), fused into one rows launch: forward reads `rdist`-spaced
 // rows of n reals and writes `cdist`-spaced rows of m+1 complex; backward the reverse.
 struct RealArgs {
@@ -178,6 +210,11 @@ hipError_t launch_conv_gather(Prec pio, Prec pc, const void* x, void* w, const C
                               hipStream_t st);
 hipError_t launch_conv_scatter(Prec pio, Prec pc, const void* w, void* y,
                                const ConvScatterArgs& a, hipStream_t st);
+// The three launches of a large convolution over a four-step (ConvLArgs); ptA / ptB are the pass
+// tables of the N1- and N2-point lines, twA / twB their twiddles.
+hipError_t launch_convl(Prec pio, Prec pc, const void* x, void* y, const ConvLArgs& a,
+                        const PassTable& ptA, const void* twA, const PassTable& ptB,
+                        const void* twB, hipStream_t st);
 
 // Largest per-thread complex capacity compiled (template E values).
 constexpr int kEValues[] = {8, 16};

@@ -113,10 +113,20 @@ hipError_t launch_conv_scatter(Prec pio, Prec pc, const void* w, void* y,
     return scatter_f64_f64(w, y, a, st);
 }
 
+hipError_t launch_convl(Prec pio, Prec pc, const void* x, void* y, const ConvLArgs& a,
+                        const PassTable& ptA, const void* twA, const PassTable& ptB,
+                        const void* twB, hipStream_t st) {
+    if (pio == Prec::F32)
+        return pc == Prec::F32 ? convl_f32_f32(x, y, a, ptA, twA, ptB, twB, st)
+                               : convl_f32_f64(x, y, a, ptA, twA, ptB, twB, st);
+    return convl_f64_f64(x, y, a, ptA, twA, ptB, twB, st);
+}
+
 hipError_t launch_multi(const MultiLaunch& L, const MultiArgs& ma, hipStream_t st) {
     if (L.nitems <= 0) return hipSuccess;
     const MultiKernel k = (MultiKernel)L.kernel;
-    if (k == MultiKernel::Gather || k == MultiKernel::Scatter) {
+    if (k == MultiKernel::Gather || k == MultiKernel::Scatter || k == MultiKernel::ConvLA ||
+        k == MultiKernel::ConvLBA || k == MultiKernel::ConvLC) {
         if (L.prec == (int)Prec::F32)
             return L.cprec == (int)Prec::F32 ? gs_multi_f32_f32(L, ma, st)
                                              : gs_multi_f32_f64(L, ma, st);

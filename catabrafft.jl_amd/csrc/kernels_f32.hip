@@ -45,6 +45,11 @@ hipError_t transpose_f32(const void* src, void* dst, int64_t rows, int64_t cols,
 hipError_t real_aux_f32(const void* in, void* out, const RealAuxArgs& a, hipStream_t st) {
     return real_aux_t<float>(in, out, a, st);
 }
+hipError_t convl_f32_f32(const void* x, void* y, const ConvLArgs& a, const PassTable& ptA,
+                         const void* twA, const PassTable& ptB, const void* twB, hipStream_t st) {
+    return a.E <= 8 ? convl_t<float, float, 8>(x, y, a, ptA, twA, ptB, twB, st)
+                    : convl_t<float, float, 16>(x, y, a, ptA, twA, ptB, twB, st);
+}
 // Lets every kernel of this precision use up to 160 KiB of LDS (called once per context).
 void init_kernels_f32() {
     allow_big_lds(k_rows<float, 8>);
@@ -57,6 +62,19 @@ void init_kernels_f32() {
     allow_big_lds(k_fs_b<float, 16>);
     allow_big_lds(k_conv_small<float, 8>);
     allow_big_lds(k_conv_small<float, 16>);
+    allow_big_lds(k_convl_a<float, float, 8>);
+    allow_big_lds(k_convl_a<float, float, 16>);
+    allow_big_lds(k_convl_ba<float, 8>);
+    allow_big_lds(k_convl_ba<float, 16>);
+    allow_big_lds(k_convl_c<float, float, 8>);
+    allow_big_lds(k_convl_c<float, float, 16>);
+    allow_big_lds(k_convl_a_multi<float, float>);
+    allow_big_lds(k_convl_c_multi<float, float>);
+    allow_big_lds(k_convl_ba_multi<float>);
+    allow_big_lds(k_conv_small<float, 8, true>);
+    allow_big_lds(k_conv_small<float, 16, true>);
+    allow_big_lds(k_rows<float, 8, float2, 16, true>);
+    allow_big_lds(k_rows<float, 16, float2, 16, true>);
     allow_big_lds(k_rows_multi<float>);
     allow_big_lds(k_fs_a_multi<float>);
     allow_big_lds(k_fs_b_multi<float>);

@@ -8,6 +8,9 @@ using namespace detail;
 #define CFFT_E(FN, T, E_, ...) ((E_) <= 8 ? FN<T, 8>(__VA_ARGS__) : FN<T, 16>(__VA_ARGS__))
 hipError_t rows_f64(const void* in, void* out, const RowsArgs& a, const PassTable& pt,
                     const void* tw, hipStream_t st) {
+    if (pt.wave)  // lines whose last pass runs across wavefront lanes
+        return a.E <= 8 ? rows_t<double, 8, double2, 16, true>(in, out, a, pt, tw, st)
+                        : rows_t<double, 16, double2, 16, true>(in, out, a, pt, tw, st);
     return CFFT_E(rows_t, double, a.E, in, out, a, pt, tw, st);
 }
 hipError_t fsa_f64(const void* in, void* out, const FourStepAArgs& a, const PassTable& pt,
@@ -33,6 +36,16 @@ hipError_t scatter_f32_f64(const void* w, void* y, const ConvScatterArgs& a, hip
 }
 hipError_t scatter_f64_f64(const void* w, void* y, const ConvScatterArgs& a, hipStream_t st) {
     return scatter_t<double, double>(w, y, a, st);
+}
+hipError_t convl_f32_f64(const void* x, void* y, const ConvLArgs& a, const PassTable& ptA,
+                         const void* twA, const PassTable& ptB, const void* twB, hipStream_t st) {
+    return a.E <= 8 ? convl_t<float, double, 8>(x, y, a, ptA, twA, ptB, twB, st)
+                    : convl_t<float, double, 16>(x, y, a, ptA, twA, ptB, twB, st);
+}
+hipError_t convl_f64_f64(const void* x, void* y, const ConvLArgs& a, const PassTable& ptA,
+                         const void* twA, const PassTable& ptB, const void* twB, hipStream_t st) {
+    return a.E <= 8 ? convl_t<double, double, 8>(x, y, a, ptA, twA, ptB, twB, st)
+                    : convl_t<double, double, 16>(x, y, a, ptA, twA, ptB, twB, st);
 }
 hipError_t multi_f64(const MultiLaunch& L, const MultiArgs& ma, hipStream_t st) {
     return multi_t<double>(L, ma, st);
@@ -65,6 +78,25 @@ void init_kernels_f64() {
     allow_big_lds(k_fs_b<double, 16>);
     allow_big_lds(k_conv_small<double, 8>);
     allow_big_lds(k_conv_small<double, 16>);
+    allow_big_lds(k_convl_a<float, double, 8>);
+    allow_big_lds(k_convl_a<float, double, 16>);
+    allow_big_lds(k_convl_a<double, double, 8>);
+    allow_big_lds(k_convl_a<double, double, 16>);
+    allow_big_lds(k_convl_ba<double, 8>);
+    allow_big_lds(k_convl_ba<double, 16>);
+    allow_big_lds(k_convl_c<float, double, 8>);
+    allow_big_lds(k_convl_c<float, double, 16>);
+    allow_big_lds(k_convl_c<double, double, 8>);
+    allow_big_lds(k_convl_c<double, double, 16>);
+    allow_big_lds(k_convl_a_multi<float, double>);
+    allow_big_lds(k_convl_c_multi<float, double>);
+    allow_big_lds(k_convl_a_multi<double, double>);
+    allow_big_lds(k_convl_c_multi<double, double>);
+    allow_big_lds(k_convl_ba_multi<double>);
+    allow_big_lds(k_conv_small<double, 8, true>);
+    allow_big_lds(k_conv_small<double, 16, true>);
+    allow_big_lds(k_rows<double, 8, double2, 16, true>);
+    allow_big_lds(k_rows<double, 16, double2, 16, true>);
     allow_big_lds(k_rows_multi<double>);
     allow_big_lds(k_fs_a_multi<double>);
     allow_big_lds(k_fs_b_multi<double>);

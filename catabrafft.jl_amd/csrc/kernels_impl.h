@@ -233,7 +233,7 @@ struct ItemLines {  // an MItem's lines: uniform distances or a table of {in, ou
 // rows body: up to C lines of length N (`count` valid).  A line with no passes (N = 1) is a
 // scaled copy.
 // ------------------------------------------------------------------------------------------
-template <typename T, int E, int RM = 16, class LINES>
+template <typename T, int E, int RM = 16, bool WT = false, class LINES>
 __device__ __forceinline__ void rows_body(unsigned char* smem, const LINES& lines, int count,
                                           int N, int C, int Tt, int Ls, const PassTable& pt,
                                           const V_t<T>* tw, int sgn, double scale,
@@ -253,11 +253,12 @@ __device__ __forceinline__ void rows_body(unsigned char* smem, const LINES& line
         return;
     }
     const LdsLine<T> L = lds_line<T>(smem, C, Ls, c < C ? c : 0);
-    run_line<T, E, RM>(pt, N, t, Tt, active, tw, gs, gd, L, true, true);
+    run_line<T, E, RM, WT>(pt, N, t, Tt, active, tw, gs, gd, L, true, true);
 }
 
-// RM bounds the radices the kernel holds (lines of radices <= 9 run on a lighter instance).
-template <typename T, int E, typename G = V_t<T>, int RM = 16>
+// RM bounds the radices the kernel holds (lines of radices <= 9 run on a lighter instance); WT
+// instances hold the wave-tail path (lines whose last pass runs across wavefront lanes).
+template <typename T, int E, typename G = V_t<T>, int RM = 16, bool WT = false>
 __global__ void __launch_bounds__(1024)
 k_rows(const G* __restrict__ in, G* __restrict__ out, RowsArgs a, PassTable pt,
        const V_t<T>* __restrict__ tw) {
@@ -267,7 +268,7 @@ k_rows(const G* __restrict__ in, G* __restrict__ out, RowsArgs a, PassTable pt,
     const int count = (int)(left < a.C ? left : a.C);
     UniformLines<T, G, G> lines{in + line0 * a.idist, out + line0 * a.odist, a.idist, a.odist,
                                 a.epi.dc ? (V_t<T>*)a.epi.dc + 2 * line0 : nullptr};
-    rows_body<T, E, RM>(smem, lines, count, a.N, a.C, a.Tt, a.Ls, pt, tw, a.sgn_in, a.scale,
+    rows_body<T, E, RM, WT>(smem, lines, count, a.N, a.C, a.Tt, a.Ls, pt, tw, a.sgn_in, a.scale,
                         (const V_t<T>*)a.epi.K);
 }
 
@@ -573,7 +574,7 @@ struct ScatterDst {  // inverse result c = conj(v)/L into the output X
     }
 };
 
-template <typename T, int E, class LINES>
+template <typename T, int E, bool WT = false, class LINES>
 __device__ __forceinline__ void conv_body(unsigned char* smem, const LINES& lines, int nl, int n,
                                           int L, int C, int Tt, int Ls, int mode, int sgn,
                                           double scale, const int* perm, const int* iperm,
@@ -607,12 +608,12 @@ __device__ __forceinline__ void conv_body(unsigned char* smem, const LINES& line
     // 2. forward FFT_L of the gathered / chirped sequence; pointwise product with K on the way out
     GatherSrc<T> gsrc{X, perm, chirp, n, mode};
     ConvEpiDst<T> edst{A, K, sdc + cc};
-    run_line_t<T, E>(pt, L, t, Tt, active, TwGlobal<V_t<T>>{tw}, gsrc, edst, A, true, true, true,
+    run_line_t<T, E, 16, WT>(pt, L, t, Tt, active, TwGlobal<V_t<T>>{tw}, gsrc, edst, A, true, true, true,
                      0, 1, true);
     __syncthreads();
     // 3. second forward FFT_L of the conjugated product = conj of the inverse; scatter into X
     ScatterDst<T> sdst{X, iperm, chirp, x0, (T)(1.0 / L), n, mode};
-    run_line<T, E>(pt, L, t, Tt, active, tw, A, sdst, A, false, true, true);
+    run_line<T, E, 16, WT>(pt, L, t, Tt, active, tw, A, sdst, A, false, true, true);
     __syncthreads();
     if (mode == (int)ConvMode::Rader && active && t == 0) {
         const V_t<T> d = sdc[cc];
@@ -633,7 +634,7 @@ __device__ __forceinline__ void conv_body(unsigned char* smem, const LINES& line
         });
 }
 
-template <typename T, int E>
+template <typename T, int E, bool WT = false>
 __global__ void __launch_bounds__(1024)
 k_conv_small(const V_t<T>* __restrict__ in, V_t<T>* __restrict__ out, ConvSmallArgs a,
              PassTable pt, const V_t<T>* __restrict__ tw) {
@@ -641,7 +642,7 @@ k_conv_small(const V_t<T>* __restrict__ in, V_t<T>* __restrict__ out, ConvSmallA
     const int64_t line0 = (int64_t)blockIdx.x * a.C;
     const int nl = (int)((a.nlines - line0) < a.C ? (a.nlines - line0) : a.C);
     UniformLines<T> lines{in + line0 * a.idist, out + line0 * a.odist, a.idist, a.odist, nullptr};
-    conv_body<T, E>(smem, lines, nl, a.n, a.L, a.C, a.Tt, a.Ls, a.mode, a.sgn_in,
+    conv_body<T, E, WT>(smem, lines, nl, a.n, a.L, a.C, a.Tt, a.Ls, a.mode, a.sgn_in,
                     a.scale, a.perm, a.iperm, (const V_t<T>*)a.chirp, (const V_t<T>*)a.K, pt, tw);
 }
 
@@ -765,6 +766,226 @@ k_conv_scatter(const V_t<Tc>* __restrict__ w, V_t<Ti>* __restrict__ y, ConvScatt
                              (int64_t)blockIdx.x * blockDim.x + threadIdx.x, a.n,
                              (int64_t)gridDim.x * blockDim.x, a.n, a.mode, a.sgn_in, a.invL,
                              a.scale, a.iperm, (const V_t<Tc>*)a.chirp);
+}
+
+// ------------------------------------------------------------------------------------------
+// Large convolution over a four-step, three launches (ConvLArgs in kernels.h).  Ti: precision
+// of x and y; Tc: precision of the convolution.
+// ------------------------------------------------------------------------------------------
+// Column j2 of the gathered sequence as an N1 x N2 matrix: element e is a[e N2 + j2], with
+// Rader a[i] = x[perm[i]] (i < n - 1), Bluestein a[i] = x[i] chirp[i] (i < n), zero beyond (the
+// padding); sg = -1 conjugates x.
+template <typename Ti, typename Tc>
+struct GConvGatherCol {
+    const V_t<Ti>* x;
+    const int* perm;
+    const V_t<Tc>* chirp;
+    int64_t live;  // n - 1 (Rader) or n (Bluestein)
+    int N2, j2;
+    bool rader;
+    Tc sg;
+    __device__ __forceinline__ V_t<Tc> operator()(int e) const {
+        const int64_t i = (int64_t)e * N2 + j2;
+        if (i >= live) return V_t<Tc>{0, 0};
+        const V_t<Ti> u = x[rader ? (int64_t)perm[i] : i];
+        const V_t<Tc> v = V_t<Tc>{(Tc)u.x, (Tc)u.y * sg};
+        return rader ? v : cmul(v, chirp[i]);
+    }
+};
+
+// The bodies below run one tile of one transform: xl / yl its input / output row, m1 / m2 its
+// two work lines, slot its {A^[0], x0}.  The single-plan kernels and the work-list kernels
+// (one item per workgroup) call the same bodies.
+
+// A: columns j20 .. j20 + C - 1 (j20 = tile C); FFT_N1 of the gathered column -> m1[j2 N1 + k1].
+// The first tile keeps x0 for the scatter (Rader).
+template <typename Ti, typename Tc, int E>
+__device__ __forceinline__ void convl_a_body(unsigned char* smem, const ConvLArgs& a, int tile,
+                                             const V_t<Ti>* xl, V_t<Tc>* m1, V_t<Tc>* slot,
+                                             const PassTable& pt, const V_t<Tc>* tw) {
+    const int C = a.CA;
+    const int cshift = __ffs(C) - 1;
+    const int tid = threadIdx.x;
+    const int c = tid & (C - 1);
+    const int t = tid >> cshift;
+    const int j2 = tile * C + c;
+    const bool active = j2 < a.N2;
+    const int j2c = active ? j2 : 0;
+    const bool rader = a.mode == (int)ConvMode::Rader;
+    const Tc sg = (Tc)a.sgn;
+    if (rader && tile == 0 && tid == 0) {
+        const V_t<Ti> u0 = xl[0];
+        slot[1] = V_t<Tc>{(Tc)u0.x, (Tc)u0.y * sg};
+    }
+    GConvGatherCol<Ti, Tc> gs{xl, a.perm, (const V_t<Tc>*)a.chirp, rader ? a.n - 1 : a.n, a.N2,
+                              j2c, rader, sg};
+    GPlainDst<Tc, false> d{m1 + (int64_t)j2c * a.N1};
+    run_line<Tc, E>(pt, a.N1, t, a.TtA, active, tw, gs, d, lds_line<Tc>(smem, C, a.LsA, c), true,
+                    true);
+}
+
+// BA: lines k10 .. k10 + C - 1 (k10 = tile C, k1 < N1): FFT_N2 of w_L^{j2 k1} m1[j2 N1 + k1] (the
+// first FFT's pass B) in LDS; conj(X1 K[k1 + N1 k2]) (A^[0] = X1[0] kept in slot[0]); FFT_N2 of
+// that (the second FFT's pass A, its column k1) -> m2[k1 N2 + k1'].
+template <typename Tc, int E>
+__device__ __forceinline__ void convl_ba_body(unsigned char* smem, const ConvLArgs& a, int tile,
+                                              const V_t<Tc>* m1, V_t<Tc>* m2, V_t<Tc>* slot,
+                                              const PassTable& pt, const V_t<Tc>* tw) {
+    const int C = a.CB;
+    const int k10 = tile * C;
+    const int cshift = __ffs(C) - 1;
+    V_t<Tc>* t3 = reinterpret_cast<V_t<Tc>*>(smem + fsb_tw_offset(sizeof(Tc), C, a.LsB));
+    stage_table<Tc>((const V_t<Tc>*)a.tw3, t3, tw3_entries(a.tbits, a.tnc));
+    __syncthreads();
+    const int tid = threadIdx.x;
+    const int c = tid & (C - 1);
+    const int t = tid >> cshift;
+    const bool active = k10 + c < a.N1;
+    const int k1 = active ? k10 + c : 0;
+    const LdsLine<Tc> Ld = lds_line<Tc>(smem, C, a.LsB, c);
+    GTwColSrc<Tc> gs{m1 + k1, a.N1, t3, a.tbits, k1, (Tc)1};
+    run_line<Tc, E>(pt, a.N2, t, a.TtB, active, tw, gs, Ld, Ld, true, false);
+    // run_line ended with a barrier; the epilogue reads and rewrites each element in place, the
+    // K loads of a batch sharing one wait
+    const V_t<Tc>* K = (const V_t<Tc>*)a.K;
+    batched_for<kStageU>(
+        tid, blockDim.x, C * a.N2,
+        [&](int idx) {
+            const int cc = idx & (C - 1);
+            const int e = idx >> cshift;
+            const int kk = k10 + cc < a.N1 ? k10 + cc : a.N1 - 1;
+            return K[(int64_t)kk + (int64_t)a.N1 * e];
+        },
+        [&](int idx, V_t<Tc> kv) {
+            const int cc = idx & (C - 1);
+            const int e = idx >> cshift;
+            if (k10 + cc >= a.N1) return;
+            const LdsLine<Tc> Lc = lds_line<Tc>(smem, C, a.LsB, cc);
+            const V_t<Tc> v = Lc(e);
+            if (k10 + cc == 0 && e == 0) slot[0] = v;
+            Lc.put(e, cconj(cmul(v, kv)));
+        });
+    __syncthreads();
+    GPlainDst<Tc, false> d{m2 + (int64_t)k1 * a.N2};
+    run_line<Tc, E>(pt, a.N2, t, a.TtB, active, tw, Ld, d, Ld, false, true);
+}
+
+// C: lines k10 .. k10 + C - 1 (k1' < N2): FFT_N1 of w_L^{j2 k1'} m2[j2 N2 + k1'] (the second
+// FFT's pass B) in LDS, then X2[i], i = k1' + N2 k2, scattered into y: Rader y[iperm[i]] =
+// x0 + conj(X2[i]) / L (and y[0] = x0 + A^[0]), Bluestein y[i] = chirp[i] conj(X2[i]) / L;
+// real part times scale, imaginary part times scale * sgn.
+template <typename Tc>
+struct ConvLScatterLoad {
+    V_t<Tc> v, w;
+    int64_t dst;
+};
+
+template <typename Ti, typename Tc, int E>
+__device__ __forceinline__ void convl_c_body(unsigned char* smem, const ConvLArgs& a, int tile,
+                                             const V_t<Tc>* m2, V_t<Ti>* yl,
+                                             const V_t<Tc>* slots, const PassTable& pt,
+                                             const V_t<Tc>* tw) {
+    const int C = a.CC;
+    const int k10 = tile * C;
+    const int cshift = __ffs(C) - 1;
+    const int64_t L = (int64_t)a.N1 * a.N2;
+    V_t<Tc>* t3 = reinterpret_cast<V_t<Tc>*>(smem + fsb_tw_offset(sizeof(Tc), C, a.LsC));
+    stage_table<Tc>((const V_t<Tc>*)a.tw3, t3, tw3_entries(a.tbits, a.tnc));
+    __syncthreads();
+    const int tid = threadIdx.x;
+    const int c = tid & (C - 1);
+    const int t = tid >> cshift;
+    const bool active = k10 + c < a.N2;
+    const int k1 = active ? k10 + c : 0;
+    const LdsLine<Tc> Ld = lds_line<Tc>(smem, C, a.LsC, c);
+    GTwColSrc<Tc> gs{m2 + k1, a.N2, t3, a.tbits, k1, (Tc)1};
+    run_line<Tc, E>(pt, a.N1, t, a.TtC, active, tw, gs, Ld, Ld, true, false);
+    const bool rader = a.mode == (int)ConvMode::Rader;
+    const Tc invL = (Tc)(1.0 / (double)L);
+    const Tc sx = (Tc)a.scale, sy = (Tc)(a.scale * a.sgn);
+    const int64_t live = rader ? a.n - 1 : a.n;
+    const V_t<Tc> x0 = rader ? slots[1] : V_t<Tc>{0, 0};
+    if (rader && tile == 0 && tid == 0) {
+        const V_t<Tc> d0 = slots[0];
+        yl[0] = V_t<Ti>{(Ti)((x0.x + d0.x) * sx), (Ti)((x0.y + d0.y) * sy)};
+    }
+    const int* iperm = a.iperm;
+    const V_t<Tc>* chirp = (const V_t<Tc>*)a.chirp;
+    batched_for<kStageU>(
+        tid, blockDim.x, C * a.N1,
+        [&](int idx) {
+            const int cc = idx & (C - 1);
+            const int e = idx >> cshift;
+            const int64_t i0 = (int64_t)(k10 + cc < a.N2 ? k10 + cc : 0) + (int64_t)a.N2 * e;
+            const int64_t i = i0 < live ? i0 : live - 1;
+            ConvLScatterLoad<Tc> r;
+            r.v = lds_line<Tc>(smem, C, a.LsC, cc)(e);
+            if (rader) {
+                r.dst = iperm[i];
+                r.w = V_t<Tc>{0, 0};
+            } else {
+                r.dst = i;
+                r.w = chirp[i];
+            }
+            return r;
+        },
+        [&](int idx, ConvLScatterLoad<Tc> r) {
+            const int cc = idx & (C - 1);
+            const int e = idx >> cshift;
+            if (k10 + cc >= a.N2) return;
+            const int64_t i = (int64_t)(k10 + cc) + (int64_t)a.N2 * e;
+            if (i >= live) return;
+            const V_t<Tc> cv = V_t<Tc>{r.v.x * invL, -r.v.y * invL};
+            const V_t<Tc> o = rader ? V_t<Tc>{x0.x + cv.x, x0.y + cv.y} : cmul(cv, r.w);
+            yl[r.dst] = V_t<Ti>{(Ti)(o.x * sx), (Ti)(o.y * sy)};
+        });
+}
+
+// Single-plan kernels: grid = tiles x lines, consecutive tiles of a line on one XCD.
+__device__ __forceinline__ void convl_tile(int tiles, int64_t& line, int& tile) {
+    const int bid = xcd_block(1);
+    line = bid / tiles;
+    tile = bid - (int)(line * tiles);
+}
+
+template <typename Ti, typename Tc, int E>
+__global__ void __launch_bounds__(1024)
+k_convl_a(const V_t<Ti>* __restrict__ x, ConvLArgs a, PassTable pt,
+          const V_t<Tc>* __restrict__ tw) {
+    CFFT_DYN_LDS(smem);
+    int64_t line;
+    int tile;
+    convl_tile((a.N2 + a.CA - 1) / a.CA, line, tile);
+    const int64_t L = (int64_t)a.N1 * a.N2;
+    convl_a_body<Ti, Tc, E>(smem, a, tile, x + line * a.idist,
+                            reinterpret_cast<V_t<Tc>*>(a.m1) + line * L,
+                            reinterpret_cast<V_t<Tc>*>(a.dc) + 2 * line, pt, tw);
+}
+
+template <typename Tc, int E>
+__global__ void __launch_bounds__(1024)
+k_convl_ba(ConvLArgs a, PassTable pt, const V_t<Tc>* __restrict__ tw) {
+    CFFT_DYN_LDS(smem);
+    int64_t line;
+    int tile;
+    convl_tile((a.N1 + a.CB - 1) / a.CB, line, tile);
+    const int64_t L = (int64_t)a.N1 * a.N2;
+    convl_ba_body<Tc, E>(smem, a, tile, reinterpret_cast<const V_t<Tc>*>(a.m1) + line * L,
+                         reinterpret_cast<V_t<Tc>*>(a.m2) + line * L,
+                         reinterpret_cast<V_t<Tc>*>(a.dc) + 2 * line, pt, tw);
+}
+
+template <typename Ti, typename Tc, int E>
+__global__ void __launch_bounds__(1024)
+k_convl_c(V_t<Ti>* __restrict__ y, ConvLArgs a, PassTable pt, const V_t<Tc>* __restrict__ tw) {
+    CFFT_DYN_LDS(smem);
+    int64_t line;
+    int tile;
+    convl_tile((a.N2 + a.CC - 1) / a.CC, line, tile);
+    const int64_t L = (int64_t)a.N1 * a.N2;
+    convl_c_body<Ti, Tc, E>(smem, a, tile, reinterpret_cast<const V_t<Tc>*>(a.m2) + line * L,
+                            y + line * a.odist,
+                            reinterpret_cast<const V_t<Tc>*>(a.dc) + 2 * line, pt, tw);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1035,6 +1256,86 @@ k_scatter_multi(MultiArgs ma, const MItem* __restrict__ items, const MPlan* __re
                          (const V_t<Tc>*)P->chirp);
 }
 
+// Fused large-convolution launches: one item per tile (count = tile index); in / out are the
+// tile's input and output lines (x, m1 / m1, m2 / m2, y), dc its slots; the plan carries the
+// convolution tables and its FFT_L four-step (N1, N2, tiles, lines, twiddles).
+__device__ __forceinline__ ConvLArgs convl_args_of(const MPlan& P, int sgn, double scale) {
+    ConvLArgs a{};
+    a.n = P.n;
+    a.N1 = P.N1;
+    a.N2 = P.N2;
+    a.CA = P.CA;
+    a.CB = P.CB;
+    a.CC = P.CC;
+    a.TtA = P.la.Tt;
+    a.LsA = P.la.Ls;
+    a.TtB = P.lb.Tt;
+    a.LsB = P.lb.Ls;
+    a.TtC = P.la.Tt;
+    a.LsC = P.LsC;
+    a.E = kMultiE;
+    a.nlines = 1;
+    a.mode = P.mode;
+    a.sgn = sgn;
+    a.scale = scale;
+    a.perm = P.perm;
+    a.iperm = P.iperm;
+    a.chirp = P.chirp;
+    a.K = P.K;
+    a.tw3 = P.tw3;
+    a.tbits = P.tbits;
+    a.tnc = P.tnc;
+    return a;
+}
+
+template <typename Ti, typename Tc>
+__global__ void __launch_bounds__(1024)
+k_convl_a_multi(MultiArgs ma, const MItem* __restrict__ items, const MPlan* __restrict__ plans) {
+    CFFT_DYN_LDS(smem);
+    const MItem it = items[xcd_item()];
+    const MPlan* P = plans + it.plan;
+    int sgn;
+    double scale;
+    item_direction(it, P->n, ma, sgn, scale);
+    const ConvLArgs a = convl_args_of(*P, sgn, scale);
+    convl_a_body<Ti, Tc, kMultiE>(
+        smem, a, it.count, mptr<const V_t<Ti>>(it.in, it.flags & kSelInMask, ma.in_base, ma.out_base),
+        mptr<V_t<Tc>>(it.out, (it.flags >> kSelOutShift) & 15, ma.in_base, ma.out_base),
+        reinterpret_cast<V_t<Tc>*>(it.dc), P->la.pt, (const V_t<Tc>*)P->la.tw);
+}
+
+template <typename Tc>
+__global__ void __launch_bounds__(1024)
+k_convl_ba_multi(MultiArgs ma, const MItem* __restrict__ items, const MPlan* __restrict__ plans) {
+    CFFT_DYN_LDS(smem);
+    const MItem it = items[xcd_item()];
+    const MPlan* P = plans + it.plan;
+    int sgn;
+    double scale;
+    item_direction(it, P->n, ma, sgn, scale);
+    const ConvLArgs a = convl_args_of(*P, sgn, scale);
+    convl_ba_body<Tc, kMultiE>(
+        smem, a, it.count, mptr<const V_t<Tc>>(it.in, it.flags & kSelInMask, ma.in_base, ma.out_base),
+        mptr<V_t<Tc>>(it.out, (it.flags >> kSelOutShift) & 15, ma.in_base, ma.out_base),
+        reinterpret_cast<V_t<Tc>*>(it.dc), P->lb.pt, (const V_t<Tc>*)P->lb.tw);
+}
+
+template <typename Ti, typename Tc>
+__global__ void __launch_bounds__(1024)
+k_convl_c_multi(MultiArgs ma, const MItem* __restrict__ items, const MPlan* __restrict__ plans) {
+    CFFT_DYN_LDS(smem);
+    const MItem it = items[xcd_item()];
+    const MPlan* P = plans + it.plan;
+    int sgn;
+    double scale;
+    item_direction(it, P->n, ma, sgn, scale);
+    const ConvLArgs a = convl_args_of(*P, sgn, scale);
+    convl_c_body<Ti, Tc, kMultiE>(
+        smem, a, it.count, mptr<const V_t<Tc>>(it.in, it.flags & kSelInMask, ma.in_base, ma.out_base),
+        mptr<V_t<Ti>>(it.out, (it.flags >> kSelOutShift) & 15, ma.in_base, ma.out_base),
+        reinterpret_cast<const V_t<Tc>*>(it.dc), P->la.pt, (const V_t<Tc>*)P->la.tw);
+}
+
 // ------------------------------------------------------------------------------------------
 // Launchers
 // ------------------------------------------------------------------------------------------
@@ -1048,13 +1349,13 @@ void allow_big_lds(K kernel) {
 }
 
 // Launchers; G is the storage type of the lines' global input / output (V_t<T> or ch16).
-template <typename T, int E, typename G = V_t<T>, int RM = 16>
+template <typename T, int E, typename G = V_t<T>, int RM = 16, bool WT = false>
 hipError_t rows_t(const void* in, void* out, const RowsArgs& a, const PassTable& pt,
                   const void* tw, hipStream_t st) {
     const size_t lds = 2 * sizeof(T) * (size_t)a.C * a.Ls;
     const int64_t blocks = (a.nlines + a.C - 1) / a.C;
-    hipLaunchKernelGGL((k_rows<T, E, G, RM>), dim3((unsigned)blocks), dim3(a.C * a.Tt), lds, st,
-                       (const G*)in, (G*)out, a, pt, (const V_t<T>*)tw);
+    hipLaunchKernelGGL((k_rows<T, E, G, RM, WT>), dim3((unsigned)blocks), dim3(a.C * a.Tt), lds,
+                       st, (const G*)in, (G*)out, a, pt, (const V_t<T>*)tw);
     return hipGetLastError();
 }
 
@@ -1063,6 +1364,11 @@ hipError_t rows_t(const void* in, void* out, const RowsArgs& a, const PassTable&
 template <typename T, typename G = V_t<T>>
 hipError_t rows_rm_t(const void* in, void* out, const RowsArgs& a, const PassTable& pt,
                      const void* tw, hipStream_t st) {
+    if constexpr (std::is_same<G, V_t<T>>::value) {
+        if (pt.wave)  // lines with a wave tail (planned for float / double storage only)
+            return a.E <= 8 ? rows_t<T, 8, G, 16, true>(in, out, a, pt, tw, st)
+                            : rows_t<T, 16, G, 16, true>(in, out, a, pt, tw, st);
+    }
     if (pass_table_rmax(pt) <= 9)
         return a.E <= 8 ? rows_t<T, 8, G, 9>(in, out, a, pt, tw, st)
                         : rows_t<T, 16, G, 9>(in, out, a, pt, tw, st);
@@ -1096,8 +1402,12 @@ hipError_t conv_t(const void* in, void* out, const ConvSmallArgs& a, const PassT
                   const void* tw, hipStream_t st) {
     const size_t lds = 2 * sizeof(T) * (size_t)a.C * a.Ls + sizeof(V_t<T>) * (size_t)a.C;
     const int64_t blocks = (a.nlines + a.C - 1) / a.C;
-    hipLaunchKernelGGL((k_conv_small<T, E>), dim3((unsigned)blocks), dim3(a.C * a.Tt), lds, st,
-                       (const V_t<T>*)in, (V_t<T>*)out, a, pt, (const V_t<T>*)tw);
+    if (pt.wave)
+        hipLaunchKernelGGL((k_conv_small<T, E, true>), dim3((unsigned)blocks), dim3(a.C * a.Tt),
+                           lds, st, (const V_t<T>*)in, (V_t<T>*)out, a, pt, (const V_t<T>*)tw);
+    else
+        hipLaunchKernelGGL((k_conv_small<T, E>), dim3((unsigned)blocks), dim3(a.C * a.Tt), lds,
+                           st, (const V_t<T>*)in, (V_t<T>*)out, a, pt, (const V_t<T>*)tw);
     return hipGetLastError();
 }
 
@@ -1118,6 +1428,31 @@ hipError_t scatter_t(const void* w, void* y, const ConvScatterArgs& a, hipStream
     const unsigned gy = (unsigned)(a.nlines < 65535 ? a.nlines : 65535);
     hipLaunchKernelGGL((k_conv_scatter<Ti, Tc>), dim3(gx, gy), dim3(256), 0, st,
                        (const V_t<Tc>*)w, (V_t<Ti>*)y, a);
+    return hipGetLastError();
+}
+
+template <typename Ti, typename Tc, int E>
+hipError_t convl_t(const void* x, void* y, const ConvLArgs& a, const PassTable& ptA,
+                   const void* twA, const PassTable& ptB, const void* twB, hipStream_t st) {
+    const size_t es = sizeof(Tc);
+    const size_t t3 = (size_t)tw3_entries(a.tbits, a.tnc) * 2 * es;
+    const size_t ldsA = 2 * es * (size_t)a.CA * a.LsA;
+    const size_t ldsB = fsb_tw_offset(es, a.CB, a.LsB) + t3;
+    const size_t ldsC = fsb_tw_offset(es, a.CC, a.LsC) + t3;
+    const int64_t gA = (int64_t)((a.N2 + a.CA - 1) / a.CA) * a.nlines;
+    const int64_t gB = (int64_t)((a.N1 + a.CB - 1) / a.CB) * a.nlines;
+    const int64_t gC = (int64_t)((a.N2 + a.CC - 1) / a.CC) * a.nlines;
+    if (gA > 0x7fffffff || gB > 0x7fffffff || gC > 0x7fffffff) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_convl_a<Ti, Tc, E>), dim3((unsigned)gA), dim3(a.CA * a.TtA), ldsA, st,
+                       (const V_t<Ti>*)x, a, ptA, (const V_t<Tc>*)twA);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_convl_ba<Tc, E>), dim3((unsigned)gB), dim3(a.CB * a.TtB), ldsB, st, a,
+                       ptB, (const V_t<Tc>*)twB);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_convl_c<Ti, Tc, E>), dim3((unsigned)gC), dim3(a.CC * a.TtC), ldsC, st,
+                       (V_t<Ti>*)y, a, ptA, (const V_t<Tc>*)twA);
     return hipGetLastError();
 }
 
@@ -1166,13 +1501,29 @@ hipError_t multi_t(const MultiLaunch& L, const MultiArgs& ma, hipStream_t st) {
 // Work-list gather / scatter of large convolutions (data Ti, work buffer Tc).
 template <typename Ti, typename Tc>
 hipError_t gs_multi_t(const MultiLaunch& L, const MultiArgs& ma, hipStream_t st) {
-    const dim3 grid((unsigned)L.nitems), block(256);
-    if ((MultiKernel)L.kernel == MultiKernel::Gather)
-        hipLaunchKernelGGL((k_gather_multi<Ti, Tc>), grid, block, 0, st, ma, L.items, L.plans);
-    else if ((MultiKernel)L.kernel == MultiKernel::Scatter)
-        hipLaunchKernelGGL((k_scatter_multi<Ti, Tc>), grid, block, 0, st, ma, L.items, L.plans);
-    else
-        return hipErrorInvalidValue;
+    const dim3 grid((unsigned)L.nitems), block(256), lblock(L.block);
+    switch ((MultiKernel)L.kernel) {
+        case MultiKernel::Gather:
+            hipLaunchKernelGGL((k_gather_multi<Ti, Tc>), grid, block, 0, st, ma, L.items, L.plans);
+            break;
+        case MultiKernel::Scatter:
+            hipLaunchKernelGGL((k_scatter_multi<Ti, Tc>), grid, block, 0, st, ma, L.items, L.plans);
+            break;
+        case MultiKernel::ConvLA:
+            hipLaunchKernelGGL((k_convl_a_multi<Ti, Tc>), grid, lblock, L.lds, st, ma, L.items,
+                               L.plans);
+            break;
+        case MultiKernel::ConvLBA:
+            hipLaunchKernelGGL((k_convl_ba_multi<Tc>), grid, lblock, L.lds, st, ma, L.items,
+                               L.plans);
+            break;
+        case MultiKernel::ConvLC:
+            hipLaunchKernelGGL((k_convl_c_multi<Ti, Tc>), grid, lblock, L.lds, st, ma, L.items,
+                               L.plans);
+            break;
+        default:
+            return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -72,4 +72,11 @@ hipError_t scatter_f32_f32(const void* w, void* y, const ConvScatterArgs& a, hip
 hipError_t scatter_f32_f64(const void* w, void* y, const ConvScatterArgs& a, hipStream_t st);
 hipError_t scatter_f64_f64(const void* w, void* y, const ConvScatterArgs& a, hipStream_t st);
 
+// Large convolutions over a four-step, three launches (k_convl_a, k_convl_ba, k_convl_c).
+hipError_t convl_f32_f32(const void* x, void* y, const ConvLArgs& a, const PassTable& ptA,
+                         const void* twA, const PassTable& ptB, const void* twB, hipStream_t st);
+hipError_t convl_f32_f64(const void* x, void* y, const ConvLArgs& a, const PassTable& ptA,
+                         const void* twA, const PassTable& ptB, const void* twB, hipStream_t st);
+hipError_t convl_f64_f64(const void* x, void* y, const ConvLArgs& a, const PassTable& ptA,
+                         const void* twA, const PassTable& ptB, const void* twB, hipStream_t st);
 }  // namespace cfft

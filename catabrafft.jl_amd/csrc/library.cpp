@@ -237,8 +237,22 @@ int line_stride_for(Prec p, int64_t N) {
     return p == Prec::F32 ? line_stride<float>(N) : line_stride<double>(N);
 }
 
-// Builds the pass table, thread count and twiddles of one line FFT.  E = 0 picks the default.
-int make_line(LinePlan& lp, int64_t N, Prec pr, int E, int max_threads) {
+bool radices_valid(const std::vector<int>& rad, int64_t N) {
+    if (rad.empty() || (int)rad.size() > kMaxPasses) return false;
+    int64_t prod = 1;
+    for (int R : rad) {
+        bool known = false;
+        for (int k : {2, 3, 4, 5, 7, 8, 9, 11, 13, 16}) known = known || R == k;
+        if (!known || prod > N) return false;
+        prod *= R;
+    }
+    return prod == N;
+}
+
+// Builds the pass table, thread count and twiddles of one line FFT.  E = 0 picks the default;
+// `radices` (any order of supported radices whose product is N) replaces the default sequence.
+int make_line(LinePlan& lp, int64_t N, Prec pr, int E, int max_threads,
+              const std::vector<int>* radices) {
     lp.N = (int)N;
     if (N == 1) {  // no passes: the rows kernel copies
         lp.E = E ? E : 16;
@@ -248,6 +262,12 @@ int make_line(LinePlan& lp, int64_t N, Prec pr, int E, int max_threads) {
         return upload(lp.dtw, std::vector<cld>{cld(1, 0)}, pr);
     }
     std::vector<int> rad = line_radices(N);
+    if (radices && !radices->empty()) {
+        if (!radices_valid(*radices, N))
+            return fail(CFFT_ERR_UNSUPPORTED,
+                        "radix sequence does not factor the line length " + std::to_string(N));
+        rad = *radices;
+    }
     if (rad.empty() || (int)rad.size() > kMaxPasses)
         return fail(CFFT_ERR_UNSUPPORTED, "line length " + std::to_string(N) + " is not smooth");
     // The kernels are instantiated for E = 8 and E = 16 (a launch picks E <= 8 ? 8 : 16), so
@@ -259,6 +279,7 @@ int make_line(LinePlan& lp, int64_t N, Prec pr, int E, int max_threads) {
         return fail(CFFT_ERR_UNSUPPORTED, "line length " + std::to_string(N) + " needs " +
                                               std::to_string(lp.Tt) + " threads");
     lp.pt.npass = (int)rad.size();
+    lp.pt.wave = 0;
     std::vector<cld> tw;
     const OmegaTable om(N);  // w_g^{pk} = w_N^{s p k}
     int64_t s = 1;
@@ -275,6 +296,46 @@ int make_line(LinePlan& lp, int64_t N, Prec pr, int E, int max_threads) {
     lp.Ls = line_stride_for(pr, N);
     return upload(lp.dtw, tw, pr);
 }
+
+// Threads per line when its last pass runs across the lanes of a wavefront (wave_tail_pass in
+// engine.h), or 0 when the line does not allow it.  Conditions: the last radix RL and the
+// penultimate radix R are powers of two up to 16 (the kernels' wave-tail instances); the stride
+// s before the penultimate pass is a power of two with s RL <= 64, so a digit group (lanes s
+// apart) lies in one wavefront and xor on the lane index flips digit bits; every thread holds at
+// most one penultimate butterfly (b = t); and the thread count is a multiple of s RL, so the
+// groups of every line of a workgroup start on lane multiples of s RL.  The last pass then needs
+// no threads of its own, so the count is the largest of the other passes' (s RL <= 64 keeps the
+// lines to N <= 1024).
+static int wave_tail_threads(const LinePlan& lp) {
+    const PassTable& pt = lp.pt;
+    if (pt.npass < 2) return 0;
+    const int RL = pt.radix[pt.npass - 1], R = pt.radix[pt.npass - 2];
+    auto pow2 = [](int64_t v) { return v > 0 && (v & (v - 1)) == 0; };
+    if (!pow2(RL) || RL > 16 || !pow2(R) || R > 16) return 0;
+    int64_t s = 1;
+    for (int i = 0; i < pt.npass - 2; ++i) s *= pt.radix[i];
+    if (!pow2(s) || s * RL > 64) return 0;
+    int tt = lp.N / R;
+    for (int i = 0; i < pt.npass - 2; ++i) {
+        const int nb = lp.N / pt.radix[i];
+        const int mb = bfly_per_thread(lp.E, pt.radix[i]);
+        tt = std::max(tt, (nb + mb - 1) / mb);
+    }
+    if (tt % (s * RL)) return 0;
+    return tt;
+}
+
+bool apply_wave_tail(LinePlan& lp) {
+    const int tt = wave_tail_threads(lp);
+    if (tt <= 0) return false;
+    lp.Tt = tt;
+    lp.pt.wave = 1;
+    return true;
+}
+
+// Whether a direct or fused-convolution line runs its last pass across wavefront lanes by
+// default (PlanOptions::wave = -1).
+constexpr bool kWaveDefault = false;
 
 static int64_t esize(Prec p) { return p == Prec::F32 ? 8 : 16; }
 // Bytes of one element of the plan's input and output (ComplexF16: 4).
@@ -314,9 +375,12 @@ static int pipe_resident(Ctx* ctx, const Plan& p) {
 // for F32 lines with a radix above 9; lines of radices <= 9 run faster on the plain kernel's
 // radix-bounded instance, which fits more waves per SIMD).
 constexpr int kPipeAuto = 2;
-static int plan_direct(Ctx* ctx, Plan& p, int E, int target_threads, int pipe = -1) {
-    int rc = make_line(p.line, p.n, p.prec, E, 1024);
+static int plan_direct(Ctx* ctx, Plan& p, int E, int target_threads, int pipe = -1,
+                       const std::vector<int>* rad = nullptr, int wave = -1) {
+    int rc = make_line(p.line, p.n, p.prec, E, 1024, rad);
     if (rc) return rc;
+    // the wave-tail kernels are instantiated for float / double lines and the plain rows kernel
+    if (!p.half && (wave > 0 || (wave < 0 && kWaveDefault)) && apply_wave_tail(p.line)) pipe = 0;
     if (pipe == kPipeAuto) pipe = p.prec == Prec::F32 && pass_table_rmax(p.line.pt) > 9;
     const size_t per = 2 * (size_t)esize(p.prec) / 2 * p.line.Ls;
     if (per > ctx->lds_max) return fail(CFFT_ERR_UNSUPPORTED, "line does not fit LDS");
@@ -464,7 +528,9 @@ static int upload_tw3(FourStepPlan& fs, int64_t n, Prec pr) {
 static int plan_fft_core(Ctx* ctx, Plan& p, const PlanOptions& o);
 
 static int plan_fourstep_factors(Ctx* ctx, FourStepPlan& fs, int64_t n, Prec pr, int E,
-                                 int64_t n1_force, int ca, int cb, int64_t chunk) {
+                                 int64_t n1_force, int ca, int cb, int64_t chunk,
+                                 const std::vector<int>* rad_a = nullptr,
+                                 const std::vector<int>* rad_b = nullptr) {
     fs.chunk = chunk;
     int64_t best1 = 0;
     if (n1_force > 0) {
@@ -486,9 +552,9 @@ static int plan_fourstep_factors(Ctx* ctx, FourStepPlan& fs, int64_t n, Prec pr,
     if (best1 == 0) return fail(CFFT_ERR_UNSUPPORTED, "no four-step split for n=" + std::to_string(n));
     fs.N1 = (int)best1;
     fs.N2 = (int)(n / best1);
-    int rc = make_line(fs.a, fs.N1, pr, E, 1024);
+    int rc = make_line(fs.a, fs.N1, pr, E, 1024, rad_a);
     if (rc) return rc;
-    rc = make_line(fs.b, fs.N2, pr, E, 1024);
+    rc = make_line(fs.b, fs.N2, pr, E, 1024, rad_b);
     if (rc) return rc;
     set_tw3_shape(fs, n);
     const size_t es = (size_t)esize(pr);
@@ -693,6 +759,7 @@ static int plan_conv(Ctx* ctx, Plan& p, const PlanOptions& o) {
     if (p.cprec == p.prec && line_fits(ctx, p.L, p.prec, E)) {
         rc = make_line(p.line, p.L, p.prec, E, 1024);
         if (rc) return rc;
+        if (!p.half && (o.wave > 0 || (o.wave < 0 && kWaveDefault))) apply_wave_tail(p.line);
         // the input lines and the convolution work share one LDS line (the kernel saves x0 and
         // separates reads from writes where they overlap)
         p.LsC = std::max(line_stride_for(p.prec, n), p.line.Ls);
@@ -731,10 +798,27 @@ static int plan_conv(Ctx* ctx, Plan& p, const PlanOptions& o) {
     s.dtype = p.cprec == Prec::F32 ? CFFT_C32 : CFFT_C64;
     PlanOptions so = o;
     so.n1 = 0;
+    so.rad_a.clear();  // radix sequences and the wave tail were chosen for the transform itself
+    so.rad_b.clear();
+    so.wave = -1;
     rc = plan_fft_core(ctx, s, so);
     if (rc) return rc;
     if (s.kind != Kind::Direct && s.kind != Kind::FourStep && s.kind != Kind::Large)
         return fail(CFFT_ERR_UNSUPPORTED, "convolution length not plannable");
+    if (s.kind == Kind::FourStep && o.fuse_conv != 0) {
+        // three fused launches; the last one runs lines of N1 points over the second FFT's
+        // middle (rows of N2), like a pass B with the first line's plan
+        const size_t table = (size_t)tw3_entries(s.fs.tbits, s.fs.tnc) * esize(p.cprec);
+        const int cc = pick_tile(ctx, s.fs.a, p.cprec, 4, table);
+        const int lsc = fs_line_stride(ctx, p.cprec, s.fs.a, cc, table + 16);
+        const size_t es = (size_t)esize(p.cprec) / 2;
+        if (cc >= 1 && cc * s.fs.a.Tt <= 1024 &&
+            fsb_tw_offset(es, cc, lsc) + table <= ctx->lds_max) {
+            p.convl = true;
+            p.convl_cc = cc;
+            p.convl_lsc = lsc;
+        }
+    }
     if (!o.alloc_work) return CFFT_OK;  // the caller supplies per-transform work buffers
     const size_t wbytes = (size_t)p.batch * p.L * esize(p.cprec);
     if ((rc = p.w1.alloc(wbytes))) return rc;
@@ -779,20 +863,29 @@ static int plan_fft_core(Ctx* ctx, Plan& p, const PlanOptions& o) {
         // Direct lines default to E = 8 (more threads per line; measured faster than 16 on
         // MI355X for F32 and F64) when the line fits a workgroup that way, and F32 lines with a
         // radix above 9 to the pipelined rows kernel.
-        if (o.n1 == 0) {
+        // a forced radix sequence for the whole length makes a direct line; for a factor of n
+        // it is the four-step's first line and fixes the split
+        int64_t rad_prod = 1;
+        for (int R : o.rad_a) rad_prod *= R;
+        const bool rad_direct = !o.rad_a.empty() && rad_prod == p.n;
+        const int64_t n1 = o.n1 ? o.n1 : (!o.rad_a.empty() && !rad_direct ? rad_prod : 0);
+        if (n1 == 0) {
             const int Ed = o.E ? o.E : (line_fits(ctx, p.n, p.prec, 8) ? 8 : 16);
             const int pipe = o.pipe >= 0 ? o.pipe : (o.explicit_opts ? 0 : kPipeAuto);
-            if (line_fits(ctx, p.n, p.prec, Ed))
-                return plan_direct(ctx, p, Ed, o.target_threads, pipe);
+            if (line_fits(ctx, p.n, p.prec, Ed) || rad_direct)
+                return plan_direct(ctx, p, Ed, o.target_threads, pipe,
+                                   rad_direct ? &o.rad_a : nullptr, o.wave);
         }
         if (o.large) return plan_large(ctx, p, o);
         // Four-step lines default to E = 8 in F64 (twice the threads per tile), 16 in F32.
         const int64_t chunk = o.chunk >= 0 ? o.chunk : default_chunk(p);
+        const std::vector<int>* ra = o.rad_a.empty() ? nullptr : &o.rad_a;
+        const std::vector<int>* rb = o.rad_b.empty() ? nullptr : &o.rad_b;
         int rc = CFFT_ERR_UNSUPPORTED;
         if (!o.E && p.prec == Prec::F64)
-            rc = plan_fourstep_factors(ctx, p.fs, p.n, p.prec, 8, o.n1, o.ca, o.cb, chunk);
-        if (rc) rc = plan_fourstep_factors(ctx, p.fs, p.n, p.prec, E, o.n1, o.ca, o.cb, chunk);
-        if (rc && o.n1 == 0) {  // too long for the four-step: six steps
+            rc = plan_fourstep_factors(ctx, p.fs, p.n, p.prec, 8, n1, o.ca, o.cb, chunk, ra, rb);
+        if (rc) rc = plan_fourstep_factors(ctx, p.fs, p.n, p.prec, E, n1, o.ca, o.cb, chunk, ra, rb);
+        if (rc && n1 == 0) {  // too long for the four-step: six steps
             p.fs.b.dtw.reset();
             p.fs.tw3.reset();
             p.fs.CB = 1;
@@ -825,6 +918,7 @@ static std::string describe(const Plan& p) {
         case Kind::Trivial: os << "trivial"; break;
         case Kind::Direct:
             os << "direct " << rad(p.line) << " C=" << p.C;
+            if (p.line.pt.wave) os << " wave";
             if (p.pipe_res) os << " pipe";
             break;
         case Kind::FourStep:
@@ -842,12 +936,14 @@ static std::string describe(const Plan& p) {
         case Kind::ConvSmall:
             os << (p.mode == ConvMode::Rader ? "rader" : "bluestein") << "-fused L=" << p.L << " "
                << rad(p.line) << " C=" << p.C;
+            if (p.line.pt.wave) os << " wave";
             if (p.mode == ConvMode::Rader) os << " g=" << p.g;
             break;
         case Kind::ConvLarge:
             os << (p.mode == ConvMode::Rader ? "rader" : "bluestein") << "-large L=" << p.L
-               << " conv=" << (p.cprec == Prec::F32 ? "f32" : "f64") << " {" << describe(*p.sub)
-               << "}";
+               << " conv=" << (p.cprec == Prec::F32 ? "f32" : "f64");
+            if (p.convl) os << " fused3 CC=" << p.convl_cc;
+            os << " {" << describe(*p.sub) << "}";
             if (p.mode == ConvMode::Rader) os << " g=" << p.g;
             break;
         default: os << "uneven"; break;
@@ -1102,6 +1198,47 @@ static int run_fft_s(const Plan& p, const void* in, void* out, int sgn, double s
             return CFFT_OK;
         }
         case Kind::ConvLarge: {
+            void* w1 = sh.w1 ? sh.w1 : p.w1.p;
+            void* w2 = sh.w1 ? sh.w2 : p.w2.p;
+            void* dc = sh.w1 ? sh.dc : p.dc.p;
+            if (!w1 || !w2 || !dc) return fail(CFFT_ERR_INVALID, "no convolution work buffers");
+            if (p.convl) {
+                const Plan& s = *p.sub;
+                const FourStepPlan& f = s.fs;
+                ConvLArgs a{};
+                a.n = p.n;
+                a.N1 = f.N1;
+                a.N2 = f.N2;
+                a.CA = f.CA;
+                a.CB = f.CB;
+                a.CC = p.convl_cc;
+                a.TtA = f.a.Tt;
+                a.LsA = f.a.Ls;
+                a.TtB = f.b.Tt;
+                a.LsB = f.b.Ls;
+                a.TtC = f.a.Tt;
+                a.LsC = p.convl_lsc;
+                a.E = f.a.E;
+                a.nlines = B;
+                a.idist = p.idist;
+                a.odist = p.odist;
+                a.mode = (int)p.mode;
+                a.sgn = sgn;
+                a.scale = scale;
+                a.perm = (const int*)p.perm.p;
+                a.iperm = (const int*)p.iperm.p;
+                a.chirp = p.chirp.p;
+                a.K = p.K.p;
+                a.m1 = w1;
+                a.m2 = w2;
+                a.dc = dc;
+                a.tw3 = f.tw3.p;
+                a.tbits = f.tbits;
+                a.tnc = f.tnc;
+                CFFT_HIP(launch_convl(p.prec, p.cprec, in, out, a, f.a.pt, f.a.dtw.p, f.b.pt,
+                                      f.b.dtw.p, st));
+                return CFFT_OK;
+            }
             ConvGatherArgs g{};
             g.n = p.n;
             g.L = p.L;
@@ -1112,10 +1249,6 @@ static int run_fft_s(const Plan& p, const void* in, void* out, int sgn, double s
             g.sgn_in = sgn;
             g.perm = (const int*)p.perm.p;
             g.chirp = p.chirp.p;
-            void* w1 = sh.w1 ? sh.w1 : p.w1.p;
-            void* w2 = sh.w1 ? sh.w2 : p.w2.p;
-            void* dc = sh.w1 ? sh.dc : p.dc.p;
-            if (!w1 || !w2 || !dc) return fail(CFFT_ERR_INVALID, "no convolution work buffers");
             g.dc = dc;
             CFFT_HIP(launch_conv_gather(p.prec, p.cprec, in, w1, g, st));
             ExecShape ss;

@@ -88,6 +88,10 @@ struct Plan {
     Prec cprec = Prec::F32;
     DevBuf perm, iperm, chirp, K, dc, w1, w2;
     std::unique_ptr<Plan> sub;  // large convolution: FFT_L; six-step: the M-point row FFTs
+    // large convolution whose FFT_L is a four-step, run as three fused launches: the tile and
+    // LDS line stride of the last launch (lines of N1 points over the second FFT's middle)
+    bool convl = false;
+    int convl_cc = 0, convl_lsc = 0;
     // real-input transforms: 1 = fused (n even, n/2 fits a workgroup), 2 = even via a complex
     // sub-plan of n/2, 3 = odd via a complex sub-plan of n; idist / odist hold rdist (reals) and
     // cdist (complex elements)
@@ -123,6 +127,16 @@ struct PlanOptions {
     int pipe = -1;             // direct: pipelined rows kernel (0 = off, 1 = on, -1 = default)
     int large = 0;             // 1: the six-step path with N1 = n1 even where the four-step fits
     int force_conv = 0;
+    // Radix sequences (the reference's MEASURE searches them, radix_plan.jl:23-68): rad_a for a
+    // direct line or the four-step's first line (its product then sets the split), rad_b for the
+    // four-step's second line; empty = the default sequence (line_radices).
+    std::vector<int> rad_a, rad_b;
+    // Direct and fused-convolution lines: 1 = the last pass runs across the lanes of a wavefront
+    // where the line allows it (wave_tail_threads), 0 = through LDS, -1 = the default.
+    int wave = -1;
+    // Large convolutions over a four-step FFT_L: 1 = the three fused launches (k_convl_*), 0 =
+    // gather, two four-steps and scatter, -1 = the default (fused).
+    int fuse_conv = -1;
     bool explicit_opts = false;
     bool alloc_work = true;    // large convolutions: allocate work buffers for `batch` lines
 };
@@ -182,7 +196,13 @@ int64_t next_smooth(int64_t m);
 std::vector<int> line_radices(int64_t n);
 int threads_for(const std::vector<int>& rad, int64_t N, int E);
 int line_stride_for(Prec p, int64_t N);
-int make_line(LinePlan& lp, int64_t N, Prec pr, int E, int max_threads);
+int make_line(LinePlan& lp, int64_t N, Prec pr, int E, int max_threads,
+              const std::vector<int>* radices = nullptr);
+// Whether rad is a radix sequence the kernels implement whose product is N.
+bool radices_valid(const std::vector<int>& rad, int64_t N);
+// Switches the line to a last pass across wavefront lanes when it allows it (returns whether it
+// did; the line's thread count may shrink).
+bool apply_wave_tail(LinePlan& lp);
 
 // Plans p (fields n, batch, idist, odist, prec already set) with the given knobs.
 int plan_build(Ctx* ctx, Plan& p, const PlanOptions& o);

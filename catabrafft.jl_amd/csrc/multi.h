@@ -21,7 +21,9 @@ struct MLine {
 };
 
 struct MPlan {
-    int n, C, pad0, pad1;       // transform length, lines per workgroup (rows / fused conv)
+    int n, C;                   // transform length, lines per workgroup (rows / fused conv)
+    int CC, LsC;                // large convolution in fused launches: the last launch's tile
+                                // and LDS line stride (its FFT_L four-step fields below)
     MLine line;                 // rows: the transform; fused conv and conv sub-FFTs: length L
     // four-step pass A (la, N1, column tile CA) and pass B (lb, N2, row tile CB)
     int N1, N2, CA, CB, tbits, tnc;
@@ -79,7 +81,10 @@ struct MultiLaunch {
     const MPlan* plans;  // device table
 };
 
-enum class MultiKernel : int { Rows = 0, FourStepA, FourStepB, ConvSmall, Gather, Scatter };
+enum class MultiKernel : int {
+    Rows = 0, FourStepA, FourStepB, ConvSmall, Gather, Scatter,
+    ConvLA, ConvLBA, ConvLC  // the three fused launches of a large convolution
+};
 
 // Launch-wide arguments of a work-list launch.
 struct MultiArgs {

@@ -123,6 +123,20 @@ PlanEntry WorkList::add_plan(const Plan& p) {
             m.iperm = (const int*)p.iperm.p;
             m.chirp = p.chirp.p;
             m.K = p.K.p;
+            if (p.kind == Kind::ConvLarge && p.convl) {  // its FFT_L four-step, for the fused launches
+                const FourStepPlan& f = p.sub->fs;
+                m.N1 = f.N1;
+                m.N2 = f.N2;
+                m.CA = f.CA;
+                m.CB = f.CB;
+                m.CC = p.convl_cc;
+                m.LsC = p.convl_lsc;
+                m.tbits = f.tbits;
+                m.tnc = f.tnc;
+                m.la = make_mline(f.a);
+                m.lb = make_mline(f.b);
+                m.tw3 = f.tw3.p;
+            }
             break;
         default:
             break;
@@ -266,6 +280,51 @@ void WorkList::add_transforms(const PlanEntry& e, const std::vector<XformRef>& x
         case Kind::ConvLarge: {
             const Plan& s = *p.sub;
             const Prec cp = p.cprec;
+            if (p.convl) {
+                // three fused launches (phases 1-3), one item per tile of each transform
+                const FourStepPlan& f = s.fs;
+                const size_t ce = esz(cp) / 2;
+                const size_t table = (size_t)tw3_entries(f.tbits, f.tnc) * esz(cp);
+                const size_t lda = 2 * ce * (size_t)f.CA * f.a.Ls;
+                const size_t ldb = fsb_tw_offset(ce, f.CB, f.b.Ls) + table;
+                const size_t ldc = fsb_tw_offset(ce, p.convl_cc, p.convl_lsc) + table;
+                const int ta = (f.N2 + f.CA - 1) / f.CA, tb = (f.N1 + f.CB - 1) / f.CB;
+                const int tc = (f.N2 + p.convl_cc - 1) / p.convl_cc;
+                const double la = std::log2((double)f.N1), lb = std::log2((double)f.N2);
+                for (const XformRef& x : xs) {
+                    MItem it{};
+                    it.plan = e.mplan;
+                    it.dc = x.dc;
+                    it.sgn = x.sgn;
+                    it.scale = x.scale;
+                    const int dirf = x.launch_dir ? kLaunchDir : 0;
+                    for (int t = 0; t < ta; ++t) {
+                        it.count = t;
+                        it.in = x.in;
+                        it.out = x.w1;
+                        it.flags = x.in_sel | (kAbs << kSelOutShift) | dirf;
+                        add_item(1, MultiKernel::ConvLA, pr, cp, f.CA * f.a.Tt, lda, it,
+                                 (double)f.CA * f.N1 * la);
+                    }
+                    for (int t = 0; t < tb; ++t) {
+                        it.count = t;
+                        it.in = x.w1;
+                        it.out = x.w2;
+                        it.flags = kAbs | (kAbs << kSelOutShift) | dirf;
+                        add_item(2, MultiKernel::ConvLBA, pr, cp, f.CB * f.b.Tt, ldb, it,
+                                 2.0 * f.CB * f.N2 * lb);
+                    }
+                    for (int t = 0; t < tc; ++t) {
+                        it.count = t;
+                        it.in = x.w2;
+                        it.out = x.out;
+                        it.flags = kAbs | (x.out_sel << kSelOutShift) | dirf;
+                        add_item(3, MultiKernel::ConvLC, pr, cp, p.convl_cc * f.a.Tt, ldc, it,
+                                 (double)p.convl_cc * f.N1 * la);
+                    }
+                }
+                break;
+            }
             const int64_t gchunks = (p.L + kChunkHost - 1) / kChunkHost;
             const int64_t schunks = (p.n + kChunkHost - 1) / kChunkHost;
             for (const XformRef& x : xs) {
@@ -352,9 +411,59 @@ int WorkList::finalize(DevBuf& dplans, DevBuf& ditems, DevBuf& dtable,
                        HostStage* stage) {
     int rc;
     out.clear();
+    // Groups of one phase, kernel and precision pair differ only in workgroup size and LDS.  A
+    // group too small to fill the GPU is tail-bound as a launch of its own: it is folded into the
+    // next group of the family (ascending block, then LDS), whose launch then takes the larger
+    // block and LDS of the two (threads beyond an item's lines idle, or repeat a butterfly of
+    // their line with the same values).
+    std::vector<Group> merged;
+    {
+        std::vector<const Group*> fam;
+        for (auto& kv : groups_) fam.push_back(&kv.second);
+        std::stable_sort(fam.begin(), fam.end(), [](const Group* a, const Group* b) {
+            return std::tie(a->phase, a->kernel, a->prec, a->cprec, a->block, a->lds) <
+                   std::tie(b->phase, b->kernel, b->prec, b->cprec, b->block, b->lds);
+        });
+        constexpr size_t kMinItems = 2048;
+        for (size_t i = 0; i < fam.size(); ++i) {
+            const Group& g = *fam[i];
+            const bool same_family =
+                !merged.empty() && merged.back().phase == g.phase &&
+                merged.back().kernel == g.kernel && merged.back().prec == g.prec &&
+                merged.back().cprec == g.cprec;
+            if (same_family && merged.back().items.size() < kMinItems) {
+                Group& m = merged.back();
+                const size_t off = m.items.size();
+                m.items.insert(m.items.end(), g.items.begin(), g.items.end());
+                for (const auto& r : g.runs) m.runs.emplace_back(r.first, r.second + off);
+                m.block = std::max(m.block, g.block);
+                m.lds = std::max(m.lds, g.lds);
+                continue;
+            }
+            merged.push_back(g);
+        }
+        // a small last group of a family joins the group before it
+        for (size_t i = merged.size(); i-- > 1;) {
+            Group& g = merged[i];
+            Group& prev = merged[i - 1];
+            const bool last_of_family =
+                i + 1 == merged.size() || merged[i + 1].phase != g.phase ||
+                merged[i + 1].kernel != g.kernel || merged[i + 1].prec != g.prec ||
+                merged[i + 1].cprec != g.cprec;
+            if (!last_of_family || g.items.size() >= kMinItems || prev.phase != g.phase ||
+                prev.kernel != g.kernel || prev.prec != g.prec || prev.cprec != g.cprec)
+                continue;
+            const size_t off = prev.items.size();
+            prev.items.insert(prev.items.end(), g.items.begin(), g.items.end());
+            for (const auto& r : g.runs) prev.runs.emplace_back(r.first, r.second + off);
+            prev.block = std::max(prev.block, g.block);
+            prev.lds = std::max(prev.lds, g.lds);
+            merged.erase(merged.begin() + (std::ptrdiff_t)i);
+        }
+    }
     // order: phase, then kernel; inside a group, most expensive items first
     std::vector<const Group*> gs;
-    for (auto& kv : groups_) gs.push_back(&kv.second);
+    for (const Group& g : merged) gs.push_back(&g);
     std::stable_sort(gs.begin(), gs.end(), [](const Group* a, const Group* b) {
         return std::tie(a->phase, a->kernel) < std::tie(b->phase, b->kernel);
     });

@@ -240,6 +240,42 @@ def emit(R: int) -> str:
     return "\n".join(L)
 
 
+def emit_lane_w16() -> str:
+    """w_16^m = e^{-2 pi i m / 16} for a lane-dependent m (0..15): the stage twiddles of the
+    cross-lane radix-2 stages (engine.h, wave_tail_pass), by selects over three constants."""
+    c, s = cos_sin(1, 16)
+    r = mpmath.sqrt(mpmath.mpf(2)) / 2
+    return "\n".join([
+        "// w_16^m = e^{-2 pi i m/16} for a lane-dependent m in 0..15: w^(m & 3) from three",
+        "// constants by selects, then times (-i)^(m >> 2).",
+        "template <typename V>",
+        "__device__ __forceinline__ V lane_w16(int m) {",
+        "    using T = decltype(V{}.x);",
+        f"    const T c = T({_const(c)}), s = T({_const(s)}), r = T({_const(r)});",
+        "    const int q = m & 3;",
+        "    const T x = q == 0 ? T(1) : (q == 1 ? c : (q == 2 ? r : s));",
+        "    const T y = q == 0 ? T(0) : (q == 1 ? -s : (q == 2 ? -r : -c));",
+        "    const bool a1 = (m & 4) != 0, a2 = (m & 8) != 0;",
+        "    const T x1 = a1 ? y : x, y1 = a1 ? -x : y;",
+        "    return V{a2 ? -x1 : x1, a2 ? -y1 : y1};",
+        "}",
+    ])
+
+
+def check_lane_w16():
+    """The select formula of lane_w16 equals e^{-2 pi i m / 16} for every m."""
+    c, s = (float(v) for v in cos_sin(1, 16))
+    r = float(mpmath.sqrt(2) / 2)
+    for m in range(16):
+        q = m & 3
+        x = [1.0, c, r, s][q]
+        y = [0.0, -s, -r, -c][q]
+        x1, y1 = (y, -x) if m & 4 else (x, y)
+        got = complex(-x1, -y1) if m & 8 else complex(x1, y1)
+        want = complex(mpmath.cos(2 * mpmath.pi * m / 16), -mpmath.sin(2 * mpmath.pi * m / 16))
+        assert abs(got - want) < 1e-15, (m, got, want)
+
+
 def check_mul_diag():
     """The emitted mul_diag formula equals a * (cr + i ci) for the four diagonal constants."""
     for cr, ci in [(1, 1), (1, -1), (-1, 1), (-1, -1)]:
@@ -260,6 +296,7 @@ def main():
     out = sys.argv[1] if len(sys.argv) > 1 else os.path.join(here, "..", "csrc", "codelets.h")
     self_check()
     check_mul_diag()
+    check_lane_w16()
     parts = [
         "// codelets.h -- GENERATED by gen/gen_codelets.py; do not edit by hand.",
         "// Straight-line forward DFT_R butterflies for R in {" + ", ".join(map(str, RADICES)) + "}.",
@@ -276,6 +313,8 @@ def main():
         parts.append(f"template <> struct Dft<{R}> {{ template <typename V> "
                      f"__device__ __forceinline__ static void run(V* v) {{ dft{R}(v); }} }};")
     parts.append("constexpr int kRadices[] = {" + ", ".join(map(str, RADICES)) + "};")
+    parts.append("")
+    parts.append(emit_lane_w16())
     parts.append("}  // namespace cfft")
     with open(out, "w") as f:
         f.write("\n".join(parts) + "\n")

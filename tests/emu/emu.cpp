@@ -100,6 +100,23 @@ void launch(std::function<void()> body, dim3 grid, dim3 block, size_t lds, Strea
 
 }  // namespace emu
 
+int cfft_emu_lane_xor(int v, int d) {
+    static thread_local std::vector<int> buf;
+    const unsigned n = emu::block_dim().x * emu::block_dim().y * emu::block_dim().z;
+    if (buf.size() < n) buf.resize(n);
+    const unsigned i = emu::thread_idx().x;
+    const unsigned j = i ^ (unsigned)d;
+    if (j >= n || (i >> 6) != (j >> 6)) {
+        std::fprintf(stderr, "emu: lane exchange %u ^ %d leaves the wavefront\n", i, d);
+        std::abort();
+    }
+    buf[i] = v;
+    emu::barrier();
+    const int r = buf[j];
+    emu::barrier();
+    return r;
+}
+
 struct emu_event {
     double t = 0;
 };

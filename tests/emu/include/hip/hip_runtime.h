@@ -20,6 +20,11 @@
 #define CFFT_OPAQUE(x) asm volatile("" : "+r"(x))
 // Non-temporal (streaming) global accesses are a cache hint on the device; here they are plain.
 #define CFFT_PLAIN_STREAMING 1
+// Cross-lane exchanges (the device's DPP / permlane moves within a wavefront): the value of the
+// thread whose index differs by xor d, traded through a buffer between two barriers of the block
+// (every thread of the block calls it the same number of times).
+#define CFFT_EMU_LANES 1
+int cfft_emu_lane_xor(int v, int d);
 
 struct float2 {
     float x, y;

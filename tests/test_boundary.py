@@ -85,6 +85,7 @@ def test_generated_codelets_are_current(tmp_path):
 
     G.self_check()
     G.check_mul_diag()
+    G.check_lane_w16()
     tmp = str(tmp_path / "codelets_check.h")
     subprocess.run([sys.executable, os.path.join(PKG, "gen", "gen_codelets.py"), tmp], check=True,
                    capture_output=True)

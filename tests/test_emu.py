@@ -546,3 +546,167 @@ def test_emulated_exec_noalloc_matches_exec(E):
     assert lib.cfft_exec_noalloc(rl, 16, 32, -1, 0, None) == -4
     for h in (big, rl):
         lib.cfft_plan_destroy(h)
+
+
+@pytest.mark.parametrize("n,batch,dt,fields,expect", [
+    (2520, 3, np.complex64, "ra=5,7,8,9", "[5,7,8,9]"),
+    (2520, 2, np.complex128, "ra=8,9,7,5", "[8,9,7,5]"),
+    (2520, 2, np.complex64, "ra=3,3,2,4,7,5", "[3,3,2,4,7,5]"),
+    (1024, 4, np.complex64, "ra=4,16,16", "[4,16,16]"),
+    (1024, 2, np.complex128, "ra=8,8,16", "[8,8,16]"),
+    (1 << 16, 1, np.complex64, "ra=4,16,4 rb=16,4,4", "N1=256 [4,16,4]"),
+    (1 << 16, 2, np.complex128, "ra=2,8,16 rb=8,8,4", "[8,8,4]"),
+    (3 ** 10, 1, np.complex64, "ra=3,3,3,9 rb=9,9,3", "[3,3,3,9]")])
+def test_emulated_forced_radix_sequences(E, tmp_path, n, batch, dt, fields, expect):
+    """Radix sequences other than the default (the search space of MEASURE / ENCHANT, the
+    reference's radix_plan.jl:23-68), forced through wisdom: the plan runs them (its description
+    shows them), the result matches the oracle forward and backward, and the wisdom file
+    round-trips the sequences."""
+    lib, _ = E
+    dcode = 0 if dt == np.complex64 else 1
+    path = tmp_path / "w.txt"
+    path.write_text(f"{n} {batch} {n} {n} {dcode} 1 : 0 64 0 0 0 -1 0 0 {fields}\n")
+    c2 = ctypes.c_void_p()
+    assert lib.cfft_init(0, ctypes.byref(c2)) == 0
+    assert lib.cfft_wisdom_import(c2, str(path).encode()) == 0, lib.cfft_last_error()
+    p = _plan(lib, c2, n, batch, dt, flags=1)
+    d = _desc(lib, p)
+    assert "(wisdom)" in d and expect in d, d
+    x = seeded_complex(n, batch=batch, seed=11, dtype=dt)
+    for direction, norm in ((-1, 0), (1, 1)):
+        y = np.zeros_like(x)
+        assert lib.cfft_exec(p, x.ctypes.data, y.ctypes.data, direction, norm, None) == 0
+        ref = O.fft(x.astype(np.complex128), "d", direction, bool(norm))
+        assert rel_l2(y, ref) < _tol(n, dt), (direction, norm)
+    out = tmp_path / "out.txt"
+    assert lib.cfft_wisdom_export(c2, str(out).encode()) == 0
+    assert fields in out.read_text(), out.read_text()
+    lib.cfft_plan_destroy(p)
+    lib.cfft_finalize(c2)
+
+
+@pytest.mark.parametrize("line", [
+    "2520 2 2520 2520 0 1 : 0 64 0 0 0 -1 0 0 ra=9,8,7,6",      # radix 6 is not a codelet
+    "2520 2 2520 2520 0 1 : 0 64 0 0 0 -1 0 0 ra=9,8,7,",       # truncated list
+    "2520 2 2520 2520 0 1 : 0 64 0 0 0 -1 0 0 ra=9,8,7,17"])
+def test_emulated_wisdom_rejects_bad_radix_lists(E, tmp_path, line):
+    lib, _ = E
+    path = tmp_path / "bad.txt"
+    path.write_text(line + "\n")
+    c2 = ctypes.c_void_p()
+    assert lib.cfft_init(0, ctypes.byref(c2)) == 0
+    rc = lib.cfft_wisdom_import(c2, str(path).encode())
+    if rc == 0:  # a list of known radices whose product is not n: the plan falls back
+        p = _plan(lib, c2, 2520, 2, np.complex64, flags=1)
+        assert "(wisdom)" not in _desc(lib, p)
+        lib.cfft_plan_destroy(p)
+    else:
+        assert rc == -1
+    lib.cfft_finalize(c2)
+
+
+def test_emulated_measure_searches_radix_sequences(E):
+    """MEASURE's candidate set includes radix sequences other than the default: the tuned plan's
+    description counts them, and whatever wins matches the oracle."""
+    lib, ctx = E
+    p = _plan(lib, ctx, 1024, 8, np.complex64, flags=1)
+    d = _desc(lib, p)
+    assert "tuned(measure" in d, d
+    ncand = int(d.split("tuned(measure, ")[1].split(" candidates")[0])
+    assert ncand >= 4 * 2 + 5 * 2, d  # shapes x pipe, plus radix sequences x threads
+    x = seeded_complex(1024, batch=8, seed=12, dtype=np.complex64)
+    y = np.zeros_like(x)
+    assert lib.cfft_exec(p, x.ctypes.data, y.ctypes.data, -1, 0, None) == 0
+    assert rel_l2(y, O.fft(x.astype(np.complex128), "d")) < 1e-6
+    lib.cfft_plan_destroy(p)
+
+
+@pytest.mark.parametrize("n,batch,dt,fields", [
+    (64, 37, np.complex64, "16 64 0 0 0 -1 0 0 wv=1"),
+    (128, 9, np.complex128, "16 64 0 0 0 -1 0 0 wv=1"),
+    (256, 5, np.complex64, "16 256 0 0 0 -1 0 0 wv=1"),
+    (512, 3, np.complex64, "8 64 0 0 0 -1 0 0 wv=1"),
+    (1024, 3, np.complex64, "16 64 0 0 0 -1 0 0 wv=1"),
+    (1024, 2, np.complex128, "8 64 0 0 0 -1 0 0 wv=1"),
+    (1024, 2, np.complex64, "16 64 0 0 0 -1 0 0 ra=4,16,16 wv=1"),
+    (1024, 2, np.complex64, "16 64 0 0 0 -1 0 0 ra=8,16,8 wv=1"),
+    (257, 6, np.complex64, "16 64 0 0 0 -1 0 0 wv=1"),
+    (11 * 23, 3, np.complex128, "16 64 0 0 0 -1 0 0 wv=1")])
+def test_emulated_wave_tail_matches_oracle(E, tmp_path, n, batch, dt, fields):
+    """Lines whose last pass runs across wavefront lanes (the cross-lane exchanges emulated):
+    direct lines of 64..1024 points, several lines per workgroup, other radix orders, and the
+    fused Rader / Bluestein convolutions (L = 256 / 512); forward and backward, in place, against
+    the oracle.  The plan describes itself with "wave"."""
+    lib, _ = E
+    dcode = 0 if dt == np.complex64 else 1
+    path = tmp_path / "w.txt"
+    path.write_text(f"{n} {batch} {n} {n} {dcode} 1 : {fields}\n")
+    c2 = ctypes.c_void_p()
+    assert lib.cfft_init(0, ctypes.byref(c2)) == 0
+    assert lib.cfft_wisdom_import(c2, str(path).encode()) == 0, lib.cfft_last_error()
+    p = _plan(lib, c2, n, batch, dt, flags=1)
+    d = _desc(lib, p)
+    assert "(wisdom)" in d and " wave" in d, d
+    x = seeded_complex(n, batch=batch, seed=21, dtype=dt)
+    for direction, norm in ((-1, 0), (1, 1), (1, 0)):
+        y = np.zeros_like(x)
+        assert lib.cfft_exec(p, x.ctypes.data, y.ctypes.data, direction, norm, None) == 0
+        ref = O.fft(x.astype(np.complex128), "d", direction, bool(norm))
+        assert rel_l2(y, ref) < _tol(n, dt), (direction, norm)
+    z = x.copy()
+    assert lib.cfft_exec(p, z.ctypes.data, z.ctypes.data, -1, 0, None) == 0
+    assert rel_l2(z, O.fft(x.astype(np.complex128), "d")) < _tol(n, dt)
+    lib.cfft_plan_destroy(p)
+    lib.cfft_finalize(c2)
+
+
+def test_emulated_wave_tail_refused_where_lanes_do_not_hold_the_digit(E, tmp_path):
+    """A line whose last digit spans more than one wavefront (2048 = [16,16,8]: lanes 16 apart,
+    8 of them) or whose last radix is not a power of two (2520) keeps its LDS last pass."""
+    lib, _ = E
+    for n in (2048, 2520):
+        path = tmp_path / "w.txt"
+        path.write_text(f"{n} 2 {n} {n} 0 1 : 16 64 0 0 0 -1 0 0 wv=1\n")
+        c2 = ctypes.c_void_p()
+        assert lib.cfft_init(0, ctypes.byref(c2)) == 0
+        assert lib.cfft_wisdom_import(c2, str(path).encode()) == 0
+        p = _plan(lib, c2, n, 2, np.complex64, flags=1)
+        assert " wave" not in _desc(lib, p), _desc(lib, p)
+        lib.cfft_plan_destroy(p)
+        lib.cfft_finalize(c2)
+
+
+@pytest.mark.parametrize("n,batch,dt,kind", [(65537, 2, np.complex64, "rader-large"),
+                                             (65537, 1, np.complex128, "rader-large"),
+                                             (3 * 8191, 2, np.complex64, "bluestein-large"),
+                                             (100003, 1, np.complex128, "rader-large"),
+                                             (40009, 2, np.complex64, "rader-large"),
+                                             (17 * 2357, 2, np.complex64, "bluestein-large")])
+def test_emulated_large_convolution_fused_and_unfused(E, tmp_path, n, batch, dt, kind):
+    """A large convolution over a four-step FFT_L runs as three fused launches by default (gather
+    fused into the first pass A, the first pass B, the spectrum product and the second pass A in
+    one workgroup, the second pass B fused with the scatter); forced through wisdom, the four-launch
+    path (gather, two four-steps, scatter) is kept.  Both against the oracle, forward and
+    backward, in place."""
+    lib, _ = E
+    dcode = 0 if dt == np.complex64 else 1
+    x = seeded_complex(n, batch=batch, seed=31, dtype=dt)
+    for fc in (1, 0):
+        path = tmp_path / f"w{fc}.txt"
+        path.write_text(f"{n} {batch} {n} {n} {dcode} 1 : 0 64 0 0 0 -1 -1 0 fc={fc}\n")
+        c2 = ctypes.c_void_p()
+        assert lib.cfft_init(0, ctypes.byref(c2)) == 0
+        assert lib.cfft_wisdom_import(c2, str(path).encode()) == 0
+        p = _plan(lib, c2, n, batch, dt, flags=1)
+        d = _desc(lib, p)
+        assert kind in d and ("fused3" in d) == bool(fc), d
+        for direction, norm in ((-1, 0), (1, 1)):
+            y = np.zeros_like(x)
+            assert lib.cfft_exec(p, x.ctypes.data, y.ctypes.data, direction, norm, None) == 0
+            ref = O.fft(x.astype(np.complex128), "d", direction, bool(norm))
+            assert rel_l2(y, ref) < (1e-6 if dt == np.complex64 else 1e-13), (fc, direction)
+        z = x.copy()
+        assert lib.cfft_exec(p, z.ctypes.data, z.ctypes.data, -1, 0, None) == 0
+        assert rel_l2(z, O.fft(x.astype(np.complex128), "d")) < (1e-6 if dt == np.complex64 else 1e-13)
+        lib.cfft_plan_destroy(p)
+        lib.cfft_finalize(c2)

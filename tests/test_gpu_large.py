@@ -95,3 +95,4 @@ def test_convolution_over_sixstep_impulse_and_round_trip(gpu_ready, n, dt, kind)
     err = (torch.linalg.vector_norm((z - x).to(torch.complex128)) /
            torch.linalg.vector_norm(x.to(torch.complex128))).item()
     assert err < _tol(dt, n), err
+

@@ -259,3 +259,30 @@ def test_one_plan_on_two_streams(gpu_ready, n, batch, dt, inplace, code):
     for y, r in zip(outs, refs):
         assert torch.equal(y.cpu().view(torch.int16) if code == 2 else y.cpu(), 
                            r.view(torch.int16) if code == 2 else r)
+
+
+@pytest.mark.parametrize("n,batch,dt,kind", [(65537, 4, np.complex64, "rader-large"),
+                                             (1048573, 1, np.complex64, "rader-large"),
+                                             (65537, 2, np.complex128, "rader-large"),
+                                             (3 * 8191, 3, np.complex64, "bluestein-large"),
+                                             (40009, 2, np.complex64, "rader-large"),
+                                             (17 * 2357, 2, np.complex128, "bluestein-large")])
+def test_large_convolution_fused_and_unfused(gpu_ready, tmp_path, n, batch, dt, kind):
+    """Large Rader / Bluestein convolutions: the three fused launches (default) and the
+    four-launch path (gather, two four-steps, scatter; forced through wisdom) against the oracle,
+    forward and backward normalised (prime.jl:5-82)."""
+    import catabrafft as C
+
+    code = 0 if dt == np.complex64 else 1
+    x = seeded_complex(n, batch, seed=n % 101, dtype=dt)
+    for fc in (1, 0):
+        w = tmp_path / f"w{fc}.txt"
+        w.write_text(f"{n} {batch} {n} {n} {code} 1 : 0 64 0 0 0 -1 -1 0 fc={fc}\n")
+        C.import_wisdom(str(w))
+        p = C.Plan(n, batch, dt, flags=C.MEASURE)
+        d = p.describe()
+        print(f"large conv fc={fc}: {d}", flush=True)
+        assert kind in d and ("fused3" in d) == bool(fc), d
+        for direction, norm in ((-1, False), (1, True)):
+            y = run_dev(lambda xd: p.exec_torch(xd, direction=direction, normalize=norm), x)
+            assert rel_l2(y, oracle_ref(x, direction, norm)) <= max(tol(n, dt), 1e-13), (fc, direction)

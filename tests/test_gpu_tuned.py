@@ -3,9 +3,9 @@ the autotuner can pick, forced through wisdom.
 
 The reference's MEASURE / ENCHANT planner (radix_plan.jl:23-68, radix_exec.jl:140-297) times
 candidate radix decompositions and keeps the fastest; each candidate computes the same DFT.
-Here the candidates are kernel configurations of one transform (threads per line, per-thread
-capacity E, the four-step split, its tile widths, chunking, the pipelined kernels, the six-step
-path), so whichever one the timing picks must match the oracle to the same tolerance as the
+Here the candidates are the radix sequences of each line and the kernel configurations of one
+transform (threads per line, per-thread capacity E, the four-step split, its tile widths,
+chunking, the pipelined kernels, the six-step path), so whichever one the timing picks must match the oracle to the same tolerance as the
 default plan:  ComplexF64 4 * eps64 * log2(n), ComplexF32 1e-6 (relative L2).
 """
 import numpy as np
@@ -16,8 +16,8 @@ from test_gpu_parity import oracle_ref, run_dev, tol
 
 pytestmark = pytest.mark.gpu
 
-# wisdom fields after the key: E target_threads n1 ca cb chunk pipe large
-# (0 = the planner's default for n1 / ca / cb, chunk -1 = default)
+# wisdom fields after the key: E target_threads n1 ca cb chunk pipe large [ra=...] [rb=...]
+# (0 = the planner's default for n1 / ca / cb, chunk -1 = default; ra / rb = radix sequences)
 FORCED = [
     # n, batch, dtype, fields, what the description must show
     (1 << 16, 6, np.complex64, "16 64 256 2 32 -1 0 0", ["N1=256", "CA=2", "CB=32"]),
@@ -28,6 +28,19 @@ FORCED = [
     (2520, 7, np.complex128, "8 128 40 16 16 -1 0 0", ["N1=40", "CA=16", "CB=16"]),
     (2520, 7, np.complex64, "16 64 0 0 0 -1 1 0", ["direct", "pipe"]),
     (1000, 9, np.complex64, "8 512 0 0 0 -1 0 0", ["direct"]),
+    # radix sequences other than the default (the reference's MEASURE search space)
+    (2520, 64, np.complex64, "8 64 0 0 0 -1 0 0 ra=5,7,8,9", ["direct [5,7,8,9]"]),
+    (2520, 64, np.complex128, "8 64 0 0 0 -1 0 0 ra=8,9,7,5", ["direct [8,9,7,5]"]),
+    (2520, 33, np.complex64, "16 256 0 0 0 -1 0 0 ra=3,3,8,7,5", ["[3,3,8,7,5]"]),
+    (1024, 40, np.complex64, "16 64 0 0 0 -1 0 0 ra=4,16,16", ["direct [4,16,16]"]),
+    (1024, 40, np.complex128, "8 64 0 0 0 -1 0 0 ra=8,8,16", ["direct [8,8,16]"]),
+    (1024, 40, np.complex64, "16 64 0 0 0 -1 0 0 ra=16,4,16", ["[16,4,16]"]),
+    (1 << 20, 2, np.complex64, "16 64 0 0 0 0 0 0 ra=8,8,16 rb=4,16,16",
+     ["N1=1024 [8,8,16]", "[4,16,16]"]),
+    (1 << 20, 2, np.complex128, "8 64 0 0 0 0 0 0 ra=16,4,16 rb=8,16,8",
+     ["N1=1024 [16,4,16]", "[8,16,8]"]),
+    (3 ** 12, 2, np.complex64, "16 64 0 0 0 0 0 0 ra=3,3,3,3,9 rb=3,3,9,9",
+     ["N1=729 [3,3,3,3,9]", "[3,3,9,9]"]),
 ]
 
 
@@ -58,6 +71,8 @@ def test_forced_configuration_matches_oracle(gpu_ready, tmp_path, n, batch, dt, 
 
 @pytest.mark.parametrize("n,batch,dt,flags", [(1000, 8, np.complex64, 2),
                                               (2520, 16, np.complex128, 1),
+                                              (2520, 512, np.complex64, 1),
+                                              (1024, 256, np.complex64, 2),
                                               (1 << 16, 4, np.complex64, 1),
                                               (4099, 3, np.complex64, 2)])
 def test_tuned_plan_matches_oracle(gpu_ready, n, batch, dt, flags):

@@ -4,8 +4,10 @@
 Workload (one "step"): forward transforms of
     n = 2520 (2^3 3^2 5 7) x batch 4096  and  n = 2^20 x batch 64,
 each in ComplexF32 and ComplexF64, contiguous rows, re/im i.i.d. N(0,1) (synthetic, seeded),
-inputs resident in HBM before the timed region.  Throughput = sum of 5 n log2 n over the step
-/ time (GFLOP/s); algorithmic HBM bytes = 2 n b per transform (b = 8 / 16 bytes).
+inputs resident in HBM before the timed region; plans made with the MEASURE flag (the radix
+sequence and kernel shapes timed on the device at plan time, outside the timed region; --flags
+none gives the default heuristics).  Throughput = sum of 5 n log2 n over the step / time
+(GFLOP/s); algorithmic HBM bytes = 2 n b per transform (b = 8 / 16 bytes).
 
 The same line carries the rest of the metric, each measured in this run: `serving` (a Poisson
 trace of mixed requests through the continuous-batching server: p50 / p99 latency, throughput,
@@ -37,6 +39,11 @@ CASES = [  # (n, batch, dtype)
     (1 << 20, 64, np.complex64),
     (1 << 20, 64, np.complex128),
 ]
+
+
+# planner flags of the batched plans (spells.jl:4-8): MEASURE times candidate radix sequences
+# and kernel shapes on the device when the plan is made, outside the timed region
+FLAGS = {"none": 0, "measure": 1, "enchant": 2}
 
 
 def flops(n, batch):
@@ -113,7 +120,8 @@ def measured_traffic(plan_desc: str):
             table = json.load(f)["cases"]
     except (OSError, ValueError, KeyError):
         return None
-    entry = table.get(plan_desc)
+    key = plan_desc.split(" tuned(")[0].split(" (wisdom)")[0]  # the kernel configuration
+    entry = table.get(key)
     return entry["bytes_per_step"] if entry else None
 
 
@@ -129,6 +137,8 @@ def bench_batched(args):
     plans, xs, ys = [], [], []
     g = torch.Generator(device=dev)
     g.manual_seed(2)
+    if args.wisdom:  # plans of an earlier run (counter runs profile the same kernels)
+        C.import_wisdom(args.wisdom)
     for n, batch, dt in cases:
         tdt = torch.complex64 if dt == np.complex64 else torch.complex128
         rdt = torch.float32 if dt == np.complex64 else torch.float64
@@ -137,7 +147,9 @@ def bench_batched(args):
         assert x.dtype == tdt
         xs.append(x)
         ys.append(torch.empty_like(x))
-        plans.append(C.Plan(n, batch, dt))
+        plans.append(C.Plan(n, batch, dt, flags=FLAGS[args.flags]))
+    if args.save_wisdom:
+        C.export_wisdom(args.save_wisdom)
     h = stream.cuda_stream
 
     def step(evs=None, k=0):
@@ -209,6 +221,7 @@ def bench_batched(args):
         "data": "synthetic: re/im i.i.d. N(0,1), torch PCG seed 2, resident in HBM",
         "config": {"workload": "batched mixed-radix C2C FFT: n=2520 x 4096 and n=2^20 x 64, "
                                "ComplexF32 and ComplexF64, contiguous rows, forward",
+                   "plan_flags": args.flags,
                    "global_batch": sum(b for _, b, _ in cases), "parallelism": "single GPU"},
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -665,6 +678,101 @@ def bench_f16(args):
                       "cases": out}))
 
 
+def _forced_plan(C, n, batch, dt, fields):
+    """A plan of the given shape built from one wisdom line (the knobs `fields`), MEASURE flag."""
+    import tempfile
+
+    code = 0 if np.dtype(dt) == np.complex64 else 1
+    fd, path = tempfile.mkstemp(suffix=".txt")
+    os.close(fd)
+    with open(path, "w") as f:
+        f.write(f"{n} {batch} {n} {n} {code} 1 : {fields}\n")
+    C.import_wisdom(path)
+    os.unlink(path)
+    return C.Plan(n, batch, dt, flags=C.MEASURE)
+
+
+WAVE_CASES = [  # n, dtype, E, threads per workgroup, radix sequence ("" = default)
+    (256, np.complex64, 16, 64, ""), (256, np.complex64, 16, 256, ""),
+    (512, np.complex64, 8, 64, ""), (512, np.complex64, 8, 256, ""),
+    (1024, np.complex64, 16, 64, ""), (1024, np.complex64, 16, 256, ""),
+    (1024, np.complex64, 8, 128, ""), (1024, np.complex64, 8, 256, ""),
+    (1024, np.complex128, 8, 128, ""), (1024, np.complex128, 8, 256, ""),
+    (1024, np.complex128, 16, 64, ""), (1024, np.complex128, 16, 256, ""),
+    (1024, np.complex64, 16, 64, "ra=4,16,16"), (257, np.complex64, 16, 64, ""),
+]
+
+
+def bench_wave(args):
+    """Lines whose last pass runs across wavefront lanes (wv=1) against the same plans with the
+    LDS last pass (wv=0): 128 MiB per buffer, forward.  --case i times one case and one variant
+    (--wave 0 / 1) for three launches only (for counter runs)."""
+    import torch
+
+    import catabrafft as C
+
+    s = torch.cuda.current_stream()
+    cases = list(enumerate(WAVE_CASES)) if args.case < 0 else [(args.case, WAVE_CASES[args.case])]
+    out = []
+    for i, (n, dt, E, tt, extra) in cases:
+        batch = (1 << 27) // (n * np.dtype(dt).itemsize)
+        x = torch.randn(batch, n, dtype=torch.complex64 if dt == np.complex64 else torch.complex128,
+                        device="cuda")
+        y = torch.empty_like(x)
+        row = {"case": i, "n": n, "dtype": np.dtype(dt).name, "E": E, "threads": tt,
+               "radices": extra or "default", "batch": batch}
+        for wv in ((0, 1) if args.case < 0 else (args.wave,)):
+            p = _forced_plan(C, n, batch, dt, f"{E} {tt} 0 0 0 -1 0 0 {extra} wv={wv}")
+            fn = lambda: p.exec_ptr(x.data_ptr(), y.data_ptr(), -1, False, s.cuda_stream)  # noqa
+            if args.case >= 0:
+                for _ in range(3):
+                    fn()
+                torch.cuda.synchronize()
+                ms = None
+            else:
+                ms = _sync_ms(torch, fn, 50, 5, s)
+            row[f"wv{wv}"] = {"ms": None if ms is None else round(ms, 4),
+                              "algo_GBs": None if ms is None else
+                              round(2 * n * batch * np.dtype(dt).itemsize / (ms * 1e-3) / 1e9, 1),
+                              "plan": p.describe()}
+        out.append(row)
+        del x, y
+    print(json.dumps({"workload": "last pass across wavefront lanes (wv=1) vs through LDS (wv=0)",
+                      "cases": out}))
+
+
+def bench_chunk(args):
+    """The 2^20 x 64 four-step run whole (chunk 0) or in rounds of `chunk` transforms whose
+    middle could stay in the Infinity Cache.  --chunks restricts the list (e.g. "f32:8")."""
+    import torch
+
+    import catabrafft as C
+
+    s = torch.cuda.current_stream()
+    n, batch = 1 << 20, 64
+    want = set(args.chunks.split(",")) if args.chunks else None
+    out = []
+    for dt in (np.complex64, np.complex128):
+        tag0 = "f32" if dt == np.complex64 else "f64"
+        x = torch.randn(batch, n, dtype=torch.complex64 if dt == np.complex64 else torch.complex128,
+                        device="cuda")
+        y = torch.empty_like(x)
+        for chunk in (0, 4, 8, 16, 32):
+            tag = f"{tag0}:{chunk}"
+            if want and tag not in want:
+                continue
+            E = 16 if dt == np.complex64 else 8
+            p = _forced_plan(C, n, batch, dt, f"{E} 64 0 0 0 {chunk} 0 0")
+            ms = _sync_ms(torch, lambda: p.exec_ptr(x.data_ptr(), y.data_ptr(), -1, False,
+                                                    s.cuda_stream), 5 if want else 20, 3, s)
+            out.append({"config": tag, "ms": round(ms, 4),
+                        "algo_GBs": round(2 * n * batch * np.dtype(dt).itemsize / (ms * 1e-3) / 1e9, 1),
+                        "plan": p.describe()})
+        del x, y
+    print(json.dumps({"workload": "2^20 x 64 four-step, whole batch vs chunked rounds",
+                      "cases": out}))
+
+
 def bench_copy(args):
     """A plain device-to-device copy of 1 GiB (torch): the reference kernel whose FETCH_SIZE /
     WRITE_SIZE counters calibrate the HBM-traffic figures of the FFT kernels."""
@@ -685,11 +793,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cases", default="all", help="comma list of case indices (batched)")
+    ap.add_argument("--flags", default="measure", choices=sorted(FLAGS),
+                    help="batched: planner flag of the plans")
     ap.add_argument("--workload", default="batched",
-                    choices=["batched", "single", "rader", "uneven", "serve", "copy", "f16"])
+                    choices=["batched", "single", "rader", "uneven", "serve", "copy", "f16",
+                             "wave", "chunk"])
+    ap.add_argument("--case", type=int, default=-1, help="wave: one case index (counter runs)")
+    ap.add_argument("--wave", type=int, default=1, help="wave: the variant of --case")
+    ap.add_argument("--chunks", default="", help="chunk: comma list such as f32:0,f32:8")
     ap.add_argument("--requests", type=int, default=2000, help="serve: trace length")
     ap.add_argument("--no-baseline", action="store_true",
                     help="serve: skip the one-launch-per-transform baseline (profiling)")
+    ap.add_argument("--wisdom", default="", help="batched: import this wisdom file first")
+    ap.add_argument("--save-wisdom", default="", help="batched: export the plans' wisdom here")
     ap.add_argument("--sections", default="serving,rader,uneven",
                     help="batched: comma list of the extra sections to run (or none)")
     args = ap.parse_args()
@@ -700,7 +816,7 @@ def main():
         sys.exit(2)
     {"batched": bench_batched, "single": bench_single, "rader": bench_rader,
      "uneven": bench_uneven, "serve": bench_serve, "copy": bench_copy,
-     "f16": bench_f16}[args.workload](args)
+     "f16": bench_f16, "wave": bench_wave, "chunk": bench_chunk}[args.workload](args)
 
 
 if __name__ == "__main__":

@@ -6,7 +6,8 @@
 // each line (orders and splits of its factors, radix_variants) and the kernel configurations of
 // the same transform: per-thread capacity E, threads per workgroup, the four-step split and its
 // column tiles.  Each candidate is built, checked to run, and timed with HIP events as the
-// median of several batches of launches on the real batch size; the fastest wins.  The choice is
+// median of several runs on the real batch size (from cold caches when the batch is larger than
+// the L2s); the fastest wins.  The choice is
 // a pure function of (n, batch, dtype, layout) and of the measured times, never of the data.
 #include <hip/hip_runtime.h>
 
@@ -110,15 +111,16 @@ static std::vector<PlanOptions> candidates(Ctx* ctx, const Plan& p, int flags) {
     const bool enchant = flags >= CFFT_ENCHANT;
     const std::vector<int> Es = enchant ? std::vector<int>{8, 16} : std::vector<int>{16};
     const std::vector<int> tts = {64, 128, 256, 512};
+    // The first candidate is the NO_FLAG plan itself, so a tuned plan is never slower than the
+    // default in the tuner's own timing.
+    out.push_back(PlanOptions{});
     PlanOptions base;
     base.explicit_opts = true;
-    if (p.n == 1) {
-        out.push_back(base);
-        return out;
-    }
+    if (p.n == 1) return out;
     if (is_smooth(p.n)) {
-        // direct shapes, with and without the pipelined rows kernel
-        for (int E : Es)
+        // direct shapes, with and without the pipelined rows kernel (E = 8 as well: direct
+        // lines default to it)
+        for (int E : {8, 16})
             for (int tt : tts)
                 for (int pipe : {0, 1}) {
                     PlanOptions o = base;
@@ -386,18 +388,35 @@ int autotune(Ctx* ctx, Plan& p, int flags, const PlanOptions& base) {
     if ((rc = in.alloc((size_t)((tb - 1) * p.idist + p.n) * es))) return rc;
     if ((rc = out.alloc((size_t)((tb - 1) * p.odist + p.n) * es))) return rc;
     if (hipMemset(in.p, 0, in.bytes) != hipSuccess) return fail(CFFT_ERR_HIP, "hipMemset failed");
+    // A batch of 16 MiB or more (input plus output) is timed from cold caches, as a batch that
+    // another kernel has just written out, or that arrived from elsewhere, runs: a write of
+    // twice the Infinity Cache (256 MB on MI355X) before every run evicts it.  Runs repeated
+    // back to back on one buffer would time batches up to ~100 MB from that cache, and favour
+    // plans that lose once the data comes from HBM.  Smaller batches, which stay in the L2s of
+    // the kernel that produced them, are timed warm.
+    DevBuf flush;
+    const size_t kColdFromBytes = (size_t)16 << 20, kFlushBytes = (size_t)512 << 20;
+    bool cold = in.bytes + out.bytes >= kColdFromBytes;
+    if (cold && flush.alloc(kFlushBytes) != CFFT_OK) {
+        (void)hipGetLastError();
+        cold = false;
+    }
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
         return fail(CFFT_ERR_HIP, "hipEventCreate failed");
     double best = 1e300;
     PlanOptions bestopt = base;
     bool found = false;
-    const int reps = flags >= CFFT_ENCHANT ? 5 : 3;
+    const int reps = flags >= CFFT_ENCHANT ? 7 : 5;
     for (const PlanOptions& o : cands) {
         Plan t;
         copy_shape(p, t);
         t.batch = tb;
         if (plan_build(ctx, t, o) != CFFT_OK) continue;
+        PlanOptions chosen = o;
+        // the default's pipelined-kernel choice, kept explicit so that its wisdom line
+        // rebuilds the same plan (imported options default to the plain kernel)
+        if (!o.explicit_opts) chosen.pipe = t.pipe_res > 0 ? 1 : 0;
         if (o.wave > 0 && !t.line.pt.wave) continue;  // the line does not allow a wave tail
         bool ok = true;
         for (int w = 0; w < 2 && ok; ++w)
@@ -408,9 +427,9 @@ int autotune(Ctx* ctx, Plan& p, int flags, const PlanOptions& base) {
         }
         std::vector<double> ms;
         for (int r = 0; r < reps; ++r) {
+            if (cold) (void)hipMemsetAsync(flush.p, r & 0xFF, flush.bytes, ctx->stream);
             (void)hipEventRecord(e0, ctx->stream);
-            for (int k = 0; k < 3; ++k)
-                (void)plan_exec(t, in.p, out.p, CFFT_FORWARD, 0, ctx->stream);
+            (void)plan_exec(t, in.p, out.p, CFFT_FORWARD, 0, ctx->stream);
             (void)hipEventRecord(e1, ctx->stream);
             (void)hipEventSynchronize(e1);
             float m = 0;
@@ -420,10 +439,10 @@ int autotune(Ctx* ctx, Plan& p, int flags, const PlanOptions& base) {
         std::sort(ms.begin(), ms.end());
         const double med = ms[ms.size() / 2];
         if (std::getenv("CFFT_TUNE_VERBOSE"))
-            std::fprintf(stderr, "tune %s : %.4f ms\n", plan_description(t).c_str(), med / 3.0);
+            std::fprintf(stderr, "tune %s : %.4f ms\n", plan_description(t).c_str(), med);
         if (med < best) {
             best = med;
-            bestopt = o;
+            bestopt = chosen;
             found = true;
         }
     }
@@ -438,7 +457,7 @@ int autotune(Ctx* ctx, Plan& p, int flags, const PlanOptions& base) {
     }
     std::ostringstream os;
     os << " tuned(" << (flags >= CFFT_ENCHANT ? "enchant" : "measure") << ", " << cands.size()
-       << " candidates, " << best / 3.0 << " ms)";
+       << " candidates, " << best << " ms" << (cold ? " cold" : "") << ")";
     p.desc = plan_description(p) + os.str();
     return CFFT_OK;
 }

@@ -45,11 +45,22 @@ struct PassTable {
     int wave;
     int radix[kMaxPasses];
     int twoff[kMaxPasses];
+    // the stride s of pass i (the product of the radices before it) as a multiplier: b / s =
+    // umulhi(b, sdiv[i]) with sdiv[i] = ceil(2^32 / s), exact for b, s < 2^16; 0 for s = 1
+    unsigned sdiv[kMaxPasses];
 };
+
+// b / s from the pass table's multiplier (a multiply-high instead of a runtime division).
+__host__ __device__ constexpr unsigned stride_divisor(int s) {
+    return s <= 1 ? 0u : (unsigned)(((1ull << 32) + (unsigned long long)s - 1) / (unsigned)s);
+}
+__device__ __forceinline__ int div_stride(int b, unsigned m) {
+    return m ? (int)__umulhi((unsigned)b, m) : b;
+}
 
 template <typename V>
 __device__ __forceinline__ V cmul(V a, V b) {
-    return V{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+    return V{fmadd(a.x, b.x, -(a.y * b.y)), fmadd(a.x, b.y, a.y * b.x)};
 }
 template <typename V>
 __device__ __forceinline__ V cconj(V a) { return V{a.x, -a.y}; }
@@ -129,7 +140,8 @@ struct HasPost<S, std::void_t<decltype(&S::post)>> : std::true_type {};
 // One Stockham pass.  SRC(e) returns element e of the pass input, DST.put(e, v) stores element e
 // of its output.  mid_barrier separates the reads from the writes when both go to the same LDS.
 template <typename T, int R, int MB, class SRC, class DST, class TWA>
-__device__ __forceinline__ void stockham_pass(const int N, const int s, const int t_in, const int Tt,
+__device__ __forceinline__ void stockham_pass(const int N, const int s, const unsigned sm,
+                                              const int t_in, const int Tt,
                                               const bool active, const TWA& tw,
                                               const SRC& src, const DST& dst,
                                               const bool mid_barrier) {
@@ -156,7 +168,7 @@ __device__ __forceinline__ void stockham_pass(const int N, const int s, const in
             const int bt = t + ib * Tt;
             const int b = bt < nb ? bt : nb - 1;
 #pragma unroll
-            for (int k = 0; k < KP; ++k) w[ib][k] = tw.template get<R>(b / s, k + 1);
+            for (int k = 0; k < KP; ++k) w[ib][k] = tw.template get<R>(div_stride(b, sm), k + 1);
         }
     }
     V v[MB][R];
@@ -186,7 +198,7 @@ __device__ __forceinline__ void stockham_pass(const int N, const int s, const in
         const int b = t + ib * Tt;
         if (active && b < nb) {
             Dft<R>::run(v[ib]);
-            const int p = b / s;
+            const int p = div_stride(b, sm);
             const int q = b - p * s;
             if (twiddled) {
 #pragma unroll
@@ -282,7 +294,8 @@ __host__ __device__ constexpr int bit_reverse(int x, int bits) {
 // the workgroup runs the exchanges (threads without a butterfly carry zeros).  barrier: the
 // penultimate pass's reads and the last pass's writes may touch the same LDS.
 template <typename T, int R, class SRC, class DST, class TWA>
-__device__ __forceinline__ void wave_tail_pass(const int N, const int s, const int RL,
+__device__ __forceinline__ void wave_tail_pass(const int N, const int s, const unsigned sm,
+                                               const int RL,
                                                const int t_in, const bool active, const TWA& tw,
                                                const SRC& src, const DST& dst,
                                                const bool barrier) {
@@ -293,7 +306,7 @@ __device__ __forceinline__ void wave_tail_pass(const int N, const int s, const i
     const int nb = N / R;
     const bool live = active && b < nb;
     const int bc = b < nb ? b : nb - 1;
-    const int p = bc / s;
+    const int p = div_stride(bc, sm);
     const int q = bc - p * s;
     constexpr int VW = (int)(sizeof(V) / 4);
     constexpr int KP0 = TWA::kPrefetch ? tw_prefetch_vgprs(R * VW) / VW : 0;
@@ -414,19 +427,19 @@ __device__ __forceinline__ void run_line_t(const PassTable& pt, const int N, con
             const bool writes_lds = !to_global || dst_aliases_lds;
             const bool bar = reads_lds && writes_lds;
             if (sg && to_global) {
-#define CFFT_CALL_WGG(RR) wave_tail_pass<T, RR>(N, s, RL, t, active, twa, gsrc, gdst, bar)
+#define CFFT_CALL_WGG(RR) wave_tail_pass<T, RR>(N, s, pt.sdiv[ip], RL, t, active, twa, gsrc, gdst, bar)
                 CFFT_WAVE_SWITCH(RM, R, CFFT_CALL_WGG)
 #undef CFFT_CALL_WGG
             } else if (sg) {
-#define CFFT_CALL_WGL(RR) wave_tail_pass<T, RR>(N, s, RL, t, active, twa, gsrc, lds, bar)
+#define CFFT_CALL_WGL(RR) wave_tail_pass<T, RR>(N, s, pt.sdiv[ip], RL, t, active, twa, gsrc, lds, bar)
                 CFFT_WAVE_SWITCH(RM, R, CFFT_CALL_WGL)
 #undef CFFT_CALL_WGL
             } else if (to_global) {
-#define CFFT_CALL_WLG(RR) wave_tail_pass<T, RR>(N, s, RL, t, active, twa, lds, gdst, bar)
+#define CFFT_CALL_WLG(RR) wave_tail_pass<T, RR>(N, s, pt.sdiv[ip], RL, t, active, twa, lds, gdst, bar)
                 CFFT_WAVE_SWITCH(RM, R, CFFT_CALL_WLG)
 #undef CFFT_CALL_WLG
             } else {
-#define CFFT_CALL_WLL(RR) wave_tail_pass<T, RR>(N, s, RL, t, active, twa, lds, lds, bar)
+#define CFFT_CALL_WLL(RR) wave_tail_pass<T, RR>(N, s, pt.sdiv[ip], RL, t, active, twa, lds, lds, bar)
                 CFFT_WAVE_SWITCH(RM, R, CFFT_CALL_WLL)
 #undef CFFT_CALL_WLL
             }
@@ -436,19 +449,19 @@ __device__ __forceinline__ void run_line_t(const PassTable& pt, const int N, con
         }
         const bool dg = last && to_global;
         if (sg && dg) {
-#define CFFT_CALL_GG(RR) stockham_pass<T, RR, bfly_per_thread(E, RR)>(N, s, t, Tt, active, twa, gsrc, gdst, dst_aliases_lds || src_aliases_lds)
+#define CFFT_CALL_GG(RR) stockham_pass<T, RR, bfly_per_thread(E, RR)>(N, s, pt.sdiv[ip], t, Tt, active, twa, gsrc, gdst, dst_aliases_lds || src_aliases_lds)
             CFFT_RADIX_SWITCH(RM, R, CFFT_CALL_GG)
 #undef CFFT_CALL_GG
         } else if (sg) {
-#define CFFT_CALL_GL(RR) stockham_pass<T, RR, bfly_per_thread(E, RR)>(N, s, t, Tt, active, twa, gsrc, lds, src_aliases_lds)
+#define CFFT_CALL_GL(RR) stockham_pass<T, RR, bfly_per_thread(E, RR)>(N, s, pt.sdiv[ip], t, Tt, active, twa, gsrc, lds, src_aliases_lds)
             CFFT_RADIX_SWITCH(RM, R, CFFT_CALL_GL)
 #undef CFFT_CALL_GL
         } else if (dg) {
-#define CFFT_CALL_LG(RR) stockham_pass<T, RR, bfly_per_thread(E, RR)>(N, s, t, Tt, active, twa, lds, gdst, dst_aliases_lds)
+#define CFFT_CALL_LG(RR) stockham_pass<T, RR, bfly_per_thread(E, RR)>(N, s, pt.sdiv[ip], t, Tt, active, twa, lds, gdst, dst_aliases_lds)
             CFFT_RADIX_SWITCH(RM, R, CFFT_CALL_LG)
 #undef CFFT_CALL_LG
         } else {
-#define CFFT_CALL_LL(RR) stockham_pass<T, RR, bfly_per_thread(E, RR)>(N, s, t, Tt, active, twa, lds, lds, true)
+#define CFFT_CALL_LL(RR) stockham_pass<T, RR, bfly_per_thread(E, RR)>(N, s, pt.sdiv[ip], t, Tt, active, twa, lds, lds, true)
             CFFT_RADIX_SWITCH(RM, R, CFFT_CALL_LL)
 #undef CFFT_CALL_LL
         }

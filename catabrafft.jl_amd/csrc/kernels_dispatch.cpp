@@ -8,6 +8,8 @@ namespace cfft {
 void init_kernels() {
     init_kernels_f32();
     init_kernels_f64();
+    init_kernels_conv_f32();
+    init_kernels_conv_f64();
     init_kernels_f16();
     init_kernels_pipe_f32();
     init_kernels_pipe_f64();

@@ -21,18 +21,6 @@ hipError_t conv_f32(const void* in, void* out, const ConvSmallArgs& a, const Pas
                     const void* tw, hipStream_t st) {
     return CFFT_E(conv_t, float, a.E, in, out, a, pt, tw, st);
 }
-hipError_t gather_f32_f32(const void* x, void* w, const ConvGatherArgs& a, hipStream_t st) {
-    return gather_t<float, float>(x, w, a, st);
-}
-hipError_t scatter_f32_f32(const void* w, void* y, const ConvScatterArgs& a, hipStream_t st) {
-    return scatter_t<float, float>(w, y, a, st);
-}
-hipError_t multi_f32(const MultiLaunch& L, const MultiArgs& ma, hipStream_t st) {
-    return multi_t<float>(L, ma, st);
-}
-hipError_t gs_multi_f32_f32(const MultiLaunch& L, const MultiArgs& ma, hipStream_t st) {
-    return gs_multi_t<float, float>(L, ma, st);
-}
 hipError_t real_rows_f32(const void* in, void* out, const RealArgs& a, const PassTable& pt,
                          const void* tw, int E, hipStream_t st) {
     return CFFT_E(real_rows_t, float, E, in, out, a, pt, tw, st);
@@ -44,11 +32,6 @@ hipError_t transpose_f32(const void* src, void* dst, int64_t rows, int64_t cols,
 }
 hipError_t real_aux_f32(const void* in, void* out, const RealAuxArgs& a, hipStream_t st) {
     return real_aux_t<float>(in, out, a, st);
-}
-hipError_t convl_f32_f32(const void* x, void* y, const ConvLArgs& a, const PassTable& ptA,
-                         const void* twA, const PassTable& ptB, const void* twB, hipStream_t st) {
-    return a.E <= 8 ? convl_t<float, float, 8>(x, y, a, ptA, twA, ptB, twB, st)
-                    : convl_t<float, float, 16>(x, y, a, ptA, twA, ptB, twB, st);
 }
 // Lets every kernel of this precision use up to 160 KiB of LDS (called once per context).
 void init_kernels_f32() {
@@ -62,23 +45,10 @@ void init_kernels_f32() {
     allow_big_lds(k_fs_b<float, 16>);
     allow_big_lds(k_conv_small<float, 8>);
     allow_big_lds(k_conv_small<float, 16>);
-    allow_big_lds(k_convl_a<float, float, 8>);
-    allow_big_lds(k_convl_a<float, float, 16>);
-    allow_big_lds(k_convl_ba<float, 8>);
-    allow_big_lds(k_convl_ba<float, 16>);
-    allow_big_lds(k_convl_c<float, float, 8>);
-    allow_big_lds(k_convl_c<float, float, 16>);
-    allow_big_lds(k_convl_a_multi<float, float>);
-    allow_big_lds(k_convl_c_multi<float, float>);
-    allow_big_lds(k_convl_ba_multi<float>);
     allow_big_lds(k_conv_small<float, 8, true>);
     allow_big_lds(k_conv_small<float, 16, true>);
     allow_big_lds(k_rows<float, 8, float2, 16, true>);
     allow_big_lds(k_rows<float, 16, float2, 16, true>);
-    allow_big_lds(k_rows_multi<float>);
-    allow_big_lds(k_fs_a_multi<float>);
-    allow_big_lds(k_fs_b_multi<float>);
-    allow_big_lds(k_conv_multi<float>);
     allow_big_lds(k_real_rows<float, 8>);
     allow_big_lds(k_real_rows<float, 16>);
 }

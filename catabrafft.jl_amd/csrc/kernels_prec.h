@@ -33,6 +33,8 @@ hipError_t transpose_f64(const void* src, void* dst, int64_t rows, int64_t cols,
 hipError_t real_aux_f64(const void* in, void* out, const RealAuxArgs& a, hipStream_t st);
 void init_kernels_f32();
 void init_kernels_f64();
+void init_kernels_conv_f32();
+void init_kernels_conv_f64();
 
 hipError_t rows_f16(const void* in, void* out, const RowsArgs& a, const PassTable& pt,
                     const void* tw, hipStream_t st);

@@ -270,6 +270,8 @@ int make_line(LinePlan& lp, int64_t N, Prec pr, int E, int max_threads,
     }
     if (rad.empty() || (int)rad.size() > kMaxPasses)
         return fail(CFFT_ERR_UNSUPPORTED, "line length " + std::to_string(N) + " is not smooth");
+    if (N >= (1 << 16))  // the stride multipliers (stride_divisor) are exact below 2^16
+        return fail(CFFT_ERR_UNSUPPORTED, "line length " + std::to_string(N) + " too long");
     // The kernels are instantiated for E = 8 and E = 16 (a launch picks E <= 8 ? 8 : 16), so
     // the thread count is computed for the instance that will run.
     E = (E > 0 && E <= 8) ? 8 : 16;
@@ -287,6 +289,7 @@ int make_line(LinePlan& lp, int64_t N, Prec pr, int E, int max_threads,
         const int R = rad[i];
         lp.pt.radix[i] = R;
         lp.pt.twoff[i] = (int)tw.size();
+        lp.pt.sdiv[i] = stride_divisor((int)s);
         const int64_t g = N / s, n1 = g / R;
         for (int64_t p = 0; p < n1; ++p)
             for (int k = 1; k < R; ++k) tw.push_back(om(s * p * k));
@@ -806,15 +809,33 @@ static int plan_conv(Ctx* ctx, Plan& p, const PlanOptions& o) {
     if (s.kind != Kind::Direct && s.kind != Kind::FourStep && s.kind != Kind::Large)
         return fail(CFFT_ERR_UNSUPPORTED, "convolution length not plannable");
     if (s.kind == Kind::FourStep && o.fuse_conv != 0) {
-        // three fused launches; the last one runs lines of N1 points over the second FFT's
-        // middle (rows of N2), like a pass B with the first line's plan
+        // Three fused launches: A runs the N1-point lines (columns), BA the N2-point lines, C the
+        // N1-point lines over the second FFT's middle (rows of N2).  Each takes up to 4 lines per
+        // workgroup that fit the LDS: 4 columns read 64-byte runs of F64 rows, which measured
+        // faster than 2 (32-byte runs) with twice the workgroups per CU (p = 1048573, L = 2^21:
+        // 1.37 vs 1.43 ms).
         const size_t table = (size_t)tw3_entries(s.fs.tbits, s.fs.tnc) * esize(p.cprec);
-        const int cc = pick_tile(ctx, s.fs.a, p.cprec, 4, table);
-        const int lsc = fs_line_stride(ctx, p.cprec, s.fs.a, cc, table + 16);
         const size_t es = (size_t)esize(p.cprec) / 2;
-        if (cc >= 1 && cc * s.fs.a.Tt <= 1024 &&
+        const size_t budget = ctx->lds_max;
+        auto tile = [&](const LinePlan& lp, size_t reserve) {
+            int c = 4;
+            while (c > 1 && (2 * es * (size_t)c * lp.Ls + reserve > budget || c * lp.Tt > 1024))
+                c /= 2;
+            return c;
+        };
+        const int ca = tile(s.fs.a, 0), cb = tile(s.fs.b, table), cc = tile(s.fs.a, table);
+        const int lsa = fs_line_stride(ctx, p.cprec, s.fs.a, ca, 16);
+        const int lsb = fs_line_stride(ctx, p.cprec, s.fs.b, cb, table + 16);
+        const int lsc = fs_line_stride(ctx, p.cprec, s.fs.a, cc, table + 16);
+        if (ca * s.fs.a.Tt <= 1024 && cb * s.fs.b.Tt <= 1024 && cc * s.fs.a.Tt <= 1024 &&
+            2 * es * (size_t)ca * lsa <= ctx->lds_max &&
+            fsb_tw_offset(es, cb, lsb) + table <= ctx->lds_max &&
             fsb_tw_offset(es, cc, lsc) + table <= ctx->lds_max) {
             p.convl = true;
+            p.convl_ca = ca;
+            p.convl_lsa = lsa;
+            p.convl_cb = cb;
+            p.convl_lsb = lsb;
             p.convl_cc = cc;
             p.convl_lsc = lsc;
         }
@@ -942,7 +963,8 @@ static std::string describe(const Plan& p) {
         case Kind::ConvLarge:
             os << (p.mode == ConvMode::Rader ? "rader" : "bluestein") << "-large L=" << p.L
                << " conv=" << (p.cprec == Prec::F32 ? "f32" : "f64");
-            if (p.convl) os << " fused3 CC=" << p.convl_cc;
+            if (p.convl)
+                os << " fused3 CA=" << p.convl_ca << " CB=" << p.convl_cb << " CC=" << p.convl_cc;
             os << " {" << describe(*p.sub) << "}";
             if (p.mode == ConvMode::Rader) os << " g=" << p.g;
             break;
@@ -1209,13 +1231,13 @@ static int run_fft_s(const Plan& p, const void* in, void* out, int sgn, double s
                 a.n = p.n;
                 a.N1 = f.N1;
                 a.N2 = f.N2;
-                a.CA = f.CA;
-                a.CB = f.CB;
+                a.CA = p.convl_ca;
+                a.CB = p.convl_cb;
                 a.CC = p.convl_cc;
                 a.TtA = f.a.Tt;
-                a.LsA = f.a.Ls;
+                a.LsA = p.convl_lsa;
                 a.TtB = f.b.Tt;
-                a.LsB = f.b.Ls;
+                a.LsB = p.convl_lsb;
                 a.TtC = f.a.Tt;
                 a.LsC = p.convl_lsc;
                 a.E = f.a.E;

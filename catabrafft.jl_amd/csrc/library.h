@@ -91,7 +91,7 @@ struct Plan {
     // large convolution whose FFT_L is a four-step, run as three fused launches: the tile and
     // LDS line stride of the last launch (lines of N1 points over the second FFT's middle)
     bool convl = false;
-    int convl_cc = 0, convl_lsc = 0;
+    int convl_ca = 0, convl_lsa = 0, convl_cb = 0, convl_lsb = 0, convl_cc = 0, convl_lsc = 0;
     // real-input transforms: 1 = fused (n even, n/2 fits a workgroup), 2 = even via a complex
     // sub-plan of n/2, 3 = odd via a complex sub-plan of n; idist / odist hold rdist (reals) and
     // cdist (complex elements)

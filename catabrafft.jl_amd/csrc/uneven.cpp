@@ -127,14 +127,16 @@ PlanEntry WorkList::add_plan(const Plan& p) {
                 const FourStepPlan& f = p.sub->fs;
                 m.N1 = f.N1;
                 m.N2 = f.N2;
-                m.CA = f.CA;
-                m.CB = f.CB;
+                m.CA = p.convl_ca;
+                m.CB = p.convl_cb;
                 m.CC = p.convl_cc;
                 m.LsC = p.convl_lsc;
                 m.tbits = f.tbits;
                 m.tnc = f.tnc;
                 m.la = make_mline(f.a);
+                m.la.Ls = p.convl_lsa;
                 m.lb = make_mline(f.b);
+                m.lb.Ls = p.convl_lsb;
                 m.tw3 = f.tw3.p;
             }
             break;
@@ -285,10 +287,11 @@ void WorkList::add_transforms(const PlanEntry& e, const std::vector<XformRef>& x
                 const FourStepPlan& f = s.fs;
                 const size_t ce = esz(cp) / 2;
                 const size_t table = (size_t)tw3_entries(f.tbits, f.tnc) * esz(cp);
-                const size_t lda = 2 * ce * (size_t)f.CA * f.a.Ls;
-                const size_t ldb = fsb_tw_offset(ce, f.CB, f.b.Ls) + table;
+                const int CA = p.convl_ca, CB = p.convl_cb;
+                const size_t lda = 2 * ce * (size_t)CA * p.convl_lsa;
+                const size_t ldb = fsb_tw_offset(ce, CB, p.convl_lsb) + table;
                 const size_t ldc = fsb_tw_offset(ce, p.convl_cc, p.convl_lsc) + table;
-                const int ta = (f.N2 + f.CA - 1) / f.CA, tb = (f.N1 + f.CB - 1) / f.CB;
+                const int ta = (f.N2 + CA - 1) / CA, tb = (f.N1 + CB - 1) / CB;
                 const int tc = (f.N2 + p.convl_cc - 1) / p.convl_cc;
                 const double la = std::log2((double)f.N1), lb = std::log2((double)f.N2);
                 for (const XformRef& x : xs) {
@@ -303,16 +306,16 @@ void WorkList::add_transforms(const PlanEntry& e, const std::vector<XformRef>& x
                         it.in = x.in;
                         it.out = x.w1;
                         it.flags = x.in_sel | (kAbs << kSelOutShift) | dirf;
-                        add_item(1, MultiKernel::ConvLA, pr, cp, f.CA * f.a.Tt, lda, it,
-                                 (double)f.CA * f.N1 * la);
+                        add_item(1, MultiKernel::ConvLA, pr, cp, CA * f.a.Tt, lda, it,
+                                 (double)CA * f.N1 * la);
                     }
                     for (int t = 0; t < tb; ++t) {
                         it.count = t;
                         it.in = x.w1;
                         it.out = x.w2;
                         it.flags = kAbs | (kAbs << kSelOutShift) | dirf;
-                        add_item(2, MultiKernel::ConvLBA, pr, cp, f.CB * f.b.Tt, ldb, it,
-                                 2.0 * f.CB * f.N2 * lb);
+                        add_item(2, MultiKernel::ConvLBA, pr, cp, CB * f.b.Tt, ldb, it,
+                                 2.0 * CB * f.N2 * lb);
                     }
                     for (int t = 0; t < tc; ++t) {
                         it.count = t;

@@ -215,21 +215,25 @@ def emit(R: int) -> str:
             L.append(f"    const V {dst} = {{T({sr}{r}) * ({a}.x {'-' if sgn == '+' else '+'} {a}.y), "
                      f"T({sr}{r}) * ({a}.y {sgn} {a}.x)}};")
         elif op == "cmul":
+            # fused: re = fma(a.x, cr, -(a.y ci)), im = fma(a.x, ci, a.y cr)
             a, cr, ci = args
-            L.append(f"    const V {dst} = {{{a}.x * T({cr}) - {a}.y * T({ci}), "
-                     f"{a}.x * T({ci}) + {a}.y * T({cr})}};")
+            L.append(f"    const V {dst} = {{fmadd({a}.x, T({cr}), -({a}.y * T({ci}))), "
+                     f"fmadd({a}.x, T({ci}), {a}.y * T({cr}))}};")
         elif op == "lin":
+            # a0 + c1 a1 + c2 a2 + ..., left to right, each c_j a_j + acc one fused multiply-add
             a0, terms = args
-            xs = [] if a0 is None else [f"{a0}.x"]
-            ys = [] if a0 is None else [f"{a0}.y"]
-            for c, a in terms:
-                if c == "1.0":
-                    xs.append(f"{a}.x")
-                    ys.append(f"{a}.y")
-                else:
-                    xs.append(f"T({c}) * {a}.x")
-                    ys.append(f"T({c}) * {a}.y")
-            L.append(f"    const V {dst} = {{{' + '.join(xs)}, {' + '.join(ys)}}};")
+            comps = []
+            for f in ("x", "y"):
+                acc = None if a0 is None else f"{a0}.{f}"
+                for c, a in terms:
+                    if c == "1.0":
+                        acc = f"{a}.{f}" if acc is None else f"{acc} + {a}.{f}"
+                    elif acc is None:
+                        acc = f"T({c}) * {a}.{f}"
+                    else:
+                        acc = f"fmadd(T({c}), {a}.{f}, {acc})"
+                comps.append(acc)
+            L.append(f"    const V {dst} = {{{comps[0]}, {comps[1]}}};")
     for k, o in enumerate(outs):
         L.append(f"    const V o{k} = {o};")
     for k in range(R):
@@ -304,6 +308,15 @@ def main():
         "#pragma once",
         "#include <hip/hip_runtime.h>",
         "namespace cfft {",
+        "// a * b + c rounded once (v_fma_f32 / v_fma_f64; fma() on the host, also correctly",
+        "// rounded, so a CPU build of these butterflies gives the same bits).",
+        "__host__ __device__ __forceinline__ float fmadd(float a, float b, float c) {",
+        "    return __builtin_fmaf(a, b, c);",
+        "}",
+        "__host__ __device__ __forceinline__ double fmadd(double a, double b, double c) {",
+        "    return __builtin_fma(a, b, c);",
+        "}",
+        "",
     ]
     for R in RADICES:
         parts.append(emit(R))

@@ -104,6 +104,9 @@ void launch(std::function<void()> body, dim3 grid, dim3 block, size_t lds, Strea
 #define gridDim (emu::grid_dim())
 inline void __syncthreads() { emu::barrier(); }
 inline int __ffs(int x) { return __builtin_ffs(x); }
+inline unsigned __umulhi(unsigned a, unsigned b) {
+    return (unsigned)(((unsigned long long)a * b) >> 32);
+}
 
 typedef int hipError_t;
 enum {
@@ -160,6 +163,7 @@ hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind k, hip
 hipError_t hipMemcpy2DAsync(void* d, size_t dp, const void* s, size_t sp, size_t w, size_t h,
                             hipMemcpyKind k, hipStream_t st);
 hipError_t hipMemset(void* p, int v, size_t n);
+hipError_t hipMemsetAsync(void* p, int v, size_t n, hipStream_t st);
 // a __device__ variable is an ordinary host object here: its address is its device address
 #define HIP_SYMBOL(x) (&(x))
 inline hipError_t hipGetSymbolAddress(void** p, const void* sym) {

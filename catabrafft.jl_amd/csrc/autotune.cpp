@@ -156,7 +156,7 @@ static std::vector<PlanOptions> candidates(Ctx* ctx, const Plan& p, int flags) {
             if (!is_smooth(a) || !is_smooth(b)) continue;
             if (a < 16 || b < 16) continue;
             const double r = std::fabs(std::log((double)a / (double)b));
-            if (r <= (enchant ? 2.1 : 0.8)) splits.push_back(a);
+            if (r <= (enchant ? 2.1 : 1.4)) splits.push_back(a);
         }
         const std::vector<int> tiles =
             enchant ? std::vector<int>{2, 4, 8, 16, 32} : std::vector<int>{4, 8, 16};
@@ -183,14 +183,14 @@ static std::vector<PlanOptions> candidates(Ctx* ctx, const Plan& p, int flags) {
                                 o.pipe = pipe;
                                 out.push_back(o);
                             }
-        // the radix sequences of the two four-step lines, at the default tiles
+        // the radix sequences of the two four-step lines, at the default tiles (E = 8 as well:
+        // F64 four-step lines default to it)
         for (int64_t a : splits) {
             const auto ra = radix_variants(a, enchant, enchant ? 6 : 4);
             const auto rb = radix_variants(p.n / a, enchant, enchant ? 6 : 4);
             for (size_t i = 0; i < ra.size(); ++i)
                 for (size_t j = 0; j < rb.size(); ++j) {
-                    if (i == 0 && j == 0) continue;  // the default pair is already a candidate
-                    for (int E : Es) {
+                    for (int E : {8, 16}) {
                         PlanOptions o = base;
                         o.E = E;
                         o.n1 = a;

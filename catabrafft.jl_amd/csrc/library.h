@@ -194,6 +194,7 @@ void ctx_destroy(Ctx* c);
 bool is_smooth(int64_t n);
 int64_t next_smooth(int64_t m);
 std::vector<int> line_radices(int64_t n);
+std::vector<int> fourstep_line_radices(int64_t n);
 int threads_for(const std::vector<int>& rad, int64_t N, int E);
 int line_stride_for(Prec p, int64_t N);
 int make_line(LinePlan& lp, int64_t N, Prec pr, int E, int max_threads,

@@ -183,6 +183,22 @@ static std::vector<PlanOptions> candidates(Ctx* ctx, const Plan& p, int flags) {
                                 o.pipe = pipe;
                                 out.push_back(o);
                             }
+        // ComplexF32: the passes with two lines per thread (A, B or both), at every tile pair
+        if (p.prec == Prec::F32)
+            for (int64_t a : splits)
+                for (int ca : tiles)
+                    for (int cb : tiles)
+                        for (int pair : {1, 2, 3}) {
+                            PlanOptions o = base;
+                            o.E = 16;
+                            o.n1 = a;
+                            o.ca = ca;
+                            o.cb = cb;
+                            o.chunk = 0;
+                            o.pipe = 0;
+                            o.pair = pair;
+                            out.push_back(o);
+                        }
         // the radix sequences of the two four-step lines, at the default tiles (E = 8 as well:
         // F64 four-step lines default to it)
         for (int64_t a : splits) {
@@ -234,7 +250,8 @@ static void copy_shape(const Plan& from, Plan& to) {
 // large [ra=R,R,...] [rb=R,R,...] [wv=0|1]" (files without the last fields read pipe = default,
 // large = 0; ra / rb are the radix sequences of the direct or first line and of the second line,
 // absent for the default sequences; wv the wave tail of a direct or fused-convolution line; fc
-// the fused launches of a large convolution).
+// the fused launches of a large convolution; pr the ComplexF32 four-step passes run with two
+// lines per thread, 1 = A, 2 = B, 3 = both).
 static std::string wisdom_key(const Plan& p, int flags) {
     std::ostringstream k;
     k << p.n << " " << p.batch << " " << p.idist << " " << p.odist << " " << p.dtype << " " << flags;
@@ -258,6 +275,10 @@ static bool parse_radix_fields(const char* text, PlanOptions& o) {
     if (const char* fc = std::strstr(text, "fc=")) {
         if (fc[3] != '0' && fc[3] != '1') return false;
         o.fuse_conv = fc[3] - '0';
+    }
+    if (const char* pr = std::strstr(text, "pr=")) {
+        if (pr[3] < '0' || pr[3] > '3') return false;
+        o.pair = pr[3] - '0';
     }
     for (const char* tag : {"ra=", "rb="}) {
         const char* at = std::strstr(text, tag);
@@ -284,12 +305,13 @@ int wisdom_export(Ctx* ctx, const char* path) {
     std::fprintf(f, "# catabrafft wisdom v1 (gfx950)\n");
     for (const auto& kv : ctx->wisdom) {
         const PlanOptions& o = kv.second;
-        std::fprintf(f, "%s : %d %d %lld %d %d %lld %d %d%s%s%s%s\n", kv.first.c_str(), o.E,
+        std::fprintf(f, "%s : %d %d %lld %d %d %lld %d %d%s%s%s%s%s\n", kv.first.c_str(), o.E,
                      o.target_threads, (long long)o.n1, o.ca, o.cb, (long long)o.chunk, o.pipe,
                      o.large, radix_field(" ra=", o.rad_a).c_str(),
                      radix_field(" rb=", o.rad_b).c_str(),
                      (o.wave >= 0 ? " wv=" + std::to_string(o.wave) : std::string()).c_str(),
-                     (o.fuse_conv >= 0 ? " fc=" + std::to_string(o.fuse_conv) : std::string()).c_str());
+                     (o.fuse_conv >= 0 ? " fc=" + std::to_string(o.fuse_conv) : std::string()).c_str(),
+                     (o.pair >= 0 ? " pr=" + std::to_string(o.pair) : std::string()).c_str());
     }
     std::fclose(f);
     return CFFT_OK;
@@ -408,22 +430,28 @@ int autotune(Ctx* ctx, Plan& p, int flags, const PlanOptions& base) {
     PlanOptions bestopt = base;
     bool found = false;
     const int reps = flags >= CFFT_ENCHANT ? 7 : 5;
-    for (const PlanOptions& o : cands) {
+    std::set<std::string> timed;  // plan descriptions already timed
+    std::vector<std::pair<double, PlanOptions>> results;
+    Kind best_kind = Kind::Trivial;
+    // Builds and times one candidate; the fastest so far becomes bestopt.
+    auto try_candidate = [&](const PlanOptions& o) {
         Plan t;
         copy_shape(p, t);
         t.batch = tb;
-        if (plan_build(ctx, t, o) != CFFT_OK) continue;
+        if (plan_build(ctx, t, o) != CFFT_OK) return;
         PlanOptions chosen = o;
         // the default's pipelined-kernel choice, kept explicit so that its wisdom line
         // rebuilds the same plan (imported options default to the plain kernel)
         if (!o.explicit_opts) chosen.pipe = t.pipe_res > 0 ? 1 : 0;
-        if (o.wave > 0 && !t.line.pt.wave) continue;  // the line does not allow a wave tail
+        if (o.wave > 0 && !t.line.pt.wave) return;  // the line does not allow a wave tail
+        if (o.pair > 0 && !t.fs.pair_a && !t.fs.pair_b) return;  // no pass takes line pairs
+        if (!timed.insert(plan_description(t)).second) return;  // the same plan as another
         bool ok = true;
         for (int w = 0; w < 2 && ok; ++w)
             ok = plan_exec(t, in.p, out.p, CFFT_FORWARD, 0, ctx->stream) == CFFT_OK;
         if (!ok || hipStreamSynchronize(ctx->stream) != hipSuccess) {
             (void)hipGetLastError();
-            continue;
+            return;
         }
         std::vector<double> ms;
         for (int r = 0; r < reps; ++r) {
@@ -440,10 +468,74 @@ int autotune(Ctx* ctx, Plan& p, int flags, const PlanOptions& base) {
         const double med = ms[ms.size() / 2];
         if (std::getenv("CFFT_TUNE_VERBOSE"))
             std::fprintf(stderr, "tune %s : %.4f ms\n", plan_description(t).c_str(), med);
+        results.push_back({med, chosen});
         if (med < best) {
             best = med;
             bestopt = chosen;
+            best_kind = t.kind;
             found = true;
+        }
+    };
+    for (const PlanOptions& o : cands) try_candidate(o);
+    // The search above varies the tiles at the default radix sequences and the radix sequences
+    // at the default tiles; a four-step winner then gets every tile pair, the pipelined passes
+    // and (ComplexF32) the line-pair passes at its own split and sequences.
+    if (found && best_kind == Kind::FourStep) {
+        const PlanOptions w = bestopt;
+        const std::vector<int> tiles = flags >= CFFT_ENCHANT ? std::vector<int>{2, 4, 8, 16, 32}
+                                                              : std::vector<int>{4, 8, 16};
+        for (int ca : tiles)
+            for (int cb : tiles)
+                for (int pipe : {0, 1})
+                    for (int pair : {0, 1, 2, 3}) {
+                        if (pair && (pipe || p.prec != Prec::F32)) continue;
+                        PlanOptions o = w;
+                        o.explicit_opts = true;
+                        o.ca = ca;
+                        o.cb = cb;
+                        o.pipe = pipe;
+                        o.pair = pair;
+                        try_candidate(o);
+                    }
+    }
+    // The fastest few are timed again, three times as often and in turn, so that a single
+    // run's noise (a few percent between cold launches) does not pick the plan.
+    if (results.size() > 1) {
+        std::stable_sort(results.begin(), results.end(),
+                         [](const auto& x, const auto& y) { return x.first < y.first; });
+        const size_t K = std::min<size_t>(4, results.size());
+        std::vector<std::unique_ptr<Plan>> fin;
+        std::vector<std::vector<double>> fms(K);
+        for (size_t i = 0; i < K; ++i) {
+            fin.emplace_back(new Plan());
+            copy_shape(p, *fin.back());
+            fin.back()->batch = tb;
+            if (plan_build(ctx, *fin.back(), results[i].second) != CFFT_OK) fin.back().reset();
+        }
+        for (int r = 0; r < 3 * reps; ++r)
+            for (size_t i = 0; i < K; ++i) {
+                if (!fin[i]) continue;
+                if (cold) (void)hipMemsetAsync(flush.p, r & 0xFF, flush.bytes, ctx->stream);
+                (void)hipEventRecord(e0, ctx->stream);
+                (void)plan_exec(*fin[i], in.p, out.p, CFFT_FORWARD, 0, ctx->stream);
+                (void)hipEventRecord(e1, ctx->stream);
+                (void)hipEventSynchronize(e1);
+                float m = 0;
+                (void)hipEventElapsedTime(&m, e0, e1);
+                fms[i].push_back(m);
+            }
+        double fbest = 1e300;
+        for (size_t i = 0; i < K; ++i) {
+            if (fms[i].empty()) continue;
+            std::sort(fms[i].begin(), fms[i].end());
+            const double med = fms[i][fms[i].size() / 2];
+            if (std::getenv("CFFT_TUNE_VERBOSE"))
+                std::fprintf(stderr, "final %s : %.4f ms\n", plan_description(*fin[i]).c_str(), med);
+            if (med < fbest) {
+                fbest = med;
+                best = med;
+                bestopt = results[i].second;
+            }
         }
     }
     (void)hipEventDestroy(e0);
@@ -456,7 +548,7 @@ int autotune(Ctx* ctx, Plan& p, int flags, const PlanOptions& base) {
         ctx->wisdom[wkey] = bestopt;
     }
     std::ostringstream os;
-    os << " tuned(" << (flags >= CFFT_ENCHANT ? "enchant" : "measure") << ", " << cands.size()
+    os << " tuned(" << (flags >= CFFT_ENCHANT ? "enchant" : "measure") << ", " << timed.size()
        << " candidates, " << best << " ms" << (cold ? " cold" : "") << ")";
     p.desc = plan_description(p) + os.str();
     return CFFT_OK;

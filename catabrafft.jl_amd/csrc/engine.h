@@ -34,6 +34,41 @@ template <typename T> struct Vec2;
 template <> struct Vec2<float> { using type = float2; };
 template <> struct Vec2<double> { using type = double2; };
 
+// Two ComplexF32 lines computed in lockstep by one thread ("pair" kernels): f2p holds one float
+// of each line (a: the first line, b: the second), cp2 one complex element of each.  Every
+// operation acts on a and b separately, with the same IEEE operations as the float path, so a
+// pair of lines gives the float path's bits.  The engine and the codelets run on them unchanged
+// (T = f2p); twiddles, equal for the two lines, stay float2 (cmul(cp2, float2)).
+struct f2p {
+    float a, b;
+    f2p() = default;
+    __host__ __device__ constexpr f2p(float x, float y) : a(x), b(y) {}
+    __host__ __device__ constexpr explicit f2p(double c) : a((float)c), b((float)c) {}
+    __host__ __device__ constexpr explicit f2p(int c) : a((float)c), b((float)c) {}
+};
+__host__ __device__ __forceinline__ f2p operator+(f2p x, f2p y) { return f2p(x.a + y.a, x.b + y.b); }
+__host__ __device__ __forceinline__ f2p operator-(f2p x, f2p y) { return f2p(x.a - y.a, x.b - y.b); }
+__host__ __device__ __forceinline__ f2p operator-(f2p x) { return f2p(-x.a, -x.b); }
+__host__ __device__ __forceinline__ f2p operator*(f2p x, f2p y) { return f2p(x.a * y.a, x.b * y.b); }
+__host__ __device__ __forceinline__ f2p operator*(f2p x, float y) { return f2p(x.a * y, x.b * y); }
+__host__ __device__ __forceinline__ f2p& operator*=(f2p& x, f2p y) {
+    x.a *= y.a;
+    x.b *= y.b;
+    return x;
+}
+__host__ __device__ __forceinline__ f2p fmadd(f2p x, f2p y, f2p z) {
+    return f2p(fmadd(x.a, y.a, z.a), fmadd(x.b, y.b, z.b));
+}
+struct cp2 {
+    f2p x, y;
+};
+template <> struct Vec2<f2p> { using type = cp2; };
+// a * w for a pair element and a twiddle shared by both lines
+__host__ __device__ __forceinline__ cp2 cmul(cp2 a, float2 w) {
+    const f2p wx(w.x, w.x), wy(w.y, w.y);
+    return cp2{fmadd(a.x, wx, -(a.y * wy)), fmadd(a.x, wy, a.y * wx)};
+}
+
 constexpr int kMaxPasses = 20;
 
 // Radix sequence of one line FFT plus the offsets of each pass's twiddle block
@@ -68,6 +103,7 @@ __device__ __forceinline__ V cconj(V a) { return V{a.x, -a.y}; }
 template <typename T> struct PadShift;
 template <> struct PadShift<float> { static constexpr int v = 5; };
 template <> struct PadShift<double> { static constexpr int v = 4; };
+template <> struct PadShift<f2p> { static constexpr int v = 4; };  // 8-byte words, as double
 
 template <typename T>
 __host__ __device__ constexpr int lds_index(int e) { return e + (e >> PadShift<T>::v); }
@@ -157,11 +193,13 @@ __device__ __forceinline__ void stockham_pass(const int N, const int s, const un
     // of loads.  tw_prefetch_vgprs bounds the registers per lane that hold them (KP twiddles per
     // butterfly); the rest are loaded after the butterflies.  The table has rows for p = 0 (all
     // ones), so every butterfly multiplies.  The last pass (s = N / R) has no twiddles.
-    constexpr int VW = (int)(sizeof(V) / 4);
-    constexpr int KP0 = TWA::kPrefetch ? tw_prefetch_vgprs(MB * R * VW) / (MB * VW) : 0;
+    // W: the twiddle type the accessor returns (V, or float2 for pair lines)
+    using W = std::remove_cv_t<std::remove_reference_t<decltype(tw.template get<R>(0, 1))>>;
+    constexpr int VW = (int)(sizeof(V) / 4), WW = (int)(sizeof(W) / 4);
+    constexpr int KP0 = TWA::kPrefetch ? tw_prefetch_vgprs(MB * R * VW) / (MB * WW) : 0;
     constexpr int KP = (R - 1) < KP0 ? (R - 1) : KP0;
     const bool twiddled = s * R < N;
-    V w[MB][KP > 0 ? KP : 1];
+    W w[MB][KP > 0 ? KP : 1];
     if (twiddled) {
 #pragma unroll
         for (int ib = 0; ib < MB; ++ib) {

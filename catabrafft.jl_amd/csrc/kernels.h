@@ -200,6 +200,15 @@ size_t fs_pipe_lds(Prec p, int C, int Ls, size_t twn, size_t t3n);
 int fs_pipe_blocks_per_cu(Prec p, int E, int pass, int R0, int block, size_t lds);
 hipError_t launch_fourstep_b(Prec p, const void* in, void* out, const FourStepBArgs& a,
                              const PassTable& pt, const void* tw, hipStream_t st);
+// ComplexF32 four-step passes with two adjacent lines per thread (kernels_pair_f32.hip): a.C
+// lines per tile (even), a.Ls the LDS stride of a line pair; in, out, the distances and the
+// line length across the pairs (N2 for pass A, N1 for pass B) must keep every pair 16-byte
+// aligned.  fs_pair_lds gives the LDS bytes (pass 1 = B, with the three-level table).
+hipError_t launch_fourstep_a_pair(const void* in, void* out, const FourStepAArgs& a,
+                                  const PassTable& pt, const void* tw, hipStream_t st);
+hipError_t launch_fourstep_b_pair(const void* in, void* out, const FourStepBArgs& a,
+                                  const PassTable& pt, const void* tw, hipStream_t st);
+size_t fs_pair_lds(int pass, int C, int Ls, int tbits, int tnc, bool tw3);
 hipError_t launch_conv_small(Prec p, const void* in, void* out, const ConvSmallArgs& a,
                              const PassTable& pt, const void* tw, hipStream_t st);
 hipError_t launch_real_rows(Prec p, const void* in, void* out, const RealArgs& a,

@@ -8,6 +8,7 @@ namespace cfft {
 void init_kernels() {
     init_kernels_f32();
     init_kernels_f64();
+    init_kernels_pair_f32();
     init_kernels_conv_f32();
     init_kernels_conv_f64();
     init_kernels_f16();
@@ -83,6 +84,14 @@ hipError_t launch_fourstep_a(Prec p, const void* in, void* out, const FourStepAA
 hipError_t launch_fourstep_b(Prec p, const void* in, void* out, const FourStepBArgs& a,
                              const PassTable& pt, const void* tw, hipStream_t st) {
     return p == Prec::F32 ? fsb_f32(in, out, a, pt, tw, st) : fsb_f64(in, out, a, pt, tw, st);
+}
+hipError_t launch_fourstep_a_pair(const void* in, void* out, const FourStepAArgs& a,
+                                  const PassTable& pt, const void* tw, hipStream_t st) {
+    return fsa_pair_f32(in, out, a, pt, tw, st);
+}
+hipError_t launch_fourstep_b_pair(const void* in, void* out, const FourStepBArgs& a,
+                                  const PassTable& pt, const void* tw, hipStream_t st) {
+    return fsb_pair_f32(in, out, a, pt, tw, st);
 }
 hipError_t launch_conv_small(Prec p, const void* in, void* out, const ConvSmallArgs& a,
                              const PassTable& pt, const void* tw, hipStream_t st) {

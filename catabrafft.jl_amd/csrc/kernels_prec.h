@@ -33,6 +33,11 @@ hipError_t transpose_f64(const void* src, void* dst, int64_t rows, int64_t cols,
 hipError_t real_aux_f64(const void* in, void* out, const RealAuxArgs& a, hipStream_t st);
 void init_kernels_f32();
 void init_kernels_f64();
+hipError_t fsa_pair_f32(const void* in, void* out, const FourStepAArgs& a, const PassTable& pt,
+                        const void* tw, hipStream_t st);
+hipError_t fsb_pair_f32(const void* in, void* out, const FourStepBArgs& a, const PassTable& pt,
+                        const void* tw, hipStream_t st);
+void init_kernels_pair_f32();
 void init_kernels_conv_f32();
 void init_kernels_conv_f64();
 

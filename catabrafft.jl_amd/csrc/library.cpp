@@ -889,6 +889,25 @@ static void fs_pipe_setup(Ctx* ctx, Plan& p) {
     }
 }
 
+// Two lines per thread for the ComplexF32 four-step passes in `mask` (bit 0: A, bit 1: B) whose
+// tile holds an even number of lines of an even-length stride (N2 for A, N1 for B) and whose
+// pair layout fits the LDS.  The pair LDS line (C / 2 lines of 8-byte words) gets its own
+// stride from the bank-conflict model of 8-byte words.
+static void fs_pair_setup(Ctx* ctx, Plan& p, int mask) {
+    FourStepPlan& fs = p.fs;
+    fs.pair_a = fs.pair_b = 0;
+    if (p.prec != Prec::F32) return;
+    if ((mask & 1) && fs.pipe_a == 0 && fs.CA >= 2 && fs.CA % 2 == 0 && fs.N2 % 2 == 0) {
+        fs.LsA2 = fs_line_stride(ctx, Prec::F64, fs.a, fs.CA / 2, 16);
+        fs.pair_a = fs_pair_lds(0, fs.CA, fs.LsA2, 0, 0, false) <= ctx->lds_max;
+    }
+    if ((mask & 2) && fs.pipe_b == 0 && fs.CB >= 2 && fs.CB % 2 == 0 && fs.N1 % 2 == 0) {
+        const size_t table = (size_t)tw3_entries(fs.tbits, fs.tnc) * sizeof(float2);
+        fs.LsB2 = fs_line_stride(ctx, Prec::F64, fs.b, fs.CB / 2, table + 16);
+        fs.pair_b = fs_pair_lds(1, fs.CB, fs.LsB2, fs.tbits, fs.tnc, true) <= ctx->lds_max;
+    }
+}
+
 static int plan_fft_core(Ctx* ctx, Plan& p, const PlanOptions& o) {
     if (p.n == 1) {
         p.kind = Kind::Trivial;
@@ -932,6 +951,10 @@ static int plan_fft_core(Ctx* ctx, Plan& p, const PlanOptions& o) {
         if (rc) return rc;
         p.kind = Kind::FourStep;
         if (o.pipe > 0) fs_pipe_setup(ctx, p);
+        // ComplexF32 pass A defaults to line pairs: MEASURE chose them for every even split
+        // it timed over 2^16 .. 2^24 and 10^6 (same bits as the float kernels)
+        const int pair = o.pair >= 0 ? o.pair : (p.prec == Prec::F32 ? 1 : 0);
+        if (pair > 0) fs_pair_setup(ctx, p, pair);
         return CFFT_OK;
     }
     return plan_conv(ctx, p, o);
@@ -964,6 +987,8 @@ static std::string describe(const Plan& p) {
             if (p.fs.chunk) os << " chunk=" << p.fs.chunk;
             if (p.fs.pipe_a || p.fs.pipe_b)
                 os << " pipe=" << (p.fs.pipe_a ? "A" : "") << (p.fs.pipe_b ? "B" : "");
+            if (p.fs.pair_a || p.fs.pair_b)
+                os << " pair=" << (p.fs.pair_a ? "A" : "") << (p.fs.pair_b ? "B" : "");
             break;
         case Kind::Large:
             os << "sixstep N1=" << p.fs.N1 << " " << rad(p.fs.a) << " CA=" << p.fs.CA
@@ -1026,8 +1051,12 @@ static int run_fourstep(const Plan& p, const void* in, void* mid, void* out, int
         a.sgn_in = sgn;
         a.xcd = 1;
         const int64_t es_ = esize(p.prec);
+        auto al16 = [](const void* q) { return ((uintptr_t)q & 15) == 0; };
         if (p.half) {
             CFFT_HIP(launch_fourstep_a_f16(inb, midb, a, fs.a.pt, fs.a.dtw.p, st));
+        } else if (fs.pair_a && al16(inb) && al16(midb) && idist % 2 == 0 && mdist % 2 == 0) {
+            a.Ls = fs.LsA2;
+            CFFT_HIP(launch_fourstep_a_pair(inb, midb, a, fs.a.pt, fs.a.dtw.p, st));
         } else if (fs.pipe_a > 0) {
             a.grid = (int)std::min<int64_t>(fs.pipe_a, (int64_t)((fs.N2 + fs.CA - 1) / fs.CA) * nb);
             CFFT_HIP(launch_fourstep_a_pipe(p.prec, inb, midb, a, fs.a.pt, fs.a.dtw.p,
@@ -1056,6 +1085,10 @@ static int run_fourstep(const Plan& p, const void* in, void* mid, void* out, int
         b.xcd = 1;
         if (p.half) {
             CFFT_HIP(launch_fourstep_b_f16(midb, outb, b, fs.b.pt, fs.b.dtw.p, st));
+        } else if (fs.pair_b && !epi.K && al16(midb) && al16(outb) && mdist % 2 == 0 &&
+                   odist % 2 == 0) {
+            b.Ls = fs.LsB2;
+            CFFT_HIP(launch_fourstep_b_pair(midb, outb, b, fs.b.pt, fs.b.dtw.p, st));
         } else if (fs.pipe_b > 0 && !epi.K) {
             b.grid = (int)std::min<int64_t>(fs.pipe_b, (int64_t)((fs.N1 + fs.CB - 1) / fs.CB) * nb);
             CFFT_HIP(launch_fourstep_b_pipe(p.prec, midb, outb, b, fs.b.pt, fs.b.dtw.p,

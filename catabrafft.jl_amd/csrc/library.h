@@ -43,6 +43,10 @@ struct FourStepPlan {
     DevBuf tw3;
     int tbits = 0, tnc = 0;
     int pipe_a = 0, pipe_b = 0;  // resident workgroups of the pipelined passes (0 = not used)
+    // ComplexF32 passes with two lines per thread (kernels_pair_f32.hip) and their LDS strides
+    // of a line pair; a call whose addresses or distances are not 16-byte aligned runs the float
+    // kernels on the same tiles
+    int pair_a = 0, pair_b = 0, LsA2 = 0, LsB2 = 0;
     int64_t chunk = 0;  // transforms per pass-A/pass-B round (0 = the whole batch)
 };
 
@@ -137,6 +141,9 @@ struct PlanOptions {
     // Large convolutions over a four-step FFT_L: 1 = the three fused launches (k_convl_*), 0 =
     // gather, two four-steps and scatter, -1 = the default (fused).
     int fuse_conv = -1;
+    // ComplexF32 four-step: passes run with two lines per thread (bit 0: pass A, bit 1: pass
+    // B), -1 = the default (pass A)
+    int pair = -1;
     bool explicit_opts = false;
     bool alloc_work = true;    // large convolutions: allocate work buffers for `batch` lines
 };

@@ -710,3 +710,56 @@ def test_emulated_large_convolution_fused_and_unfused(E, tmp_path, n, batch, dt,
         assert rel_l2(z, O.fft(x.astype(np.complex128), "d")) < (1e-6 if dt == np.complex64 else 1e-13)
         lib.cfft_plan_destroy(p)
         lib.cfft_finalize(c2)
+
+
+@pytest.mark.parametrize("n,batch,fields,expect", [
+    (1 << 16, 3, "16 64 0 4 8 0 0 0 pr=3", "pair=AB"),
+    (60 * 84, 5, "16 64 60 8 4 0 0 0 pr=3", "pair=AB"),      # ragged tiles: 84 = 10.5 x 8
+    (60 * 84, 2, "16 64 60 16 2 0 0 0 pr=1", "pair=A"),      # last tile 4 of 16 lines
+    (64 * 81, 3, "16 64 64 4 4 0 0 0 pr=3", "pair=B"),       # N2 odd: pass A keeps one line
+    (1 << 14, 2, "8 64 0 2 16 0 0 0 pr=2", "pair=B"),        # E = 8, one pair per A tile
+])
+def test_emulated_pair_fourstep_matches_oracle_and_float_bits(E, tmp_path, n, batch, fields,
+                                                              expect):
+    """ComplexF32 four-step passes with two adjacent lines per thread, forced through wisdom
+    (pr=): against the oracle forward and backward, bit for bit against the same plan without
+    pairs, in place, and with input and output addresses off 16-byte alignment (which run the
+    float kernels on the same tiles)."""
+    lib, _ = E
+    x = seeded_complex(n, batch=batch, seed=31, dtype=np.complex64)
+    outs = {}
+    for tag, line in (("pair", fields), ("float", fields.rsplit(" pr=", 1)[0] + " pr=0")):
+        path = tmp_path / f"w_{tag}.txt"
+        path.write_text(f"{n} {batch} {n} {n} 0 1 : {line}\n")
+        c2 = ctypes.c_void_p()
+        assert lib.cfft_init(0, ctypes.byref(c2)) == 0
+        assert lib.cfft_wisdom_import(c2, str(path).encode()) == 0, lib.cfft_last_error()
+        p = _plan(lib, c2, n, batch, np.complex64, flags=1)
+        d = _desc(lib, p)
+        assert "(wisdom)" in d and "fourstep" in d and ((expect in d) == (tag == "pair")), d
+        res = []
+        for direction, norm in ((-1, 0), (1, 1)):
+            y = np.zeros_like(x)
+            assert lib.cfft_exec(p, x.ctypes.data, y.ctypes.data, direction, norm, None) == 0
+            ref = O.fft(x.astype(np.complex128), "d", direction, bool(norm))
+            assert rel_l2(y, ref) < _tol(n, np.complex64), (tag, direction)
+            res.append(y)
+        z = x.copy()
+        assert lib.cfft_exec(p, z.ctypes.data, z.ctypes.data, -1, 0, None) == 0
+        res.append(z)
+        # one element off 16-byte alignment, in and out
+        xb = np.zeros(n * batch + 1, np.complex64)
+        yb = np.zeros(n * batch + 1, np.complex64)
+        base = 1 if xb.ctypes.data % 16 == 0 else 0
+        xb[base:base + n * batch] = x.ravel()
+        assert (xb.ctypes.data + 8 * base) % 16 == 8
+        assert lib.cfft_exec(p, xb.ctypes.data + 8 * base, yb.ctypes.data + 8 * base, -1, 0,
+                             None) == 0
+        res.append(yb[base:base + n * batch].reshape(batch, n).copy())
+        outs[tag] = res
+        lib.cfft_plan_destroy(p)
+        lib.cfft_finalize(c2)
+    for a, b in zip(outs["pair"], outs["float"]):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert np.array_equal(outs["pair"][0].view(np.uint32), outs["pair"][2].view(np.uint32))
+    assert np.array_equal(outs["pair"][0].view(np.uint32), outs["pair"][3].view(np.uint32))

@@ -147,7 +147,10 @@ def bench_batched(args):
         assert x.dtype == tdt
         xs.append(x)
         ys.append(torch.empty_like(x))
-        plans.append(C.Plan(n, batch, dt, flags=FLAGS[args.flags]))
+        if args.fields:  # one configuration forced (counter runs of alternative plans)
+            plans.append(_forced_plan(C, n, batch, dt, args.fields))
+        else:
+            plans.append(C.Plan(n, batch, dt, flags=FLAGS[args.flags]))
     if args.save_wisdom:
         C.export_wisdom(args.save_wisdom)
     h = stream.cuda_stream
@@ -806,6 +809,9 @@ def main():
                     help="serve: skip the one-launch-per-transform baseline (profiling)")
     ap.add_argument("--wisdom", default="", help="batched: import this wisdom file first")
     ap.add_argument("--save-wisdom", default="", help="batched: export the plans' wisdom here")
+    ap.add_argument("--fields", default="",
+                    help="batched: plan the selected cases with these wisdom knobs "
+                         "(E threads n1 ca cb chunk pipe large [ra=..] [rb=..] [pr=..])")
     ap.add_argument("--sections", default="serving,rader,uneven",
                     help="batched: comma list of the extra sections to run (or none)")
     args = ap.parse_args()

@@ -432,6 +432,7 @@ int autotune(Ctx* ctx, Plan& p, int flags, const PlanOptions& base) {
     const int reps = flags >= CFFT_ENCHANT ? 7 : 5;
     std::set<std::string> timed;  // plan descriptions already timed
     std::vector<std::pair<double, PlanOptions>> results;
+    std::vector<size_t> order;  // candidate index of each result (the default plan is 0)
     Kind best_kind = Kind::Trivial;
     // Builds and times one candidate; the fastest so far becomes bestopt.
     auto try_candidate = [&](const PlanOptions& o) {
@@ -468,6 +469,7 @@ int autotune(Ctx* ctx, Plan& p, int flags, const PlanOptions& base) {
         const double med = ms[ms.size() / 2];
         if (std::getenv("CFFT_TUNE_VERBOSE"))
             std::fprintf(stderr, "tune %s : %.4f ms\n", plan_description(t).c_str(), med);
+        order.push_back(results.size());
         results.push_back({med, chosen});
         if (med < best) {
             best = med;
@@ -501,8 +503,18 @@ int autotune(Ctx* ctx, Plan& p, int flags, const PlanOptions& base) {
     // The fastest few are timed again, three times as often and in turn, so that a single
     // run's noise (a few percent between cold launches) does not pick the plan.
     if (results.size() > 1) {
-        std::stable_sort(results.begin(), results.end(),
-                         [](const auto& x, const auto& y) { return x.first < y.first; });
+        std::vector<size_t> idx(results.size());
+        for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
+        std::stable_sort(idx.begin(), idx.end(),
+                         [&](size_t x, size_t y) { return results[x].first < results[y].first; });
+        std::vector<std::pair<double, PlanOptions>> sorted;
+        std::vector<size_t> sorted_order;
+        for (size_t i : idx) {
+            sorted.push_back(results[i]);
+            sorted_order.push_back(order[i]);
+        }
+        results.swap(sorted);
+        order.swap(sorted_order);
         const size_t K = std::min<size_t>(4, results.size());
         std::vector<std::unique_ptr<Plan>> fin;
         std::vector<std::vector<double>> fms(K);
@@ -524,18 +536,25 @@ int autotune(Ctx* ctx, Plan& p, int flags, const PlanOptions& base) {
                 (void)hipEventElapsedTime(&m, e0, e1);
                 fms[i].push_back(m);
             }
+        std::vector<double> fmed(K, 1e300);
         double fbest = 1e300;
         for (size_t i = 0; i < K; ++i) {
             if (fms[i].empty()) continue;
             std::sort(fms[i].begin(), fms[i].end());
-            const double med = fms[i][fms[i].size() / 2];
+            fmed[i] = fms[i][fms[i].size() / 2];
             if (std::getenv("CFFT_TUNE_VERBOSE"))
-                std::fprintf(stderr, "final %s : %.4f ms\n", plan_description(*fin[i]).c_str(), med);
-            if (med < fbest) {
-                fbest = med;
-                best = med;
-                bestopt = results[i].second;
-            }
+                std::fprintf(stderr, "final %s : %.4f ms\n", plan_description(*fin[i]).c_str(),
+                             fmed[i]);
+            fbest = std::min(fbest, fmed[i]);
+        }
+        // Finalists within 0.5 % of the fastest are a tie, below the runs' noise: the earliest
+        // candidate (the default plan first) wins it, so that repeated tuning picks one plan.
+        size_t pick = K;
+        for (size_t i = 0; i < K; ++i)
+            if (fmed[i] <= fbest * 1.005 && (pick == K || order[i] < order[pick])) pick = i;
+        if (pick < K) {
+            best = fmed[pick];
+            bestopt = results[pick].second;
         }
     }
     (void)hipEventDestroy(e0);

@@ -438,10 +438,12 @@ static int bank_cost(const int* words, int n, int banks) {
 }
 
 static int fs_line_stride(Ctx* ctx, Prec pr, const LinePlan& lp, int C, size_t reserve) {
+    // the model replays the line's passes: the key holds everything it reads
     static std::mutex mu;
-    static std::map<std::tuple<int, int, int, int, size_t>, int> cache;
-    const auto key = std::make_tuple((int)pr, lp.N, lp.E, C,
-                                     ctx->lds_max - std::min(ctx->lds_max, reserve));
+    static std::map<std::tuple<int, int, int, int, int, size_t, std::vector<int>>, int> cache;
+    const auto key = std::make_tuple((int)pr, lp.N, lp.E, lp.Tt, C,
+                                     ctx->lds_max - std::min(ctx->lds_max, reserve),
+                                     std::vector<int>(lp.pt.radix, lp.pt.radix + lp.pt.npass));
     {
         std::lock_guard<std::mutex> g(mu);
         auto it = cache.find(key);

@@ -763,3 +763,44 @@ def test_emulated_pair_fourstep_matches_oracle_and_float_bits(E, tmp_path, n, ba
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
     assert np.array_equal(outs["pair"][0].view(np.uint32), outs["pair"][2].view(np.uint32))
     assert np.array_equal(outs["pair"][0].view(np.uint32), outs["pair"][3].view(np.uint32))
+
+
+_STRIDE_SCRIPT = r"""
+import ctypes, sys
+sys.path.insert(0, sys.argv[1])
+import catabrafft._lib as L
+lib = L._sig(ctypes.CDLL(sys.argv[2]))
+ctx = ctypes.c_void_p()
+assert lib.cfft_init(0, ctypes.byref(ctx)) == 0
+assert lib.cfft_wisdom_import(ctx, sys.argv[3].encode()) == 0
+out = []
+for n in (1 << 21, 1 << 20):
+    p = ctypes.c_void_p()
+    assert lib.cfft_plan_1d(ctx, n, 4, n, n, 0, 1, ctypes.byref(p)) == 0
+    buf = ctypes.create_string_buffer(1024)
+    lib.cfft_plan_describe(p, buf, len(buf))
+    out.append(buf.value.decode())
+print("\n".join(out))
+"""
+
+
+def test_emulated_lds_stride_does_not_depend_on_earlier_plans(E, tmp_path):
+    """The LDS line stride of a four-step line is chosen by a bank-conflict model that replays
+    the line's passes; a plan built after others with the same line length but other radix
+    sequences gets the stride it gets alone (a fresh process), so a tuned plan and its wisdom
+    line plan the same kernels."""
+    w = tmp_path / "w.txt"
+    w.write_text("2097152 4 2097152 2097152 0 1 : 16 64 2048 4 8 0 0 0 ra=8,16,16 rb=16,16,4\n"
+                 "1048576 4 1048576 1048576 0 1 : 16 64 2048 4 8 0 0 0 ra=16,16,8 rb=4,8,16\n")
+    w2 = tmp_path / "w2.txt"
+    w2.write_text(w.read_text().splitlines()[1] + "\n")
+    lib_path = os.path.join(EMU, "_build", "libcatabrafft_emu.so")
+    pkg = os.path.join(ROOT, "catabrafft.jl_amd")
+    both = subprocess.run([sys.executable, "-c", _STRIDE_SCRIPT, pkg, lib_path, str(w)],
+                          check=True, capture_output=True, text=True).stdout.split("\n")
+    alone = subprocess.run([sys.executable, "-c", _STRIDE_SCRIPT.replace("(1 << 21, 1 << 20)",
+                                                                        "(1 << 20,)"),
+                            pkg, lib_path, str(w2)],
+                           check=True, capture_output=True, text=True).stdout.split("\n")
+    assert "ra=" not in both[1] and "(wisdom)" in both[1], both
+    assert both[1] == alone[0], (both[1], alone[0])

@@ -103,3 +103,33 @@ def test_wisdom_round_trip_reproduces_the_tuned_bits(gpu_ready, tmp_path):
     y2 = run_dev(lambda xd: p2.exec_torch(xd), x)
     assert np.array_equal(y1.view(np.uint32), y2.view(np.uint32))
     assert rel_l2(y1, oracle_ref(x)) <= tol(n, dt)
+
+
+@pytest.mark.parametrize("n,batch,dt", [(1 << 20, 16, np.complex64), (1 << 20, 8, np.complex128),
+                                        (1 << 18, 32, np.complex64)])
+def test_tuned_fourstep_refined_matches_oracle_and_round_trips(gpu_ready, tmp_path, n, batch, dt):
+    """MEASURE on four-step lengths: the winner of the candidate list refined over tiles,
+    pipelined and (ComplexF32) line-pair passes, and the finalists timed again.  The tuned plan
+    matches the oracle, and its wisdom line, exported and imported, plans the same kernels (same
+    description, same bits)."""
+    import catabrafft as C
+
+    x = seeded_complex(n, batch, seed=n % 83 + batch, dtype=dt)
+    p1 = C.Plan(n, batch, dt, flags=C.MEASURE)
+    d1 = p1.describe()
+    print(f"tuned four-step n={n} batch={batch} {np.dtype(dt)}: {d1}", flush=True)
+    assert "fourstep" in d1 and ("tuned(" in d1 or "(wisdom)" in d1), d1
+    _check_plan(p1, x, dt)
+    y1 = run_dev(lambda xd: p1.exec_torch(xd), x)
+    path = tmp_path / "wisdom.txt"
+    C.export_wisdom(str(path))
+    code = 0 if dt == np.complex64 else 1
+    lines = [l for l in path.read_text().splitlines() if l.startswith(f"{n} {batch} {n} {n} {code} 1 :")]
+    assert len(lines) == 1, path.read_text()
+    C.import_wisdom(str(path))
+    p2 = C.Plan(n, batch, dt, flags=C.MEASURE)
+    d2 = p2.describe()
+    assert "(wisdom)" in d2 and d2.split(" (wisdom)")[0] == d1.split(" tuned(")[0].split(" (wisdom)")[0], (d1, d2)
+    y2 = run_dev(lambda xd: p2.exec_torch(xd), x)
+    word = np.uint32 if dt == np.complex64 else np.uint64
+    assert np.array_equal(y1.view(word), y2.view(word))

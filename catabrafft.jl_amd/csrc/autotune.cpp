@@ -184,21 +184,23 @@ static std::vector<PlanOptions> candidates(Ctx* ctx, const Plan& p, int flags) {
                                 out.push_back(o);
                             }
         // ComplexF32: the passes with two lines per thread (A, B or both), at every tile pair
+        // (tiles of 2 lines too: one line pair) and both capacities
         if (p.prec == Prec::F32)
             for (int64_t a : splits)
-                for (int ca : tiles)
-                    for (int cb : tiles)
-                        for (int pair : {1, 2, 3}) {
-                            PlanOptions o = base;
-                            o.E = 16;
-                            o.n1 = a;
-                            o.ca = ca;
-                            o.cb = cb;
-                            o.chunk = 0;
-                            o.pipe = 0;
-                            o.pair = pair;
-                            out.push_back(o);
-                        }
+                for (int E : {8, 16})
+                    for (int ca : {2, 4, 8, 16})
+                        for (int cb : {2, 4, 8, 16})
+                            for (int pair : {1, 2, 3}) {
+                                PlanOptions o = base;
+                                o.E = E;
+                                o.n1 = a;
+                                o.ca = ca;
+                                o.cb = cb;
+                                o.chunk = 0;
+                                o.pipe = 0;
+                                o.pair = pair;
+                                out.push_back(o);
+                            }
         // the radix sequences of the two four-step lines, at the default tiles (E = 8 as well:
         // F64 four-step lines default to it)
         for (int64_t a : splits) {
@@ -486,19 +488,21 @@ int autotune(Ctx* ctx, Plan& p, int flags, const PlanOptions& base) {
         const PlanOptions w = bestopt;
         const std::vector<int> tiles = flags >= CFFT_ENCHANT ? std::vector<int>{2, 4, 8, 16, 32}
                                                               : std::vector<int>{4, 8, 16};
-        for (int ca : tiles)
-            for (int cb : tiles)
-                for (int pipe : {0, 1})
-                    for (int pair : {0, 1, 2, 3}) {
-                        if (pair && (pipe || p.prec != Prec::F32)) continue;
-                        PlanOptions o = w;
-                        o.explicit_opts = true;
-                        o.ca = ca;
-                        o.cb = cb;
-                        o.pipe = pipe;
-                        o.pair = pair;
-                        try_candidate(o);
-                    }
+        for (int E : {8, 16})
+            for (int ca : tiles)
+                for (int cb : tiles)
+                    for (int pipe : {0, 1})
+                        for (int pair : {0, 1, 2, 3}) {
+                            if (pair && (pipe || p.prec != Prec::F32)) continue;
+                            PlanOptions o = w;
+                            o.explicit_opts = true;
+                            o.E = E;
+                            o.ca = ca;
+                            o.cb = cb;
+                            o.pipe = pipe;
+                            o.pair = pair;
+                            try_candidate(o);
+                        }
     }
     // The fastest few are timed again, three times as often and in turn, so that a single
     // run's noise (a few percent between cold launches) does not pick the plan.

@@ -1137,7 +1137,7 @@ k_real_aux(const void* __restrict__ in, void* __restrict__ out, RealAuxArgs a) {
 // ------------------------------------------------------------------------------------------
 // Work-list wrappers (one MItem per workgroup).
 // ------------------------------------------------------------------------------------------
-constexpr int kMultiE = 16;
+constexpr int kMultiE = kMultiEHost;
 constexpr int kChunk = 4096;  // elements per gather / scatter item
 
 template <typename V>

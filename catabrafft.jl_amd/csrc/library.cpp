@@ -821,6 +821,9 @@ static int plan_conv(Ctx* ctx, Plan& p, const PlanOptions& o) {
     so.rad_a.clear();  // radix sequences and the wave tail were chosen for the transform itself
     so.rad_b.clear();
     so.wave = -1;
+    // the fused convolution kernels have no line-pair variant: the FFT_L keeps the float
+    // kernels' default lines (E = 16 in F32)
+    if (so.pair < 0) so.pair = 0;
     rc = plan_fft_core(ctx, s, so);
     if (rc) return rc;
     if (s.kind != Kind::Direct && s.kind != Kind::FourStep && s.kind != Kind::Large)
@@ -941,6 +944,15 @@ static int plan_fft_core(Ctx* ctx, Plan& p, const PlanOptions& o) {
         int rc = CFFT_ERR_UNSUPPORTED;
         if (!o.E && p.prec == Prec::F64)
             rc = plan_fourstep_factors(ctx, p.fs, p.n, p.prec, 8, n1, o.ca, o.cb, chunk, ra, rb);
+        // ComplexF32 lines of even length on both sides default to E = 8 with both passes on
+        // line pairs (MEASURE's choice for every power of two 2^16 .. 2^24 and 4 5^8; E = 8
+        // without pairs is slower than 16).  ComplexF16 plans run the float kernels: E = 16.
+        const bool f32_pairs = !o.E && p.prec == Prec::F32 && p.dtype != CFFT_C16 &&
+                               o.pair != 0 && p.n % 4 == 0;
+        if (f32_pairs) {
+            rc = plan_fourstep_factors(ctx, p.fs, p.n, p.prec, 8, n1, o.ca, o.cb, chunk, ra, rb);
+            if (rc == CFFT_OK && (p.fs.N1 % 2 || p.fs.N2 % 2)) rc = CFFT_ERR_UNSUPPORTED;
+        }
         if (rc) rc = plan_fourstep_factors(ctx, p.fs, p.n, p.prec, E, n1, o.ca, o.cb, chunk, ra, rb);
         if (rc && n1 == 0) {  // too long for the four-step: six steps
             p.fs.b.dtw.reset();
@@ -953,9 +965,11 @@ static int plan_fft_core(Ctx* ctx, Plan& p, const PlanOptions& o) {
         if (rc) return rc;
         p.kind = Kind::FourStep;
         if (o.pipe > 0) fs_pipe_setup(ctx, p);
-        // ComplexF32 pass A defaults to line pairs: MEASURE chose them for every even split
-        // it timed over 2^16 .. 2^24 and 10^6 (same bits as the float kernels)
-        const int pair = o.pair >= 0 ? o.pair : (p.prec == Prec::F32 ? 1 : 0);
+        // ComplexF32 defaults to line pairs (same bits as the float kernels): both passes with
+        // the E = 8 lines above, pass A otherwise (MEASURE chose pass A pairs for every even
+        // split it timed with E = 16)
+        const int pair = o.pair >= 0 ? o.pair
+                                     : (p.prec != Prec::F32 ? 0 : (p.fs.a.E == 8 ? 3 : 1));
         if (pair > 0) fs_pair_setup(ctx, p, pair);
         return CFFT_OK;
     }

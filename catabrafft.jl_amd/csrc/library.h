@@ -142,7 +142,7 @@ struct PlanOptions {
     // gather, two four-steps and scatter, -1 = the default (fused).
     int fuse_conv = -1;
     // ComplexF32 four-step: passes run with two lines per thread (bit 0: pass A, bit 1: pass
-    // B), -1 = the default (pass A)
+    // B), -1 = the default (both with E = 8 lines, pass A with E = 16)
     int pair = -1;
     bool explicit_opts = false;
     bool alloc_work = true;    // large convolutions: allocate work buffers for `batch` lines

@@ -69,6 +69,9 @@ __host__ __device__ inline uint64_t mresolve(uint64_t a, int sel, uint64_t in_ba
 
 // Elements per gather / scatter work item (host copy of the device constant kChunk).
 constexpr int64_t kChunkHost = 4096;
+// Per-thread capacity E of the work-list kernels (host copy of the device constant kMultiE):
+// plans whose lines run in work lists are made with it.
+constexpr int kMultiEHost = 16;
 
 // One launch of a work-list kernel.
 struct MultiLaunch {

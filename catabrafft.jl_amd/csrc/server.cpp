@@ -170,6 +170,19 @@ static int srv_plan(Server* s, int64_t n, int dtype, Plan** out, size_t* new_byt
         PlanOptions o;
         o.alloc_work = false;  // a large convolution's work buffers come with each request
         if ((rc = plan_build(s->ctx, *p, o))) return rc;
+        if (p->kind == Kind::Direct || p->kind == Kind::ConvSmall) {
+            // these run in work lists: lines planned for the work-list kernels' capacity, so
+            // that a workgroup has the threads those kernels use
+            p.reset(new Plan());
+            p->ctx = s->ctx;
+            p->n = n;
+            p->batch = 256;
+            p->idist = p->odist = n;
+            p->dtype = dtype;
+            p->prec = dtype == CFFT_C32 ? Prec::F32 : Prec::F64;
+            o.E = kMultiEHost;
+            if ((rc = plan_build(s->ctx, *p, o))) return rc;
+        }
         // six-step plans (or convolutions over them) own one scratch: rounds on several streams
         // would race
         if (p->kind == Kind::Large || (p->sub && p->sub->kind == Kind::Large))

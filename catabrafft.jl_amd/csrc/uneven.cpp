@@ -625,7 +625,10 @@ int uneven_create(Ctx* ctx, int64_t count, const int64_t* lengths, const int64_t
             p->kind = Kind::Direct;
             p->C = 256;
         } else {
+            // lines planned for the work-list kernels' per-thread capacity, so that a workgroup
+            // has the threads those kernels use (a line planned for E = 8 would get twice as many)
             PlanOptions o;
+            o.E = kMultiEHost;
             if ((rc = plan_build(ctx, *p, o))) return rc;
             if (p->kind == Kind::Large || (p->sub && p->sub->kind == Kind::Large))
                 return fail(CFFT_ERR_UNSUPPORTED, "uneven batches take lengths the four-step "

@@ -1761,6 +1761,7 @@ int plan_create(Ctx* ctx, int64_t n, int64_t batch, int64_t idist, int64_t odist
         p->half = true;
         p->pipe_res = 0;
         p->fs.pipe_a = p->fs.pipe_b = 0;
+        p->fs.pair_a = p->fs.pair_b = 0;
         p->desc.clear();
     }
     if (p->desc.empty()) p->desc = describe(*p);

@@ -776,6 +776,61 @@ def bench_chunk(args):
                       "cases": out}))
 
 
+TUNE_SIZES = [1 << 16, 1 << 18, 1 << 20, 1 << 22, 1 << 24, 10 ** 6, 3 ** 12, 4 * 5 ** 8, 2520,
+              1024, 4099, 65537]
+
+
+def bench_tune(args):
+    """What MEASURE buys: for each length (512 MiB of input per precision), the NO_FLAG plan and
+    the MEASURE plan timed alike, each launch from cold caches (a 512 MiB write first), median of
+    9 launches; planning time of MEASURE beside it."""
+    import torch
+
+    import catabrafft as C
+
+    s = torch.cuda.current_stream()
+    flush = torch.empty(512 << 20, dtype=torch.uint8, device="cuda")
+
+    def cold_ms(p, x, y, reps=9):
+        for _ in range(2):
+            p.exec_ptr(x.data_ptr(), y.data_ptr(), -1, False, s.cuda_stream)
+        ms = []
+        for r in range(reps):
+            flush.fill_(r & 255)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(s)
+            p.exec_ptr(x.data_ptr(), y.data_ptr(), -1, False, s.cuda_stream)
+            e1.record(s)
+            e1.synchronize()
+            ms.append(e0.elapsed_time(e1))
+        return float(np.median(ms))
+
+    out = []
+    for n in TUNE_SIZES:
+        for dt in (np.complex64, np.complex128):
+            es = np.dtype(dt).itemsize
+            batch = max(1, (512 << 20) // (n * es))
+            tdt = torch.complex64 if dt == np.complex64 else torch.complex128
+            x = torch.randn(batch, n, dtype=tdt, device="cuda")
+            y = torch.empty_like(x)
+            p0 = C.Plan(n, batch, dt)
+            t0 = time.perf_counter()
+            p1 = C.Plan(n, batch, dt, flags=C.MEASURE)
+            plan_s = time.perf_counter() - t0
+            m0, m1 = cold_ms(p0, x, y), cold_ms(p1, x, y)
+            gbs = lambda m: round(2 * n * batch * es / (m * 1e-3) / 1e9, 1)  # noqa
+            out.append({"n": n, "batch": batch, "dtype": np.dtype(dt).name,
+                        "default_ms": round(m0, 4), "measure_ms": round(m1, 4),
+                        "default_GBs": gbs(m0), "measure_GBs": gbs(m1),
+                        "speedup": round(m0 / m1, 3), "measure_plan_s": round(plan_s, 2),
+                        "default_plan": p0.describe(), "measure_plan": p1.describe()})
+            print(f"tune n={n} {np.dtype(dt).name}: {m0:.4f} -> {m1:.4f} ms", file=sys.stderr,
+                  flush=True)
+            del x, y
+    print(json.dumps({"workload": "NO_FLAG against MEASURE plans, 512 MiB per case, cold launches",
+                      "cases": out}))
+
+
 def bench_copy(args):
     """A plain device-to-device copy of 1 GiB (torch): the reference kernel whose FETCH_SIZE /
     WRITE_SIZE counters calibrate the HBM-traffic figures of the FFT kernels."""
@@ -800,7 +855,7 @@ def main():
                     help="batched: planner flag of the plans")
     ap.add_argument("--workload", default="batched",
                     choices=["batched", "single", "rader", "uneven", "serve", "copy", "f16",
-                             "wave", "chunk"])
+                             "wave", "chunk", "tune"])
     ap.add_argument("--case", type=int, default=-1, help="wave: one case index (counter runs)")
     ap.add_argument("--wave", type=int, default=1, help="wave: the variant of --case")
     ap.add_argument("--chunks", default="", help="chunk: comma list such as f32:0,f32:8")
@@ -822,7 +877,8 @@ def main():
         sys.exit(2)
     {"batched": bench_batched, "single": bench_single, "rader": bench_rader,
      "uneven": bench_uneven, "serve": bench_serve, "copy": bench_copy,
-     "f16": bench_f16, "wave": bench_wave, "chunk": bench_chunk}[args.workload](args)
+     "f16": bench_f16, "wave": bench_wave, "chunk": bench_chunk,
+     "tune": bench_tune}[args.workload](args)
 
 
 if __name__ == "__main__":

@@ -388,17 +388,13 @@ static int pipe_resident(Ctx* ctx, const Plan& p) {
     return bpc > 0 ? bpc * ctx->num_cu : 0;
 }
 
-// pipe: 1 = the pipelined rows kernel, 0 = the plain one, kPipeAuto = the default (pipelined
-// for F32 lines with a radix above 9; lines of radices <= 9 run faster on the plain kernel's
-// radix-bounded instance, which fits more waves per SIMD).
-constexpr int kPipeAuto = 2;
+// pipe: 1 = the pipelined rows kernel, 0 (or -1) = the plain one.
 static int plan_direct(Ctx* ctx, Plan& p, int E, int target_threads, int pipe = -1,
                        const std::vector<int>* rad = nullptr, int wave = -1) {
     int rc = make_line(p.line, p.n, p.prec, E, 1024, rad);
     if (rc) return rc;
     // the wave-tail kernels are instantiated for float / double lines and the plain rows kernel
     if (!p.half && (wave > 0 || (wave < 0 && kWaveDefault)) && apply_wave_tail(p.line)) pipe = 0;
-    if (pipe == kPipeAuto) pipe = p.prec == Prec::F32 && pass_table_rmax(p.line.pt) > 9;
     const size_t per = 2 * (size_t)esize(p.prec) / 2 * p.line.Ls;
     if (per > ctx->lds_max) return fail(CFFT_ERR_UNSUPPORTED, "line does not fit LDS");
     p.C = rows_lines_per_block(p.line, p.prec, p.batch, std::max<size_t>(per, ctx->lds_rows_budget),
@@ -920,9 +916,12 @@ static int plan_fft_core(Ctx* ctx, Plan& p, const PlanOptions& o) {
     }
     const int E = o.E ? o.E : 16;
     if (o.force_conv == 0 && is_smooth(p.n)) {
-        // Direct lines default to E = 8 (more threads per line; measured faster than 16 on
-        // MI355X for F32 and F64) when the line fits a workgroup that way, and F32 lines with a
-        // radix above 9 to the pipelined rows kernel.
+        // Direct lines default to E = 8 (more threads per line) when the line fits a workgroup
+        // that way, except F32 lines of 512 points or more with a radix above 9, which take
+        // E = 16: measured cold on MI355X (512 MiB batches), E = 16 ran those lines 10-40 %
+        // faster than E = 8 (512 .. 8192 points; 128 points favour E = 8) and the pipelined
+        // rows kernel (their former default) 20-80 % slower than either; it stays a MEASURE
+        // candidate.
         // a forced radix sequence for the whole length makes a direct line; for a factor of n
         // it is the four-step's first line and fixes the split
         int64_t rad_prod = 1;
@@ -930,8 +929,13 @@ static int plan_fft_core(Ctx* ctx, Plan& p, const PlanOptions& o) {
         const bool rad_direct = !o.rad_a.empty() && rad_prod == p.n;
         const int64_t n1 = o.n1 ? o.n1 : (!o.rad_a.empty() && !rad_direct ? rad_prod : 0);
         if (n1 == 0) {
-            const int Ed = o.E ? o.E : (line_fits(ctx, p.n, p.prec, 8) ? 8 : 16);
-            const int pipe = o.pipe >= 0 ? o.pipe : (o.explicit_opts ? 0 : kPipeAuto);
+            int Ed = o.E ? o.E : (line_fits(ctx, p.n, p.prec, 8) ? 8 : 16);
+            const std::vector<int> dr = rad_direct ? o.rad_a : line_radices(p.n);
+            const int rmax = dr.empty() ? 0 : *std::max_element(dr.begin(), dr.end());
+            if (!o.E && p.prec == Prec::F32 && rmax > 9 && p.n >= 512 &&
+                line_fits(ctx, p.n, p.prec, 16))
+                Ed = 16;
+            const int pipe = o.pipe >= 0 ? o.pipe : 0;
             if (line_fits(ctx, p.n, p.prec, Ed) || rad_direct)
                 return plan_direct(ctx, p, Ed, o.target_threads, pipe,
                                    rad_direct ? &o.rad_a : nullptr, o.wave);

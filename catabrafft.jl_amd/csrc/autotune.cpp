@@ -503,6 +503,18 @@ int autotune(Ctx* ctx, Plan& p, int flags, const PlanOptions& base) {
                             o.pair = pair;
                             try_candidate(o);
                         }
+        // then that winner in chunks whose middle (written with ordinary stores) fits the
+        // Infinity Cache: 32, 64 or 128 MB per round
+        const PlanOptions w2 = bestopt;
+        const int64_t tb_bytes = p.n * (p.prec == Prec::F32 ? 8 : 16);
+        for (int64_t mb : {32, 64, 128}) {
+            const int64_t c = std::max<int64_t>(1, (mb << 20) / tb_bytes);
+            if (c >= p.batch) continue;
+            PlanOptions o = w2;
+            o.explicit_opts = true;
+            o.chunk = c;
+            try_candidate(o);
+        }
     }
     // The fastest few are timed again, three times as often and in turn, so that a single
     // run's noise (a few percent between cold launches) does not pick the plan.

@@ -36,6 +36,9 @@ struct FourStepAArgs {
     int sgn_in;
     int grid;              // pipelined kernel: workgroups launched (each loops over tiles)
     int xcd;               // nonzero: consecutive tiles on one XCD
+    // nonzero: the middle is written with ordinary stores (kept in the caches for pass B, as in
+    // chunks whose middle fits the Infinity Cache); zero: streaming stores
+    int mid_plain;
 };
 
 // Four-step pass B (lines = the N1 values of k1, elements j2 at stride N1):

@@ -41,6 +41,8 @@ void init_kernels_f32() {
     allow_big_lds(k_rows<float, 16, float2, 9>);
     allow_big_lds(k_fs_a<float, 8>);
     allow_big_lds(k_fs_a<float, 16>);
+    allow_big_lds(k_fs_a<float, 8, V_t<float>, false>);
+    allow_big_lds(k_fs_a<float, 16, V_t<float>, false>);
     allow_big_lds(k_fs_b<float, 8>);
     allow_big_lds(k_fs_b<float, 16>);
     allow_big_lds(k_conv_small<float, 8>);

@@ -43,6 +43,8 @@ void init_kernels_f64() {
     allow_big_lds(k_rows<double, 16>);
     allow_big_lds(k_fs_a<double, 8>);
     allow_big_lds(k_fs_a<double, 16>);
+    allow_big_lds(k_fs_a<double, 8, V_t<double>, false>);
+    allow_big_lds(k_fs_a<double, 16, V_t<double>, false>);
     allow_big_lds(k_fs_b<double, 8>);
     allow_big_lds(k_fs_b<double, 16>);
     allow_big_lds(k_conv_small<double, 8>);

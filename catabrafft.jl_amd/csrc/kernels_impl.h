@@ -315,7 +315,8 @@ __device__ __forceinline__ int xcd_item() {
     return (b & ~63) | ((b & 7) << 3) | ((b >> 3) & 7);
 }
 
-template <typename T, int E, typename GI = V_t<T>>
+// S: the middle is written with streaming stores (false: ordinary ones).
+template <typename T, int E, typename GI = V_t<T>, bool S = true>
 __global__ void __launch_bounds__(1024)
 k_fs_a(const GI* __restrict__ in, V_t<T>* __restrict__ out, FourStepAArgs a, PassTable pt,
        const V_t<T>* __restrict__ tw) {
@@ -324,7 +325,7 @@ k_fs_a(const GI* __restrict__ in, V_t<T>* __restrict__ out, FourStepAArgs a, Pas
     const int bid = xcd_block(a.xcd);
     const int64_t bt = bid / tiles;
     const int tile = bid - (int)(bt * tiles);
-    fs_a_body<T, E>(smem, in + bt * a.idist, out + bt * a.odist, a.N1, a.N2, a.C, a.Tt, a.Ls,
+    fs_a_body<T, E, GI, S>(smem, in + bt * a.idist, out + bt * a.odist, a.N1, a.N2, a.C, a.Tt, a.Ls,
                     tile * a.C, a.sgn_in, pt, tw);
 }
 
@@ -1381,8 +1382,12 @@ hipError_t fsa_t(const void* in, void* out, const FourStepAArgs& a, const PassTa
                  const void* tw, hipStream_t st) {
     const size_t lds = 2 * sizeof(T) * (size_t)a.C * a.Ls;
     const int64_t blocks = (int64_t)((a.N2 + a.C - 1) / a.C) * a.batch;
-    hipLaunchKernelGGL((k_fs_a<T, E, G>), dim3((unsigned)blocks), dim3(a.C * a.Tt), lds, st,
-                       (const G*)in, (V_t<T>*)out, a, pt, (const V_t<T>*)tw);
+    if (a.mid_plain)
+        hipLaunchKernelGGL((k_fs_a<T, E, G, false>), dim3((unsigned)blocks), dim3(a.C * a.Tt), lds,
+                           st, (const G*)in, (V_t<T>*)out, a, pt, (const V_t<T>*)tw);
+    else
+        hipLaunchKernelGGL((k_fs_a<T, E, G>), dim3((unsigned)blocks), dim3(a.C * a.Tt), lds, st,
+                           (const G*)in, (V_t<T>*)out, a, pt, (const V_t<T>*)tw);
     return hipGetLastError();
 }
 

@@ -39,13 +39,14 @@ struct GPairStrideSrc {
     }
 };
 
-// Pass A output: column j2's line at p, column j2 + 1's at p + N1 (streaming stores).
+// Pass A output: column j2's line at p, column j2 + 1's at p + N1 (S: streaming stores).
+template <bool S>
 struct GPairColsDst {
     float2* p;
     int N1;
     __device__ __forceinline__ void put(int e, cp2 v) const {
-        store_stream<float>(p[e], float2{v.x.a, v.y.a});
-        store_stream<float>(p[N1 + e], float2{v.x.b, v.y.b});
+        store_line_elem<float, S>(p[e], float2{v.x.a, v.y.a});
+        store_line_elem<float, S>(p[N1 + e], float2{v.x.b, v.y.b});
     }
 };
 
@@ -87,7 +88,7 @@ struct GPairStrideDst {
 
 // Pass A: columns j20 .. j20 + 2 Cp - 1 of one transform, as Cp pairs (threads pair first:
 // lane = c + Cp t, so a wave's access to a row covers 2 Cp adjacent columns).
-template <int E>
+template <int E, bool S>
 __device__ __forceinline__ void fs_a_pair_body(unsigned char* smem, const float2* src,
                                                float2* dst, int N1, int N2, int Cp, int Tt,
                                                int Ls, int j20, int sgn, const PassTable& pt,
@@ -101,7 +102,7 @@ __device__ __forceinline__ void fs_a_pair_body(unsigned char* smem, const float2
     const int j2c = active ? j2 : 0;
     const LdsLine<f2p> L = lds_line<f2p>(smem, Cp, Ls, c);
     GPairStrideSrc gs{src + j2c, N2, (float)sgn};
-    GPairColsDst d{dst + (int64_t)j2c * N1, N1};
+    GPairColsDst<S> d{dst + (int64_t)j2c * N1, N1};
     run_line_t<f2p, E>(pt, N1, t, Tt, active, TwGlobal<float2>{tw}, gs, d, L, true, true, false,
                        0, 1);
 }
@@ -130,8 +131,9 @@ __device__ __forceinline__ void fs_b_pair_body(unsigned char* smem, const float2
                        pt.npass == 1, 0, 1);
 }
 
-// a.C = lines per tile (even), a.Ls = the LDS line stride of a line pair.
-template <int E>
+// a.C = lines per tile (even), a.Ls = the LDS line stride of a line pair; S: the middle is
+// written with streaming stores.
+template <int E, bool S = true>
 __global__ void __launch_bounds__(1024)
 k_fs_a_pair(const float2* __restrict__ in, float2* __restrict__ out, FourStepAArgs a,
             PassTable pt, const float2* __restrict__ tw) {
@@ -140,7 +142,7 @@ k_fs_a_pair(const float2* __restrict__ in, float2* __restrict__ out, FourStepAAr
     const int bid = xcd_block(a.xcd);
     const int64_t bt = bid / tiles;
     const int tile = bid - (int)(bt * tiles);
-    fs_a_pair_body<E>(smem, in + bt * a.idist, out + bt * a.odist, a.N1, a.N2, a.C / 2, a.Tt,
+    fs_a_pair_body<E, S>(smem, in + bt * a.idist, out + bt * a.odist, a.N1, a.N2, a.C / 2, a.Tt,
                       a.Ls, tile * a.C, a.sgn_in, pt, tw);
 }
 
@@ -168,8 +170,12 @@ static hipError_t fsa_pair_t(const void* in, void* out, const FourStepAArgs& a,
                              const PassTable& pt, const void* tw, hipStream_t st) {
     const size_t lds = fs_pair_lds(0, a.C, a.Ls, 0, 0, false);
     const int64_t blocks = (int64_t)((a.N2 + a.C - 1) / a.C) * a.batch;
-    hipLaunchKernelGGL((k_fs_a_pair<E>), dim3((unsigned)blocks), dim3(a.C / 2 * a.Tt), lds, st,
-                       (const float2*)in, (float2*)out, a, pt, (const float2*)tw);
+    if (a.mid_plain)
+        hipLaunchKernelGGL((k_fs_a_pair<E, false>), dim3((unsigned)blocks), dim3(a.C / 2 * a.Tt),
+                           lds, st, (const float2*)in, (float2*)out, a, pt, (const float2*)tw);
+    else
+        hipLaunchKernelGGL((k_fs_a_pair<E>), dim3((unsigned)blocks), dim3(a.C / 2 * a.Tt), lds,
+                           st, (const float2*)in, (float2*)out, a, pt, (const float2*)tw);
     return hipGetLastError();
 }
 
@@ -195,6 +201,8 @@ hipError_t fsb_pair_f32(const void* in, void* out, const FourStepBArgs& a, const
 void init_kernels_pair_f32() {
     detail::allow_big_lds(k_fs_a_pair<8>);
     detail::allow_big_lds(k_fs_a_pair<16>);
+    detail::allow_big_lds(k_fs_a_pair<8, false>);
+    detail::allow_big_lds(k_fs_a_pair<16, false>);
     detail::allow_big_lds(k_fs_b_pair<8>);
     detail::allow_big_lds(k_fs_b_pair<16>);
 }

@@ -1070,6 +1070,7 @@ static int run_fourstep(const Plan& p, const void* in, void* mid, void* out, int
         a.odist = mdist;
         a.sgn_in = sgn;
         a.xcd = 1;
+        a.mid_plain = fs.chunk > 0;  // a chunk's middle is read back while still cached
         const int64_t es_ = esize(p.prec);
         auto al16 = [](const void* q) { return ((uintptr_t)q & 15) == 0; };
         if (p.half) {

@@ -718,6 +718,8 @@ def test_emulated_large_convolution_fused_and_unfused(E, tmp_path, n, batch, dt,
     (60 * 84, 2, "16 64 60 16 2 0 0 0 pr=1", "pair=A"),      # last tile 4 of 16 lines
     (64 * 81, 3, "16 64 64 4 4 0 0 0 pr=3", "pair=B"),       # N2 odd: pass A keeps one line
     (1 << 14, 2, "8 64 0 2 16 0 0 0 pr=2", "pair=B"),        # E = 8, one pair per A tile
+    (1 << 16, 2, "8 64 0 4 8 0 0 0 pr=3", "pair=AB"),         # E = 8, both passes (the default)
+    (48 * 96, 3, "8 64 48 4 16 0 0 0 pr=3", "pair=AB"),      # E = 8 lines of radix 3
 ])
 def test_emulated_pair_fourstep_matches_oracle_and_float_bits(E, tmp_path, n, batch, fields,
                                                               expect):

@@ -31,6 +31,9 @@ PAIR = [  # n, batch, wisdom knobs, the description's pair marker
     (60 * 84, 70, "16 64 60 8 4 0 0 0 pr=3", "pair=AB"),     # ragged tiles
     (64 * 81, 40, "16 64 64 4 4 0 0 0 pr=3", "pair=B"),      # N2 odd: pass A stays float
     (10 ** 6, 2, "16 64 1000 4 8 0 0 0 pr=3", "pair=AB"),
+    (1 << 20, 4, "8 64 2048 4 16 0 0 0 pr=3", "pair=AB"),    # E = 8: the ComplexF32 default
+    (1 << 18, 9, "8 64 0 2 16 0 0 0 pr=3", "pair=AB"),       # E = 8, one line pair per A tile
+    (48 * 96, 50, "8 64 48 4 16 0 0 0 pr=3", "pair=AB"),     # E = 8 lines of radix 3
 ]
 
 
@@ -82,7 +85,7 @@ def test_pair_fourstep_oracle_and_float_bits(gpu_ready, tmp_path, n, batch, fiel
     assert np.array_equal(outs["pair"][0].view(np.uint32), outs["pair"][3].view(np.uint32))
 
 
-@pytest.mark.parametrize("n,batch,fields,expect", PAIR[2:5])
+@pytest.mark.parametrize("n,batch,fields,expect", PAIR[2:5] + PAIR[7:])
 def test_pair_fourstep_bits_equal_cpu_emulation(gpu_ready, tmp_path, n, batch, fields, expect):
     torch = gpu_ready
     import catabrafft as C

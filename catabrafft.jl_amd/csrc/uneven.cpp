@@ -428,12 +428,18 @@ int WorkList::finalize(DevBuf& dplans, DevBuf& ditems, DevBuf& dtable,
                    std::tie(b->phase, b->kernel, b->prec, b->cprec, b->block, b->lds);
         });
         constexpr size_t kMinItems = 2048;
+        // groups join only if neither block nor LDS grows by more than 2x: the merged launch runs
+        // every item at the larger of both, which costs occupancy
+        auto close = [](const Group& a, const Group& b) {
+            return std::max(a.block, b.block) <= 2 * std::min(a.block, b.block) &&
+                   std::max(a.lds, b.lds) <= 2 * std::min(a.lds, b.lds);
+        };
         for (size_t i = 0; i < fam.size(); ++i) {
             const Group& g = *fam[i];
             const bool same_family =
                 !merged.empty() && merged.back().phase == g.phase &&
                 merged.back().kernel == g.kernel && merged.back().prec == g.prec &&
-                merged.back().cprec == g.cprec;
+                merged.back().cprec == g.cprec && close(merged.back(), g);
             if (same_family && merged.back().items.size() < kMinItems) {
                 Group& m = merged.back();
                 const size_t off = m.items.size();
@@ -454,7 +460,8 @@ int WorkList::finalize(DevBuf& dplans, DevBuf& ditems, DevBuf& dtable,
                 merged[i + 1].kernel != g.kernel || merged[i + 1].prec != g.prec ||
                 merged[i + 1].cprec != g.cprec;
             if (!last_of_family || g.items.size() >= kMinItems || prev.phase != g.phase ||
-                prev.kernel != g.kernel || prev.prec != g.prec || prev.cprec != g.cprec)
+                prev.kernel != g.kernel || prev.prec != g.prec || prev.cprec != g.cprec ||
+                !close(prev, g))
                 continue;
             const size_t off = prev.items.size();
             prev.items.insert(prev.items.end(), g.items.begin(), g.items.end());

@@ -253,7 +253,8 @@ static void copy_shape(const Plan& from, Plan& to) {
 // large = 0; ra / rb are the radix sequences of the direct or first line and of the second line,
 // absent for the default sequences; wv the wave tail of a direct or fused-convolution line; fc
 // the fused launches of a large convolution; pr the ComplexF32 four-step passes run with two
-// lines per thread, 1 = A, 2 = B, 3 = both).
+// lines per thread, 1 = A, 2 = B, 3 = both; eb the four-step pass B's E where it differs from
+// pass A's; pm the four-step passes on the pipelined kernels, 1 = A, 2 = B, 3 = both).
 static std::string wisdom_key(const Plan& p, int flags) {
     std::ostringstream k;
     k << p.n << " " << p.batch << " " << p.idist << " " << p.odist << " " << p.dtype << " " << flags;
@@ -268,7 +269,8 @@ static std::string radix_field(const char* tag, const std::vector<int>& r) {
     return s;
 }
 
-// Parses "ra=..." / "rb=..." / "wv=..." tokens after the numeric fields; false on a malformed one.
+// Parses the "ra=..." / "rb=..." / "wv=..." / "fc=" / "pr=" / "pm=" / "eb=" tokens after the
+// numeric fields; false on a malformed one.
 static bool parse_radix_fields(const char* text, PlanOptions& o) {
     if (const char* wv = std::strstr(text, "wv=")) {
         if (wv[3] != '0' && wv[3] != '1') return false;
@@ -281,6 +283,16 @@ static bool parse_radix_fields(const char* text, PlanOptions& o) {
     if (const char* pr = std::strstr(text, "pr=")) {
         if (pr[3] < '0' || pr[3] > '3') return false;
         o.pair = pr[3] - '0';
+    }
+    if (const char* pm = std::strstr(text, "pm=")) {
+        if (pm[3] < '0' || pm[3] > '3') return false;
+        o.pipe_mask = pm[3] - '0';
+    }
+    if (const char* eb = std::strstr(text, "eb=")) {
+        char* end = nullptr;
+        const long v = std::strtol(eb + 3, &end, 10);
+        if (end == eb + 3 || (v != 8 && v != 16)) return false;
+        o.Eb = (int)v;
     }
     for (const char* tag : {"ra=", "rb="}) {
         const char* at = std::strstr(text, tag);
@@ -307,13 +319,15 @@ int wisdom_export(Ctx* ctx, const char* path) {
     std::fprintf(f, "# catabrafft wisdom v1 (gfx950)\n");
     for (const auto& kv : ctx->wisdom) {
         const PlanOptions& o = kv.second;
-        std::fprintf(f, "%s : %d %d %lld %d %d %lld %d %d%s%s%s%s%s\n", kv.first.c_str(), o.E,
+        std::fprintf(f, "%s : %d %d %lld %d %d %lld %d %d%s%s%s%s%s%s%s\n", kv.first.c_str(), o.E,
                      o.target_threads, (long long)o.n1, o.ca, o.cb, (long long)o.chunk, o.pipe,
                      o.large, radix_field(" ra=", o.rad_a).c_str(),
                      radix_field(" rb=", o.rad_b).c_str(),
                      (o.wave >= 0 ? " wv=" + std::to_string(o.wave) : std::string()).c_str(),
                      (o.fuse_conv >= 0 ? " fc=" + std::to_string(o.fuse_conv) : std::string()).c_str(),
-                     (o.pair >= 0 ? " pr=" + std::to_string(o.pair) : std::string()).c_str());
+                     (o.pair >= 0 ? " pr=" + std::to_string(o.pair) : std::string()).c_str(),
+                     (o.pipe_mask >= 0 ? " pm=" + std::to_string(o.pipe_mask) : std::string()).c_str(),
+                     (o.Eb > 0 ? " eb=" + std::to_string(o.Eb) : std::string()).c_str());
     }
     std::fclose(f);
     return CFFT_OK;
@@ -503,6 +517,33 @@ int autotune(Ctx* ctx, Plan& p, int flags, const PlanOptions& base) {
                             o.pair = pair;
                             try_candidate(o);
                         }
+        // then each pass alone on the pipelined kernel, at its own E and tile, the other pass
+        // keeping the winner's kernel, E and tile (on MI355X a pipelined pass B of 8 lines at
+        // E = 16 beat the plain pass B where the pipelined pass A at that shape was slower)
+        {
+            const PlanOptions w1 = bestopt;
+            Plan wp;
+            copy_shape(p, wp);
+            wp.batch = tb;
+            if (plan_build(ctx, wp, w1) == CFFT_OK && wp.kind == Kind::FourStep) {
+                for (int pm : {1, 2})
+                    for (int Ep : {8, 16})
+                        for (int c : tiles) {
+                            PlanOptions o = w1;
+                            o.explicit_opts = true;
+                            o.n1 = wp.fs.N1;
+                            o.E = pm == 1 ? Ep : wp.fs.a.E;
+                            o.Eb = pm == 2 ? Ep : wp.fs.b.E;
+                            o.ca = pm == 1 ? c : wp.fs.CA;
+                            o.cb = pm == 2 ? c : wp.fs.CB;
+                            o.pipe = 0;
+                            o.pipe_mask = pm;
+                            // a pipelined pass runs no line pairs; the other keeps its own
+                            o.pair = ((wp.fs.pair_a ? 1 : 0) | (wp.fs.pair_b ? 2 : 0)) & ~pm;
+                            try_candidate(o);
+                        }
+            }
+        }
         // then that winner in chunks whose middle (written with ordinary stores) fits the
         // Infinity Cache: 32, 64 or 128 MB per round
         const PlanOptions w2 = bestopt;

@@ -545,8 +545,9 @@ static int plan_fft_core(Ctx* ctx, Plan& p, const PlanOptions& o);
 static int plan_fourstep_factors(Ctx* ctx, FourStepPlan& fs, int64_t n, Prec pr, int E,
                                  int64_t n1_force, int ca, int cb, int64_t chunk,
                                  const std::vector<int>* rad_a = nullptr,
-                                 const std::vector<int>* rad_b = nullptr) {
+                                 const std::vector<int>* rad_b = nullptr, int Eb = 0) {
     fs.chunk = chunk;
+    const int E2 = Eb > 0 ? Eb : E;  // pass B's capacity
     int64_t best1 = 0;
     if (n1_force > 0) {
         best1 = n1_force;
@@ -556,7 +557,7 @@ static int plan_fourstep_factors(Ctx* ctx, FourStepPlan& fs, int64_t n, Prec pr,
         for (int64_t a = 2; a * a <= n * 64 && a < n; ++a) {
             if (n % a) continue;
             const int64_t b = n / a;
-            if (!line_fits(ctx, a, pr, E) || !line_fits(ctx, b, pr, E)) continue;
+            if (!line_fits(ctx, a, pr, E) || !line_fits(ctx, b, pr, E2)) continue;
             const double score = std::fabs(std::log((double)a / (double)b)) + (a < b ? 0.01 : 0.0);
             if (score < bestscore) {
                 bestscore = score;
@@ -570,7 +571,7 @@ static int plan_fourstep_factors(Ctx* ctx, FourStepPlan& fs, int64_t n, Prec pr,
     const std::vector<int> da = fourstep_line_radices(fs.N1), db = fourstep_line_radices(fs.N2);
     int rc = make_line(fs.a, fs.N1, pr, E, 1024, rad_a ? rad_a : &da);
     if (rc) return rc;
-    rc = make_line(fs.b, fs.N2, pr, E, 1024, rad_b ? rad_b : &db);
+    rc = make_line(fs.b, fs.N2, pr, E2, 1024, rad_b ? rad_b : &db);
     if (rc) return rc;
     set_tw3_shape(fs, n);
     const size_t es = (size_t)esize(pr);
@@ -874,10 +875,11 @@ static int64_t default_chunk(const Plan& p) {
 }
 
 // Resident workgroups of the pipelined four-step passes (0 = a pass cannot use it).
-static void fs_pipe_setup(Ctx* ctx, Plan& p) {
+static void fs_pipe_setup(Ctx* ctx, Plan& p, int mask) {
     FourStepPlan& fs = p.fs;
     fs.pipe_a = fs.pipe_b = 0;
     for (int pass = 0; pass < 2; ++pass) {
+        if (!((mask >> pass) & 1)) continue;
         const LinePlan& lp = pass ? fs.b : fs.a;
         const int C = pass ? fs.CB : fs.CA;
         if (C * lp.Tt > kPipeMaxThreads || lp.pt.npass < 2) continue;
@@ -947,17 +949,21 @@ static int plan_fft_core(Ctx* ctx, Plan& p, const PlanOptions& o) {
         const std::vector<int>* rb = o.rad_b.empty() ? nullptr : &o.rad_b;
         int rc = CFFT_ERR_UNSUPPORTED;
         if (!o.E && p.prec == Prec::F64)
-            rc = plan_fourstep_factors(ctx, p.fs, p.n, p.prec, 8, n1, o.ca, o.cb, chunk, ra, rb);
+            rc = plan_fourstep_factors(ctx, p.fs, p.n, p.prec, 8, n1, o.ca, o.cb, chunk, ra, rb,
+                                       o.Eb);
         // ComplexF32 lines of even length on both sides default to E = 8 with both passes on
         // line pairs (MEASURE's choice for every power of two 2^16 .. 2^24 and 4 5^8; E = 8
         // without pairs is slower than 16).  ComplexF16 plans run the float kernels: E = 16.
         const bool f32_pairs = !o.E && p.prec == Prec::F32 && p.dtype != CFFT_C16 &&
                                o.pair != 0 && p.n % 4 == 0;
         if (f32_pairs) {
-            rc = plan_fourstep_factors(ctx, p.fs, p.n, p.prec, 8, n1, o.ca, o.cb, chunk, ra, rb);
+            rc = plan_fourstep_factors(ctx, p.fs, p.n, p.prec, 8, n1, o.ca, o.cb, chunk, ra, rb,
+                                       o.Eb);
             if (rc == CFFT_OK && (p.fs.N1 % 2 || p.fs.N2 % 2)) rc = CFFT_ERR_UNSUPPORTED;
         }
-        if (rc) rc = plan_fourstep_factors(ctx, p.fs, p.n, p.prec, E, n1, o.ca, o.cb, chunk, ra, rb);
+        if (rc)
+            rc = plan_fourstep_factors(ctx, p.fs, p.n, p.prec, E, n1, o.ca, o.cb, chunk, ra, rb,
+                                       o.Eb);
         if (rc && n1 == 0) {  // too long for the four-step: six steps
             p.fs.b.dtw.reset();
             p.fs.tw3.reset();
@@ -968,7 +974,8 @@ static int plan_fft_core(Ctx* ctx, Plan& p, const PlanOptions& o) {
         }
         if (rc) return rc;
         p.kind = Kind::FourStep;
-        if (o.pipe > 0) fs_pipe_setup(ctx, p);
+        const int pipe_mask = o.pipe_mask >= 0 ? o.pipe_mask : (o.pipe > 0 ? 3 : 0);
+        if (pipe_mask) fs_pipe_setup(ctx, p, pipe_mask);
         // ComplexF32 defaults to line pairs (same bits as the float kernels): both passes with
         // the E = 8 lines above, pass A otherwise (MEASURE chose pass A pairs for every even
         // split it timed with E = 16)

@@ -144,6 +144,10 @@ struct PlanOptions {
     // ComplexF32 four-step: passes run with two lines per thread (bit 0: pass A, bit 1: pass
     // B), -1 = the default (both with E = 8 lines, pass A with E = 16)
     int pair = -1;
+    // Four-step: pass B's per-thread capacity (0 = E), and the passes that run the pipelined
+    // kernels (bit 0: pass A, bit 1: pass B; -1 = both when pipe > 0, else neither)
+    int Eb = 0;
+    int pipe_mask = -1;
     bool explicit_opts = false;
     bool alloc_work = true;    // large convolutions: allocate work buffers for `batch` lines
 };

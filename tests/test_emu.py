@@ -250,7 +250,9 @@ def test_emulated_wisdom_round_trip(E, tmp_path):
     "4096 2 4096 4096 0 1 : 16 64 -5 0 0 -1 0 0",   # negative split
     "4096 2 4096 4096 0 1 : 16 64 0 0 0 -2 0 0",    # chunk below -1
     "4096 2 4096 4096 0 1 : 16 64 0 0 0 -1 3 0",    # pipe out of range
-    "4096 2 4096 4096 0 1 : 16 64 0 0 0 -1 0 2"])   # large out of range
+    "4096 2 4096 4096 0 1 : 16 64 0 0 0 -1 0 2",    # large out of range
+    "4096 2 4096 4096 0 1 : 16 64 0 0 0 -1 0 0 pm=4",   # pipelined-pass mask out of range
+    "4096 2 4096 4096 0 1 : 16 64 0 0 0 -1 0 0 eb=12"])  # pass B E not built
 def test_emulated_wisdom_rejects_out_of_range_knobs(E, tmp_path, line):
     """A wisdom line whose knobs no kernel instance implements is refused (the import fails and
     the key keeps its tuned or default plan), instead of building a plan that skips butterflies."""
@@ -309,6 +311,49 @@ def test_emulated_pipelined_match_plain(E, tmp_path, n, batch, dt):
         lib.cfft_finalize(c2)
     for a, b in zip(outs[0], outs[1]):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("n,batch,dt,n1", [(1 << 16, 3, np.complex64, 256),
+                                           (1 << 17, 1, np.complex128, 256),
+                                           (7 << 14, 1, np.complex128, 448)])
+def test_emulated_per_pass_pipeline_and_capacity(E, tmp_path, n, batch, dt, n1):
+    """Four-step plans whose passes differ in kernel and per-thread capacity (wisdom `pm=` runs
+    one pass on the pipelined kernel, `eb=` gives pass B its own E): the same bits as the plain
+    plan of that split, the oracle, and the knobs kept through a wisdom export."""
+    lib, ctx = E
+    dcode = 0 if dt == np.complex64 else 1
+    key = f"{n} {batch} {n} {n} {dcode} 1"
+    lines = {"plain": f"8 128 {n1} 4 4 -1 0 0 pr=0",
+             "pipeB": f"8 128 {n1} 4 8 -1 0 0 pr=0 pm=2 eb=16",
+             "pipeA": f"16 64 {n1} 8 4 -1 0 0 pr=0 pm=1 eb=8"}
+    want = {"plain": ("pipe=", "E=8", "E=8"), "pipeB": ("pipe=B", "E=8", "E=16"),
+            "pipeA": ("pipe=A", "E=16", "E=8")}
+    x = seeded_complex(n, batch=batch, seed=6, dtype=dt)
+    ref = O.fft(x.astype(np.complex128), "d")
+    outs = {}
+    for name, fields in lines.items():
+        path = tmp_path / f"{name}.txt"
+        path.write_text(f"{key} : {fields}\n")
+        c2 = ctypes.c_void_p()
+        assert lib.cfft_init(0, ctypes.byref(c2)) == 0
+        assert lib.cfft_wisdom_import(c2, str(path).encode()) == 0
+        p = _plan(lib, c2, n, batch, dt, flags=1)
+        d = _desc(lib, p)
+        tag, ea, eb = want[name]
+        line_a, line_b = d.split(" N2=")
+        assert (tag in d) == (name != "plain") and ea in line_a and eb in line_b, d
+        y = np.zeros_like(x)
+        assert lib.cfft_exec(p, x.ctypes.data, y.ctypes.data, -1, 0, None) == 0
+        assert rel_l2(y, ref) < _tol(n, dt), name
+        outs[name] = y
+        out = tmp_path / f"{name}_out.txt"
+        assert lib.cfft_wisdom_export(c2, str(out).encode()) == 0
+        text = out.read_text()
+        assert (" pm=" in text) == (name != "plain") and (" eb=" in text) == (name != "plain"), text
+        lib.cfft_plan_destroy(p)
+        lib.cfft_finalize(c2)
+    assert np.array_equal(outs["plain"], outs["pipeB"])
+    assert np.array_equal(outs["plain"], outs["pipeA"])
 
 
 @pytest.mark.parametrize("n,batch,rt", [(16, 3, np.float32), (2520, 2, np.float64), (1000, 2, np.float32),

@@ -17,13 +17,18 @@ from test_gpu_parity import oracle_ref, run_dev, tol
 pytestmark = pytest.mark.gpu
 
 # wisdom fields after the key: E target_threads n1 ca cb chunk pipe large [ra=...] [rb=...]
-# (0 = the planner's default for n1 / ca / cb, chunk -1 = default; ra / rb = radix sequences)
+# (0 = the planner's default for n1 / ca / cb, chunk -1 = default; ra / rb = radix sequences;
+# pm = the four-step passes on the pipelined kernels, eb = pass B's E, pr = line-pair passes)
 FORCED = [
     # n, batch, dtype, fields, what the description must show
     (1 << 16, 6, np.complex64, "16 64 256 2 32 -1 0 0", ["N1=256", "CA=2", "CB=32"]),
     (1 << 16, 6, np.complex128, "8 64 128 8 4 2 0 0", ["N1=128", "CA=8", "CB=4"]),
     (1 << 16, 5, np.complex64, "16 64 512 4 16 -1 1 0", ["N1=512", "pipe"]),
     (1 << 16, 5, np.complex128, "16 64 0 0 0 -1 1 0", ["fourstep", "pipe"]),
+    # one pass on the pipelined kernel, with its own per-thread capacity (pm=, eb=)
+    (1 << 20, 3, np.complex128, "8 128 1024 4 8 -1 0 0 pm=2 eb=16", ["pipe=B", "CB=8"]),
+    (1 << 20, 3, np.complex64, "16 64 2048 4 16 -1 0 0 pm=1 eb=8 pr=2",
+     ["pipe=A", "pair=B"]),
     (1 << 16, 3, np.complex64, "16 64 256 0 0 -1 0 1", ["sixstep"]),
     (2520, 7, np.complex128, "8 128 40 16 16 -1 0 0", ["N1=40", "CA=16", "CB=16"]),
     (2520, 7, np.complex64, "16 64 0 0 0 -1 1 0", ["direct", "pipe"]),

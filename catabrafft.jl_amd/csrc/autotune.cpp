@@ -254,7 +254,8 @@ static void copy_shape(const Plan& from, Plan& to) {
 // absent for the default sequences; wv the wave tail of a direct or fused-convolution line; fc
 // the fused launches of a large convolution; pr the ComplexF32 four-step passes run with two
 // lines per thread, 1 = A, 2 = B, 3 = both; eb the four-step pass B's E where it differs from
-// pass A's; pm the four-step passes on the pipelined kernels, 1 = A, 2 = B, 3 = both).
+// pass A's; pm the four-step passes on the pipelined kernels, 1 = A, 2 = B, 3 = both; pd = 0
+// a direct line's LDS without pad words).
 static std::string wisdom_key(const Plan& p, int flags) {
     std::ostringstream k;
     k << p.n << " " << p.batch << " " << p.idist << " " << p.odist << " " << p.dtype << " " << flags;
@@ -269,8 +270,8 @@ static std::string radix_field(const char* tag, const std::vector<int>& r) {
     return s;
 }
 
-// Parses the "ra=..." / "rb=..." / "wv=..." / "fc=" / "pr=" / "pm=" / "eb=" tokens after the
-// numeric fields; false on a malformed one.
+// Parses the "ra=..." / "rb=..." / "wv=..." / "fc=" / "pr=" / "pm=" / "eb=" / "pd=" tokens after
+// the numeric fields; false on a malformed one.
 static bool parse_radix_fields(const char* text, PlanOptions& o) {
     if (const char* wv = std::strstr(text, "wv=")) {
         if (wv[3] != '0' && wv[3] != '1') return false;
@@ -287,6 +288,10 @@ static bool parse_radix_fields(const char* text, PlanOptions& o) {
     if (const char* pm = std::strstr(text, "pm=")) {
         if (pm[3] < '0' || pm[3] > '3') return false;
         o.pipe_mask = pm[3] - '0';
+    }
+    if (const char* pd = std::strstr(text, "pd=")) {
+        if (pd[3] != '0' && pd[3] != '1') return false;
+        o.lds_pad = pd[3] - '0';
     }
     if (const char* eb = std::strstr(text, "eb=")) {
         char* end = nullptr;
@@ -319,7 +324,7 @@ int wisdom_export(Ctx* ctx, const char* path) {
     std::fprintf(f, "# catabrafft wisdom v1 (gfx950)\n");
     for (const auto& kv : ctx->wisdom) {
         const PlanOptions& o = kv.second;
-        std::fprintf(f, "%s : %d %d %lld %d %d %lld %d %d%s%s%s%s%s%s%s\n", kv.first.c_str(), o.E,
+        std::fprintf(f, "%s : %d %d %lld %d %d %lld %d %d%s%s%s%s%s%s%s%s\n", kv.first.c_str(), o.E,
                      o.target_threads, (long long)o.n1, o.ca, o.cb, (long long)o.chunk, o.pipe,
                      o.large, radix_field(" ra=", o.rad_a).c_str(),
                      radix_field(" rb=", o.rad_b).c_str(),
@@ -327,7 +332,8 @@ int wisdom_export(Ctx* ctx, const char* path) {
                      (o.fuse_conv >= 0 ? " fc=" + std::to_string(o.fuse_conv) : std::string()).c_str(),
                      (o.pair >= 0 ? " pr=" + std::to_string(o.pair) : std::string()).c_str(),
                      (o.pipe_mask >= 0 ? " pm=" + std::to_string(o.pipe_mask) : std::string()).c_str(),
-                     (o.Eb > 0 ? " eb=" + std::to_string(o.Eb) : std::string()).c_str());
+                     (o.Eb > 0 ? " eb=" + std::to_string(o.Eb) : std::string()).c_str(),
+                     (o.lds_pad >= 0 ? " pd=" + std::to_string(o.lds_pad) : std::string()).c_str());
     }
     std::fclose(f);
     return CFFT_OK;
@@ -495,6 +501,22 @@ int autotune(Ctx* ctx, Plan& p, int flags, const PlanOptions& base) {
         }
     };
     for (const PlanOptions& o : cands) try_candidate(o);
+    // A direct winner is timed again at every shape with its LDS lines unpadded: where LDS bounds
+    // the workgroups per CU, one more fits (2520 F64 at 180 threads: 6 % faster on MI355X); where
+    // it does not, the pad words' fewer bank conflicts can win.
+    if (found && best_kind == Kind::Direct) {
+        const PlanOptions w = bestopt;
+        for (int E : {8, 16})
+            for (int tt : {64, 128, 256, 512}) {
+                PlanOptions o = w;
+                o.explicit_opts = true;
+                o.E = E;
+                o.target_threads = tt;
+                o.pipe = 0;
+                o.lds_pad = 0;
+                try_candidate(o);
+            }
+    }
     // The search above varies the tiles at the default radix sequences and the radix sequences
     // at the default tiles; a four-step winner then gets every tile pair, the pipelined passes
     // and (ComplexF32) the line-pair passes at its own split and sequences.

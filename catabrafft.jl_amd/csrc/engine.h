@@ -108,18 +108,25 @@ template <> struct PadShift<f2p> { static constexpr int v = 4; };  // 8-byte wor
 template <typename T>
 __host__ __device__ constexpr int lds_index(int e) { return e + (e >> PadShift<T>::v); }
 
-// One line's slice of LDS (structure of arrays).
+// Shift of an unpadded LDS line: e >> 31 is 0 for every element index.
+constexpr int kNoPadShift = 31;
+
+// One line's slice of LDS (structure of arrays).  Element e sits at word e + (e >> sh): one pad
+// word per 2^sh elements (lds_index) spreads power-of-two strides over the banks; sh =
+// kNoPadShift leaves the line unpadded, which lets a direct line whose workgroups are bounded
+// by LDS fit one more of them per CU.
 template <typename T>
 struct LdsLine {
     using V = typename Vec2<T>::type;
     T* re;
     T* im;
+    int sh = PadShift<T>::v;
     __device__ __forceinline__ V operator()(int e) const {
-        const int a = lds_index<T>(e);
+        const int a = e + (e >> sh);
         return V{re[a], im[a]};
     }
     __device__ __forceinline__ void put(int e, V v) const {
-        const int a = lds_index<T>(e);
+        const int a = e + (e >> sh);
         re[a] = v.x;
         im[a] = v.y;
     }

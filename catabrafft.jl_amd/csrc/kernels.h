@@ -25,6 +25,7 @@ struct RowsArgs {
     double scale;          // applied on store
     ConvEpilogue epi;      // epi.K == nullptr: none
     int grid;              // pipelined kernel: workgroups launched (each loops over line groups)
+    int nopad;             // plain rows kernel: LDS lines without pad words (Ls from N alone)
 };
 
 // Four-step pass A for n = N1*N2 (lines = the N2 columns of the N1 x N2 row-major input):

@@ -184,9 +184,10 @@ __device__ __forceinline__ void batched_for(int tid, int step, int total, const 
 }
 
 template <typename T>
-__device__ __forceinline__ LdsLine<T> lds_line(unsigned char* smem, int C, int Ls, int c) {
+__device__ __forceinline__ LdsLine<T> lds_line(unsigned char* smem, int C, int Ls, int c,
+                                               int sh = PadShift<T>::v) {
     T* re = reinterpret_cast<T*>(smem);
-    return LdsLine<T>{re + c * Ls, re + C * Ls + c * Ls};
+    return LdsLine<T>{re + c * Ls, re + C * Ls + c * Ls, sh};
 }
 
 // ------------------------------------------------------------------------------------------
@@ -237,7 +238,7 @@ template <typename T, int E, int RM = 16, bool WT = false, class LINES>
 __device__ __forceinline__ void rows_body(unsigned char* smem, const LINES& lines, int count,
                                           int N, int C, int Tt, int Ls, const PassTable& pt,
                                           const V_t<T>* tw, int sgn, double scale,
-                                          const V_t<T>* K) {
+                                          const V_t<T>* K, int sh = PadShift<T>::v) {
     const int tid = threadIdx.x;
     const int c = tid / Tt;
     const int t = tid - c * Tt;
@@ -252,7 +253,7 @@ __device__ __forceinline__ void rows_body(unsigned char* smem, const LINES& line
         if (active && t == 0) gd.put(0, gs(0));
         return;
     }
-    const LdsLine<T> L = lds_line<T>(smem, C, Ls, c < C ? c : 0);
+    const LdsLine<T> L = lds_line<T>(smem, C, Ls, c < C ? c : 0, sh);
     run_line<T, E, RM, WT>(pt, N, t, Tt, active, tw, gs, gd, L, true, true);
 }
 
@@ -269,7 +270,7 @@ k_rows(const G* __restrict__ in, G* __restrict__ out, RowsArgs a, PassTable pt,
     UniformLines<T, G, G> lines{in + line0 * a.idist, out + line0 * a.odist, a.idist, a.odist,
                                 a.epi.dc ? (V_t<T>*)a.epi.dc + 2 * line0 : nullptr};
     rows_body<T, E, RM, WT>(smem, lines, count, a.N, a.C, a.Tt, a.Ls, pt, tw, a.sgn_in, a.scale,
-                        (const V_t<T>*)a.epi.K);
+                            (const V_t<T>*)a.epi.K, a.nopad ? kNoPadShift : PadShift<T>::v);
 }
 
 // ------------------------------------------------------------------------------------------

@@ -241,14 +241,14 @@ int threads_for(const std::vector<int>& rad, int64_t N, int E) {
 }
 
 template <typename T>
-static int line_stride(int64_t N) {
-    int ls = lds_index<T>((int)N - 1) + 1;
+static int line_stride(int64_t N, bool pad) {
+    int ls = pad ? lds_index<T>((int)N - 1) + 1 : (int)N;
     if ((ls & 1) == 0) ++ls;  // odd line stride: consecutive lines fall in different banks
     return ls;
 }
 
-int line_stride_for(Prec p, int64_t N) {
-    return p == Prec::F32 ? line_stride<float>(N) : line_stride<double>(N);
+int line_stride_for(Prec p, int64_t N, bool pad) {
+    return p == Prec::F32 ? line_stride<float>(N, pad) : line_stride<double>(N, pad);
 }
 
 bool radices_valid(const std::vector<int>& rad, int64_t N) {
@@ -311,6 +311,7 @@ int make_line(LinePlan& lp, int64_t N, Prec pr, int E, int max_threads,
     }
     if (tw.empty()) tw.push_back(cld(1, 0));
     lp.Ls = line_stride_for(pr, N);
+    lp.pad = 1;
     return upload(lp.dtw, tw, pr);
 }
 
@@ -390,11 +391,16 @@ static int pipe_resident(Ctx* ctx, const Plan& p) {
 
 // pipe: 1 = the pipelined rows kernel, 0 (or -1) = the plain one.
 static int plan_direct(Ctx* ctx, Plan& p, int E, int target_threads, int pipe = -1,
-                       const std::vector<int>* rad = nullptr, int wave = -1) {
+                       const std::vector<int>* rad = nullptr, int wave = -1, bool pad = true) {
     int rc = make_line(p.line, p.n, p.prec, E, 1024, rad);
     if (rc) return rc;
     // the wave-tail kernels are instantiated for float / double lines and the plain rows kernel
     if (!p.half && (wave > 0 || (wave < 0 && kWaveDefault)) && apply_wave_tail(p.line)) pipe = 0;
+    // unpadded LDS lines: the plain rows kernel only (the pipelined one keeps the pad words)
+    if (!pad && pipe <= 0) {
+        p.line.pad = 0;
+        p.line.Ls = line_stride_for(p.prec, p.n, false);
+    }
     const size_t per = 2 * (size_t)esize(p.prec) / 2 * p.line.Ls;
     if (per > ctx->lds_max) return fail(CFFT_ERR_UNSUPPORTED, "line does not fit LDS");
     p.C = rows_lines_per_block(p.line, p.prec, p.batch, std::max<size_t>(per, ctx->lds_rows_budget),
@@ -940,7 +946,7 @@ static int plan_fft_core(Ctx* ctx, Plan& p, const PlanOptions& o) {
             const int pipe = o.pipe >= 0 ? o.pipe : 0;
             if (line_fits(ctx, p.n, p.prec, Ed) || rad_direct)
                 return plan_direct(ctx, p, Ed, o.target_threads, pipe,
-                                   rad_direct ? &o.rad_a : nullptr, o.wave);
+                                   rad_direct ? &o.rad_a : nullptr, o.wave, o.lds_pad != 0);
         }
         if (o.large) return plan_large(ctx, p, o);
         // Four-step lines default to E = 8 in F64 (twice the threads per tile), 16 in F32.
@@ -1005,6 +1011,7 @@ static std::string describe(const Plan& p) {
         case Kind::Direct:
             os << "direct " << rad(p.line) << " C=" << p.C;
             if (p.line.pt.wave) os << " wave";
+            if (!p.line.pad) os << " nopad";
             if (p.pipe_res) os << " pipe";
             break;
         case Kind::FourStep:
@@ -1154,6 +1161,7 @@ static int run_fft_s(const Plan& p, const void* in, void* out, int sgn, double s
             a.sgn_out = sgn;
             a.scale = scale;
             a.epi = epi;
+            a.nopad = p.line.pad ? 0 : 1;
             if (p.half) {
                 CFFT_HIP(launch_rows_f16(in, out, a, p.line.pt, p.line.dtw.p, st));
                 return CFFT_OK;

@@ -31,6 +31,7 @@ struct DevBuf {
 // One line FFT of length N: radix sequence, threads per line, LDS line stride, twiddles.
 struct LinePlan {
     int N = 0, E = 16, Tt = 1, Ls = 1;
+    int pad = 1;  // LDS lines with pad words (0: unpadded, direct lines on the plain rows kernel)
     PassTable pt{};
     DevBuf dtw;
 };
@@ -148,6 +149,9 @@ struct PlanOptions {
     // kernels (bit 0: pass A, bit 1: pass B; -1 = both when pipe > 0, else neither)
     int Eb = 0;
     int pipe_mask = -1;
+    // Direct lines: 0 = LDS lines without pad words (more workgroups per CU where LDS bounds
+    // them), 1 / -1 = padded (the default)
+    int lds_pad = -1;
     bool explicit_opts = false;
     bool alloc_work = true;    // large convolutions: allocate work buffers for `batch` lines
 };
@@ -207,7 +211,7 @@ int64_t next_smooth(int64_t m);
 std::vector<int> line_radices(int64_t n);
 std::vector<int> fourstep_line_radices(int64_t n);
 int threads_for(const std::vector<int>& rad, int64_t N, int E);
-int line_stride_for(Prec p, int64_t N);
+int line_stride_for(Prec p, int64_t N, bool pad = true);
 int make_line(LinePlan& lp, int64_t N, Prec pr, int E, int max_threads,
               const std::vector<int>* radices = nullptr);
 // Whether rad is a radix sequence the kernels implement whose product is N.

@@ -252,7 +252,8 @@ def test_emulated_wisdom_round_trip(E, tmp_path):
     "4096 2 4096 4096 0 1 : 16 64 0 0 0 -1 3 0",    # pipe out of range
     "4096 2 4096 4096 0 1 : 16 64 0 0 0 -1 0 2",    # large out of range
     "4096 2 4096 4096 0 1 : 16 64 0 0 0 -1 0 0 pm=4",   # pipelined-pass mask out of range
-    "4096 2 4096 4096 0 1 : 16 64 0 0 0 -1 0 0 eb=12"])  # pass B E not built
+    "4096 2 4096 4096 0 1 : 16 64 0 0 0 -1 0 0 eb=12",  # pass B E not built
+    "4096 2 4096 4096 0 1 : 16 64 0 0 0 -1 0 0 pd=2"])   # LDS padding flag out of range
 def test_emulated_wisdom_rejects_out_of_range_knobs(E, tmp_path, line):
     """A wisdom line whose knobs no kernel instance implements is refused (the import fails and
     the key keeps its tuned or default plan), instead of building a plan that skips butterflies."""
@@ -307,6 +308,50 @@ def test_emulated_pipelined_match_plain(E, tmp_path, n, batch, dt):
         assert lib.cfft_exec(p, z.ctypes.data, z.ctypes.data, -1, 0, None) == 0
         assert np.array_equal(z, res[0])
         outs[pipe] = res
+        lib.cfft_plan_destroy(p)
+        lib.cfft_finalize(c2)
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("n,batch,dt,fields", [
+    (2520, 5, np.complex128, "16 64 0 0 0 -1 0 0"),
+    (2520, 9, np.complex64, "8 512 0 0 0 -1 0 0"),
+    (4096, 3, np.complex128, "8 64 0 0 0 -1 0 0"),
+    (1024, 7, np.complex64, "16 64 0 0 0 -1 0 0 wv=1"),
+    (3 ** 7, 4, np.complex64, "16 128 0 0 0 -1 0 0"),
+    (12, 40, np.complex128, "8 64 0 0 0 -1 0 0")])
+def test_emulated_unpadded_lds_lines(E, tmp_path, n, batch, dt, fields):
+    """Direct lines whose LDS lines carry no pad words (wisdom `pd=0`): the same bits as the
+    padded plan, the oracle both ways, in place, and the knob kept through a wisdom export."""
+    lib, ctx = E
+    dcode = 0 if dt == np.complex64 else 1
+    key = f"{n} {batch} {n} {n} {dcode} 1"
+    x = seeded_complex(n, batch=batch, seed=8, dtype=dt)
+    outs = {}
+    for pd in (1, 0):
+        path = tmp_path / f"pd{pd}.txt"
+        path.write_text(f"{key} : {fields} pd={pd}\n")
+        c2 = ctypes.c_void_p()
+        assert lib.cfft_init(0, ctypes.byref(c2)) == 0
+        assert lib.cfft_wisdom_import(c2, str(path).encode()) == 0
+        p = _plan(lib, c2, n, batch, dt, flags=1)
+        d = _desc(lib, p)
+        assert "direct" in d and ("nopad" in d) == (pd == 0), d
+        res = []
+        for direction, norm in ((-1, 0), (1, 1)):
+            y = np.zeros_like(x)
+            assert lib.cfft_exec(p, x.ctypes.data, y.ctypes.data, direction, norm, None) == 0
+            ref = O.fft(x.astype(np.complex128), "d", direction, bool(norm))
+            assert rel_l2(y, ref) < _tol(n, dt), (pd, direction)
+            res.append(y)
+        z = x.copy()
+        assert lib.cfft_exec(p, z.ctypes.data, z.ctypes.data, -1, 0, None) == 0
+        assert np.array_equal(z, res[0])
+        out = tmp_path / f"pd{pd}_out.txt"
+        assert lib.cfft_wisdom_export(c2, str(out).encode()) == 0
+        assert f" pd={pd}" in out.read_text()
+        outs[pd] = res
         lib.cfft_plan_destroy(p)
         lib.cfft_finalize(c2)
     for a, b in zip(outs[0], outs[1]):

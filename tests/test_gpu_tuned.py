@@ -18,7 +18,8 @@ pytestmark = pytest.mark.gpu
 
 # wisdom fields after the key: E target_threads n1 ca cb chunk pipe large [ra=...] [rb=...]
 # (0 = the planner's default for n1 / ca / cb, chunk -1 = default; ra / rb = radix sequences;
-# pm = the four-step passes on the pipelined kernels, eb = pass B's E, pr = line-pair passes)
+# pm = the four-step passes on the pipelined kernels, eb = pass B's E, pr = line-pair passes,
+# pd = 0: direct lines without LDS pad words)
 FORCED = [
     # n, batch, dtype, fields, what the description must show
     (1 << 16, 6, np.complex64, "16 64 256 2 32 -1 0 0", ["N1=256", "CA=2", "CB=32"]),
@@ -33,6 +34,10 @@ FORCED = [
     (2520, 7, np.complex128, "8 128 40 16 16 -1 0 0", ["N1=40", "CA=16", "CB=16"]),
     (2520, 7, np.complex64, "16 64 0 0 0 -1 1 0", ["direct", "pipe"]),
     (1000, 9, np.complex64, "8 512 0 0 0 -1 0 0", ["direct"]),
+    # direct lines with unpadded LDS lines (pd=0)
+    (2520, 4096, np.complex128, "16 64 0 0 0 -1 0 0 ra=8,7,5,3,3 pd=0", ["Tt=180", "nopad"]),
+    (1024, 40, np.complex64, "16 64 0 0 0 -1 0 0 wv=1 pd=0", ["wave", "nopad"]),
+    (4096, 16, np.complex128, "8 64 0 0 0 -1 0 0 pd=0", ["direct", "nopad"]),
     # radix sequences other than the default (the reference's MEASURE search space)
     (2520, 64, np.complex64, "8 64 0 0 0 -1 0 0 ra=5,7,8,9", ["direct [5,7,8,9]"]),
     (2520, 64, np.complex128, "8 64 0 0 0 -1 0 0 ra=8,9,7,5", ["direct [8,9,7,5]"]),
@@ -138,3 +143,27 @@ def test_tuned_fourstep_refined_matches_oracle_and_round_trips(gpu_ready, tmp_pa
     y2 = run_dev(lambda xd: p2.exec_torch(xd), x)
     word = np.uint32 if dt == np.complex64 else np.uint64
     assert np.array_equal(y1.view(word), y2.view(word))
+
+
+@pytest.mark.parametrize("n,batch,dt,fields", [
+    (2520, 64, np.complex128, "16 64 0 0 0 -1 0 0 ra=8,7,5,3,3"),
+    (1024, 40, np.complex64, "16 64 0 0 0 -1 0 0 wv=1"),
+    (4096, 16, np.complex128, "8 64 0 0 0 -1 0 0")])
+def test_unpadded_lds_lines_give_the_padded_bits(gpu_ready, tmp_path, n, batch, dt, fields):
+    """The LDS layout of a line moves no arithmetic: the plan with pd=0 gives the padded plan's
+    bits on the GPU."""
+    import catabrafft as C
+
+    code = 0 if dt == np.complex64 else 1
+    x = seeded_complex(n, batch, seed=n % 91 + 2, dtype=dt)
+    ys = {}
+    for pd in (1, 0):
+        w = tmp_path / f"w{pd}.txt"
+        w.write_text(f"{n} {batch} {n} {n} {code} 1 : {fields} pd={pd}\n")
+        C.import_wisdom(str(w))
+        p = C.Plan(n, batch, dt, flags=C.MEASURE)
+        assert ("nopad" in p.describe()) == (pd == 0), p.describe()
+        ys[pd] = run_dev(lambda xd: p.exec_torch(xd), x)
+    word = np.uint32 if dt == np.complex64 else np.uint64
+    assert np.array_equal(ys[0].view(word), ys[1].view(word))
+    assert rel_l2(ys[0], oracle_ref(x)) <= tol(n, dt)

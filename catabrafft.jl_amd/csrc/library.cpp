@@ -551,7 +551,8 @@ static int plan_fft_core(Ctx* ctx, Plan& p, const PlanOptions& o);
 static int plan_fourstep_factors(Ctx* ctx, FourStepPlan& fs, int64_t n, Prec pr, int E,
                                  int64_t n1_force, int ca, int cb, int64_t chunk,
                                  const std::vector<int>* rad_a = nullptr,
-                                 const std::vector<int>* rad_b = nullptr, int Eb = 0) {
+                                 const std::vector<int>* rad_b = nullptr, int Eb = 0,
+                                 int min_threads = 0) {
     fs.chunk = chunk;
     const int E2 = Eb > 0 ? Eb : E;  // pass B's capacity
     int64_t best1 = 0;
@@ -585,8 +586,13 @@ static int plan_fourstep_factors(Ctx* ctx, FourStepPlan& fs, int64_t n, Prec pr,
     // Narrow tiles (4 columns; 8 in F32 pass B) let several workgroups share a CU, so one
     // loads while another computes; consecutive tiles run on one XCD (xcd_block), which shares
     // the partly covered 128-byte lines in its L2.  Measured best on MI355X for n = 2^20.
-    fs.CA = ca > 0 ? ca : pick_tile(ctx, fs.a, pr, 4);
-    fs.CB = cb > 0 ? cb : pick_tile(ctx, fs.b, pr, pr == Prec::F32 ? 8 : 4, table);
+    // (min_threads: a default tile grows until its workgroup has that many threads)
+    auto want = [min_threads](int c, const LinePlan& lp) {
+        while (c * lp.Tt < min_threads && 2 * c * lp.Tt <= 1024) c *= 2;
+        return c;
+    };
+    fs.CA = ca > 0 ? ca : pick_tile(ctx, fs.a, pr, want(4, fs.a));
+    fs.CB = cb > 0 ? cb : pick_tile(ctx, fs.b, pr, want(pr == Prec::F32 ? 8 : 4, fs.b), table);
     fs.a.Ls = fs_line_stride(ctx, pr, fs.a, fs.CA, 16);
     fs.b.Ls = fs_line_stride(ctx, pr, fs.b, fs.CB, table + 16);
     // a requested tile must be a power of two that fits 1024 threads and the LDS
@@ -827,6 +833,9 @@ static int plan_conv(Ctx* ctx, Plan& p, const PlanOptions& o) {
     // the fused convolution kernels have no line-pair variant: the FFT_L keeps the float
     // kernels' default lines (E = 16 in F32)
     if (so.pair < 0) so.pair = 0;
+    // and its own tiles: widening them (as an uneven batch does for its four-step plans) made
+    // the uneven batch's large convolutions slower (9.10 -> 9.34 ms)
+    so.min_threads = 0;
     rc = plan_fft_core(ctx, s, so);
     if (rc) return rc;
     if (s.kind != Kind::Direct && s.kind != Kind::FourStep && s.kind != Kind::Large)
@@ -956,7 +965,7 @@ static int plan_fft_core(Ctx* ctx, Plan& p, const PlanOptions& o) {
         int rc = CFFT_ERR_UNSUPPORTED;
         if (!o.E && p.prec == Prec::F64)
             rc = plan_fourstep_factors(ctx, p.fs, p.n, p.prec, 8, n1, o.ca, o.cb, chunk, ra, rb,
-                                       o.Eb);
+                                       o.Eb, o.min_threads);
         // ComplexF32 lines of even length on both sides default to E = 8 with both passes on
         // line pairs (MEASURE's choice for every power of two 2^16 .. 2^24 and 4 5^8; E = 8
         // without pairs is slower than 16).  ComplexF16 plans run the float kernels: E = 16.
@@ -964,12 +973,12 @@ static int plan_fft_core(Ctx* ctx, Plan& p, const PlanOptions& o) {
                                o.pair != 0 && p.n % 4 == 0;
         if (f32_pairs) {
             rc = plan_fourstep_factors(ctx, p.fs, p.n, p.prec, 8, n1, o.ca, o.cb, chunk, ra, rb,
-                                       o.Eb);
+                                       o.Eb, o.min_threads);
             if (rc == CFFT_OK && (p.fs.N1 % 2 || p.fs.N2 % 2)) rc = CFFT_ERR_UNSUPPORTED;
         }
         if (rc)
             rc = plan_fourstep_factors(ctx, p.fs, p.n, p.prec, E, n1, o.ca, o.cb, chunk, ra, rb,
-                                       o.Eb);
+                                       o.Eb, o.min_threads);
         if (rc && n1 == 0) {  // too long for the four-step: six steps
             p.fs.b.dtw.reset();
             p.fs.tw3.reset();

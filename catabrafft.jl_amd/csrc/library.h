@@ -152,6 +152,9 @@ struct PlanOptions {
     // Direct lines: 0 = LDS lines without pad words (more workgroups per CU where LDS bounds
     // them), 1 / -1 = padded (the default)
     int lds_pad = -1;
+    // Four-step tiles of the plan's own transform (not of a convolution's FFT_L): widened from
+    // the default until a workgroup has this many threads (0 = the default tiles)
+    int min_threads = 0;
     bool explicit_opts = false;
     bool alloc_work = true;    // large convolutions: allocate work buffers for `batch` lines
 };

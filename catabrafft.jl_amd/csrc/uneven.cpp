@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -24,6 +25,7 @@
 #include <mutex>
 #include <numeric>
 #include <sstream>
+#include <thread>
 
 #include "catabrafft.h"
 
@@ -622,13 +624,22 @@ int uneven_create(Ctx* ctx, int64_t count, const int64_t* lengths, const int64_t
     };
     std::vector<Pending> pend;
     int ndirect = 0, nfour = 0, nconvs = 0, nconvl = 0;
-    for (auto& kv : bylen) {
-        const int64_t n = kv.first;
-        const int64_t m = (int64_t)kv.second.size();
+    // The plans are built on several host threads: their twiddle, chirp and permutation tables
+    // are computed on the host in long double, which dominates the planning of thousands of
+    // lengths.  Plan building shares only mutex-guarded state (plan and line-stride caches);
+    // each thread makes the context's device current first.
+    std::vector<std::pair<int64_t, std::vector<int64_t>>> lens(bylen.begin(), bylen.end());
+    std::vector<std::unique_ptr<Plan>> built(lens.size());
+    std::vector<int> codes(lens.size(), CFFT_OK);
+    std::vector<std::string> msgs(lens.size());
+    std::atomic<size_t> next{0};
+    std::atomic<int> setup_rc{CFFT_OK};
+    auto build_one = [&](size_t k) {
+        const int64_t n = lens[k].first;
         std::unique_ptr<Plan> p(new Plan());
         p->ctx = ctx;
         p->n = n;
-        p->batch = m;
+        p->batch = (int64_t)lens[k].second.size();
         p->idist = p->odist = n;
         p->prec = pr;
         p->dtype = dtype;
@@ -649,6 +660,35 @@ int uneven_create(Ctx* ctx, int64_t count, const int64_t* lengths, const int64_t
                 return fail(CFFT_ERR_UNSUPPORTED, "uneven batches take lengths the four-step "
                                                   "covers (n=" + std::to_string(n) + " needs six steps)");
         }
+        built[k] = std::move(p);
+        return (int)CFFT_OK;
+    };
+    auto worker = [&]() {
+        if (hipSetDevice(ctx->device) != hipSuccess) {
+            setup_rc = CFFT_ERR_HIP;
+            return;
+        }
+        for (size_t k; (k = next++) < lens.size();) {
+            codes[k] = build_one(k);
+            if (codes[k]) msgs[k] = cfft_last_error();
+        }
+    };
+    {
+        const unsigned hw = std::thread::hardware_concurrency();
+        const size_t nthr = std::min<size_t>(lens.size(), std::max(1u, std::min(8u, hw)));
+        std::vector<std::thread> pool;
+        for (size_t t = 1; t < nthr; ++t) pool.emplace_back(worker);
+        worker();
+        for (auto& th : pool) th.join();
+    }
+    if (setup_rc != CFFT_OK) return fail(setup_rc, "hipSetDevice failed on a planning thread");
+    for (size_t k = 0; k < lens.size(); ++k)
+        if (codes[k]) return fail(codes[k], msgs[k]);
+    for (size_t k = 0; k < lens.size(); ++k) {
+        auto& kv = lens[k];
+        const int64_t n = kv.first;
+        const int64_t m = (int64_t)kv.second.size();
+        std::unique_ptr<Plan> p = std::move(built[k]);
         Pending q;
         q.ids = kv.second;
         q.scratch_off = q.conv_off = 0;

@@ -86,15 +86,17 @@ def build(force: bool = False, jobs: int = 8) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"build failed: {' '.join(cmd)}\n{r.stderr}")
     os.replace(loop + ".tmp", loop)
-    # the access-pattern probe of the four-step passes (a standalone program, no library)
-    probe = os.path.join(LIBDIR, "pattern_copy")
-    cmd = [HIPCC, "--offload-arch=gfx950", "-O3", "-Xclang", "-target-feature", "-Xclang",
-           "-packed-fp32-ops", "-x", "hip", os.path.join(HERE, "bench", "pattern_copy.cpp"),
-           "-o", probe + ".tmp"]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"build failed: {' '.join(cmd)}\n{r.stderr}")
-    os.replace(probe + ".tmp", probe)
+    # standalone measurement programs (no library): the access patterns of the four-step passes,
+    # and a radix-16 stage on the matrix cores against the codelet
+    for name in ("pattern_copy", "mfma_dft16"):
+        probe = os.path.join(LIBDIR, name)
+        cmd = [HIPCC, "--offload-arch=gfx950", "-O3", "-Xclang", "-target-feature", "-Xclang",
+               "-packed-fp32-ops", "-I", CSRC, "-x", "hip",
+               os.path.join(HERE, "bench", name + ".cpp"), "-o", probe + ".tmp"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"build failed: {' '.join(cmd)}\n{r.stderr}")
+        os.replace(probe + ".tmp", probe)
     return LIB
 
 

@@ -25,6 +25,7 @@
 #include <mutex>
 #include <numeric>
 #include <sstream>
+#include <system_error>
 #include <thread>
 
 #include "catabrafft.h"
@@ -677,7 +678,13 @@ int uneven_create(Ctx* ctx, int64_t count, const int64_t* lengths, const int64_t
         const unsigned hw = std::thread::hardware_concurrency();
         const size_t nthr = std::min<size_t>(lens.size(), std::max(1u, std::min(8u, hw)));
         std::vector<std::thread> pool;
-        for (size_t t = 1; t < nthr; ++t) pool.emplace_back(worker);
+        for (size_t t = 1; t < nthr; ++t) {
+            try {
+                pool.emplace_back(worker);
+            } catch (const std::system_error&) {
+                break;  // no more threads: the ones running (and this one) take the rest
+            }
+        }
         worker();
         for (auto& th : pool) th.join();
     }

@@ -552,20 +552,23 @@ static int plan_fourstep_factors(Ctx* ctx, FourStepPlan& fs, int64_t n, Prec pr,
                                  int64_t n1_force, int ca, int cb, int64_t chunk,
                                  const std::vector<int>* rad_a = nullptr,
                                  const std::vector<int>* rad_b = nullptr, int Eb = 0,
-                                 int min_threads = 0) {
+                                 int min_threads = 0, double ratio = 1.0, int cb_want = 0) {
     fs.chunk = chunk;
     const int E2 = Eb > 0 ? Eb : E;  // pass B's capacity
     int64_t best1 = 0;
     if (n1_force > 0) {
         best1 = n1_force;
     } else {
-        // balanced split with both factors fitting one workgroup
+        // the split closest to N1 / N2 = ratio (1: balanced) with both factors fitting one
+        // workgroup
         double bestscore = 1e300;
+        const double lr = std::log(ratio);
         for (int64_t a = 2; a * a <= n * 64 && a < n; ++a) {
             if (n % a) continue;
             const int64_t b = n / a;
             if (!line_fits(ctx, a, pr, E) || !line_fits(ctx, b, pr, E2)) continue;
-            const double score = std::fabs(std::log((double)a / (double)b)) + (a < b ? 0.01 : 0.0);
+            const double score =
+                std::fabs(std::log((double)a / (double)b) - lr) + (a < b ? 0.01 : 0.0);
             if (score < bestscore) {
                 bestscore = score;
                 best1 = a;
@@ -592,7 +595,9 @@ static int plan_fourstep_factors(Ctx* ctx, FourStepPlan& fs, int64_t n, Prec pr,
         return c;
     };
     fs.CA = ca > 0 ? ca : pick_tile(ctx, fs.a, pr, want(4, fs.a));
-    fs.CB = cb > 0 ? cb : pick_tile(ctx, fs.b, pr, want(pr == Prec::F32 ? 8 : 4, fs.b), table);
+    fs.CB = cb > 0 ? cb
+                   : pick_tile(ctx, fs.b, pr,
+                               want(cb_want > 0 ? cb_want : (pr == Prec::F32 ? 8 : 4), fs.b), table);
     fs.a.Ls = fs_line_stride(ctx, pr, fs.a, fs.CA, 16);
     fs.b.Ls = fs_line_stride(ctx, pr, fs.b, fs.CB, table + 16);
     // a requested tile must be a power of two that fits 1024 threads and the LDS
@@ -926,6 +931,11 @@ static void fs_pair_setup(Ctx* ctx, Plan& p, int mask) {
     }
 }
 
+// ComplexF32 four-step plans on line pairs of power-of-two lengths up to 2^22, by default:
+// N1 / N2 near this ratio, and pass-B tiles of this many lines (where they fit).
+constexpr double kPairSplitRatio = 4.0;
+constexpr int kPairTileB = 16;
+
 static int plan_fft_core(Ctx* ctx, Plan& p, const PlanOptions& o) {
     if (p.n == 1) {
         p.kind = Kind::Trivial;
@@ -973,16 +983,25 @@ static int plan_fft_core(Ctx* ctx, Plan& p, const PlanOptions& o) {
         // ComplexF32 lines of even length on both sides default to E = 8 with both passes on
         // line pairs (MEASURE's choice for every power of two 2^16 .. 2^24 and 4 5^8; E = 8
         // without pairs is slower than 16).  ComplexF16 plans run the float kernels: E = 16.
+        // ComplexF32 (and ComplexF16) powers of two up to 2^22 split N1 = kPairSplitRatio N2, and
+        // on line pairs take pass-B tiles of kPairTileB lines: MEASURE's choice there (NO_FLAG on
+        // MI355X, cold, 512 MiB: 2^15 / 2^16 / 2^17 / 2^20 / 2^22 9 / 13 / 6 / 11 / 3 % faster
+        // than balanced, 2^14 and 2^18 equal; 2^24 36 % and 10^5, 10^6, 4 5^8, 3 2^19 3-11 %
+        // slower, so they stay balanced).  Every F32 plan of such a length takes the same split,
+        // so a transform keeps its bits alone, in a work list or as ComplexF16.
+        const bool skew = p.prec == Prec::F32 && (p.n & (p.n - 1)) == 0 &&
+                          p.n <= ((int64_t)1 << 22);
+        const double ratio = skew ? kPairSplitRatio : 1.0;
         const bool f32_pairs = !o.E && p.prec == Prec::F32 && p.dtype != CFFT_C16 &&
                                o.pair != 0 && p.n % 4 == 0;
         if (f32_pairs) {
             rc = plan_fourstep_factors(ctx, p.fs, p.n, p.prec, 8, n1, o.ca, o.cb, chunk, ra, rb,
-                                       o.Eb, min_thr);
+                                       o.Eb, min_thr, ratio, skew ? kPairTileB : 0);
             if (rc == CFFT_OK && (p.fs.N1 % 2 || p.fs.N2 % 2)) rc = CFFT_ERR_UNSUPPORTED;
         }
         if (rc)
             rc = plan_fourstep_factors(ctx, p.fs, p.n, p.prec, E, n1, o.ca, o.cb, chunk, ra, rb,
-                                       o.Eb, min_thr);
+                                       o.Eb, min_thr, ratio);
         if (rc && n1 == 0) {  // too long for the four-step: six steps
             p.fs.b.dtw.reset();
             p.fs.tw3.reset();

@@ -931,10 +931,10 @@ static void fs_pair_setup(Ctx* ctx, Plan& p, int mask) {
     }
 }
 
-// ComplexF32 four-step plans on line pairs of power-of-two lengths up to 2^22, by default:
-// N1 / N2 near this ratio, and pass-B tiles of this many lines (where they fit).
-constexpr double kPairSplitRatio = 4.0;
-constexpr int kPairTileB = 16;
+// Four-step plans of short power-of-two lengths (see plan_fft_core): N1 / N2 near this ratio,
+// and by default pass-B tiles of this many lines (where they fit).
+constexpr double kSkewSplitRatio = 4.0;
+constexpr int kSkewTileB = 16;
 
 static int plan_fft_core(Ctx* ctx, Plan& p, const PlanOptions& o) {
     if (p.n == 1) {
@@ -976,27 +976,29 @@ static int plan_fft_core(Ctx* ctx, Plan& p, const PlanOptions& o) {
         const int min_thr = std::max(o.min_threads, 0);
         const std::vector<int>* ra = o.rad_a.empty() ? nullptr : &o.rad_a;
         const std::vector<int>* rb = o.rad_b.empty() ? nullptr : &o.rad_b;
+        // Powers of two up to 2^22 in ComplexF32 (and ComplexF16), up to 2^17 in ComplexF64, split
+        // N1 = kSkewSplitRatio N2, and their default F64 and line-pair plans take pass-B tiles of
+        // kSkewTileB lines: MEASURE's choice there.  NO_FLAG on MI355X, cold, 512 MiB against the
+        // balanced split: F32 2^15 / 2^16 / 2^17 / 2^20 / 2^22 9 / 13 / 6 / 11 / 3 % faster (2^14,
+        // 2^18 equal; 2^24 36 % and 10^5, 10^6, 4 5^8, 3 2^19 3-11 % slower); F64 2^14 .. 2^17
+        // 8-12 % faster (2^18 .. 2^21 and 2^24 2-9 % slower).  Every plan of such a length and
+        // precision takes the same split, so a transform keeps its bits alone, in a work list or
+        // as ComplexF16.
+        const bool skew = (p.n & (p.n - 1)) == 0 &&
+                          p.n <= ((int64_t)1 << (p.prec == Prec::F32 ? 22 : 17));
+        const double ratio = skew ? kSkewSplitRatio : 1.0;
         int rc = CFFT_ERR_UNSUPPORTED;
         if (!o.E && p.prec == Prec::F64)
             rc = plan_fourstep_factors(ctx, p.fs, p.n, p.prec, 8, n1, o.ca, o.cb, chunk, ra, rb,
-                                       o.Eb, min_thr);
+                                       o.Eb, min_thr, ratio, skew ? kSkewTileB : 0);
         // ComplexF32 lines of even length on both sides default to E = 8 with both passes on
         // line pairs (MEASURE's choice for every power of two 2^16 .. 2^24 and 4 5^8; E = 8
         // without pairs is slower than 16).  ComplexF16 plans run the float kernels: E = 16.
-        // ComplexF32 (and ComplexF16) powers of two up to 2^22 split N1 = kPairSplitRatio N2, and
-        // on line pairs take pass-B tiles of kPairTileB lines: MEASURE's choice there (NO_FLAG on
-        // MI355X, cold, 512 MiB: 2^15 / 2^16 / 2^17 / 2^20 / 2^22 9 / 13 / 6 / 11 / 3 % faster
-        // than balanced, 2^14 and 2^18 equal; 2^24 36 % and 10^5, 10^6, 4 5^8, 3 2^19 3-11 %
-        // slower, so they stay balanced).  Every F32 plan of such a length takes the same split,
-        // so a transform keeps its bits alone, in a work list or as ComplexF16.
-        const bool skew = p.prec == Prec::F32 && (p.n & (p.n - 1)) == 0 &&
-                          p.n <= ((int64_t)1 << 22);
-        const double ratio = skew ? kPairSplitRatio : 1.0;
         const bool f32_pairs = !o.E && p.prec == Prec::F32 && p.dtype != CFFT_C16 &&
                                o.pair != 0 && p.n % 4 == 0;
         if (f32_pairs) {
             rc = plan_fourstep_factors(ctx, p.fs, p.n, p.prec, 8, n1, o.ca, o.cb, chunk, ra, rb,
-                                       o.Eb, min_thr, ratio, skew ? kPairTileB : 0);
+                                       o.Eb, min_thr, ratio, skew ? kSkewTileB : 0);
             if (rc == CFFT_OK && (p.fs.N1 % 2 || p.fs.N2 % 2)) rc = CFFT_ERR_UNSUPPORTED;
         }
         if (rc)
